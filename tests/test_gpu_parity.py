@@ -1,0 +1,148 @@
+"""GPU parity: the HIP evaluator (through the C-ABI) against the CPU oracle and the golden fixtures.
+
+The bar is bit-exactness: fitness values (float32 of the FP64 score), placement counts and therefore
+every move of every game must equal the oracle's (the reference's own acceptance test demands
+device == host exactly, ta3_gpu/tests/src/parity_test.cpp:51-101).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tetris_ai3_amd as T
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ev():
+    e = T.ModelEvaluator()
+    yield e
+    e.close()
+
+
+def test_device_bits_canary():
+    say("case: bits canary")
+    assert T.device_bits_canary()
+
+
+def test_feature_norms_match_host_ieee():
+    say("case: feature norms 0..4761")
+    n = 4762
+    got = T.feature_norms(n)
+    i = np.arange(n, dtype=np.float32)
+    sq = np.where(i > 0, np.sqrt(i), np.float32(0)).astype(np.float32)
+    want = np.stack([sq / np.float32(100), i / np.float32(20), i / np.float32(200), i / np.float32(5)], axis=1)
+    assert got.astype(np.float32).tobytes() == want.astype(np.float32).tobytes()
+
+
+def test_golden_parity_cases(ev):
+    with open(os.path.join(HERE, "golden", "parity_cases.json")) as f:
+        cases = json.load(f)["cases"]
+    for ci, c in enumerate(cases):
+        say(f"case: golden {ci} kind={c['kind']} seeds={c['seeds']} cap={c['max_moves']}")
+        e = T.ModelEvaluator(model=c["kind"])
+        w = np.array(c["weights"], dtype=np.float32)
+        got = e.eval(w, 1, c["seeds"], c["max_moves"])
+        assert got.tobytes() == np.array(c["fitness"], dtype=np.float32).tobytes(), (ci, got, c["fitness"])
+        e.close()
+
+
+@pytest.mark.parametrize("kind", [T.MODEL_V11, T.MODEL_F16])
+def test_single_model_matches_oracle(kind):
+    """parity_test.cpp:206-221: three weight vectors x seeds {1,2,3,4} x 200 moves, device == host exactly"""
+    e = T.ModelEvaluator(model=kind)
+    P = T.num_params(kind)
+    seeds = [1, 2, 3, 4]
+    for ms in (11, 22, 33):
+        say(f"case: single kind={kind} modelSeed={ms}")
+        rng = np.random.default_rng(ms)
+        w = (rng.uniform(-2, 2, P) if kind == T.MODEL_V11 else rng.normal(0, 0.5, P)).astype(np.float32)
+        dev = e.eval(w, seeds, 200)
+        host = O.eval_host(kind, w, seeds, 200)
+        assert dev.tobytes() == host.tobytes(), (ms, dev, host)
+    e.close()
+
+
+def test_multi_model_batch_matches_single_and_host(ev):
+    """parity_test.cpp:225-255"""
+    rng = np.random.default_rng(100)
+    ws = [rng.uniform(-2, 2, 97).astype(np.float32) for _ in range(3)]
+    seeds = [1, 2, 3, 4]
+    say("case: batched 3 models x 4 seeds x 200")
+    batched = ev.eval(np.concatenate(ws), 3, seeds, 200)
+    for m in range(3):
+        single = ev.eval(ws[m], seeds, 200)
+        assert batched[m * 4:(m + 1) * 4].tobytes() == single.tobytes()
+        host = O.eval_host(O.MODEL_V11, ws[m], seeds, 200)
+        assert single.tobytes() == host.tobytes()
+
+
+def test_champion_games_and_placements(ev):
+    with open(os.path.join(HERE, "golden", "v11_champion.json")) as f:
+        w = np.array(json.load(f)["weights_f64"], dtype=np.float64).astype(np.float32)
+    seeds = O.expand_seeds(0x3E28DF7B1811145B, 8)
+    say("case: v11 champion 8 games x 300 moves")
+    ev.stage(w, 1, seeds, 300)
+    ev.launch()
+    ev.sync()
+    fit, pl = ev.fetch()
+    host, host_pl = O.eval_host_mt(O.MODEL_V11, w, 1, seeds, 300)
+    assert fit.tobytes() == host.tobytes()
+    assert (pl == host_pl).all()
+    assert fit.mean() > 285.0
+
+
+def test_edge_cases(ev):
+    w = np.random.default_rng(1).uniform(-2, 2, 97).astype(np.float32)
+    say("case: max_moves=0")
+    assert list(ev.eval(w, [1, 2, 3], 0)) == [-1e6] * 3
+    say("case: max_moves=1")
+    assert ev.eval(w, [5, 6], 1).tobytes() == O.eval_host(O.MODEL_V11, w, [5, 6], 1).tobytes()
+    say("case: one game, many models")
+    ws = np.random.default_rng(2).uniform(-2, 2, (70, 97)).astype(np.float32)
+    got = ev.eval(ws.reshape(-1), 70, [9], 40)
+    want, _ = O.eval_host_mt(O.MODEL_V11, ws.reshape(-1), 70, [9], 40)
+    assert got.tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("wpb,persistent,grid", [(1, False, 0), (2, True, 0), (4, True, 3), (4, False, 0), (1, True, 1)])
+def test_launch_knobs_do_not_change_results(wpb, persistent, grid):
+    rng = np.random.default_rng(7)
+    ws = rng.normal(0, 0.5, (6, 97)).astype(np.float32)
+    seeds = O.expand_seeds(3, 5)
+    e = T.ModelEvaluator()
+    e.set_launch(wpb, persistent, grid)
+    say(f"case: launch wpb={wpb} persistent={persistent} grid={grid}")
+    got = e.eval(ws.reshape(-1), 6, seeds, 120)
+    want, _ = O.eval_host_mt(O.MODEL_V11, ws.reshape(-1), 6, seeds, 120)
+    assert got.tobytes() == want.tobytes()
+    e.close()
+
+
+def test_config0_cpu_runnable_case_bit_exact():
+    """4 models x 8 games, cap 1000, weights N(0, 0.5^2), seeds from simulate(seed=1): both networks"""
+    for kind in (T.MODEL_V11, T.MODEL_F16):
+        P = T.num_params(kind)
+        rng = np.random.Generator(np.random.PCG64(0x7E7215))
+        w = rng.normal(0.0, 0.5, 4 * P).astype(np.float32)
+        seeds = T.expand_seeds(1, 8)
+        e = T.ModelEvaluator(model=kind)
+        say(f"case: config0 kind={kind}")
+        e.stage(w, 4, seeds, 1000)
+        e.launch()
+        e.sync()
+        fit, pl = e.fetch()
+        host, host_pl = O.eval_host_mt(kind, w, 4, seeds, 1000)
+        assert fit.tobytes() == host.tobytes()
+        assert (pl == host_pl).all()
+        e.close()

@@ -1,0 +1,59 @@
+"""CPU-side checks of the product library: it loads, exports every entry point include/ta3h.h declares,
+its compile-time tables equal the oracle's, and argument errors are reported without touching a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tetris_ai3_amd as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names += re.findall(r"^\s*(?:const\s+)?\w+[\s\*]+(ta3h_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_entry_point():
+    names = declared_symbols()
+    assert len(names) >= 15
+    L = T.lib()
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_tables_match_oracle():
+    lut, trees = T.tables()
+    for p in range(7):
+        assert lut[p] == O.lut(p)
+    for he in (0, 1):
+        assert trees[he] == O.tree(4, bool(he))
+
+
+def test_seed_expansion_matches_oracle():
+    for seed in (0, 1, 0x3E28DF7B1811145B, 2**64 - 1):
+        assert list(T.expand_seeds(seed, 7)) == list(O.expand_seeds(seed, 7))
+
+
+def test_num_params():
+    assert T.num_params(T.MODEL_V11) == 97 and T.num_params(T.MODEL_F16) == 1633
+    assert T.lib().ta3h_num_params(99) == -1
+
+
+def test_argument_errors_without_gpu():
+    ev = T.ModelEvaluator()
+    with pytest.raises(T.HipError, match="NUM_PARAMS"):
+        ev.eval(np.zeros(96, dtype=np.float32), 1, [1], 10)
+    # zero (model, game) items: an empty result, no launch (model_eval.cu:104-107)
+    assert ev.eval(np.zeros(97, dtype=np.float32), 1, np.zeros(0, dtype=np.uint64), 10).size == 0
+    with pytest.raises(T.HipError, match="waves_per_block"):
+        ev.set_launch(3)
+    with pytest.raises(T.HipError, match="unknown model"):
+        T.ModelEvaluator(model=7)

@@ -1,0 +1,230 @@
+"""MI355X (gfx950) beam-search evaluator: Python binding of the C-ABI in include/ta3h.h.
+
+Mirrors the reference's evaluator interface (ta3_gpu/incl/ta3/gpu/model_eval.hpp):
+  ModelEvaluator.eval(weights, num_models, seeds, max_moves) -> fitness, model-major (:46-51)
+  ModelEvaluator.eval(weights, seeds, max_moves)             -> single-model overload (:54-60)
+  simulate(evaluator, weights, num_models, games, seed, max_moves) (:84-114)
+All work runs in libta3h.so on the GPU; there is no CPU fallback. Loading fails loudly when the
+library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libta3h.so")
+
+MODEL_V11 = 0
+MODEL_F16 = 1
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    """A failure reported by the evaluator library ("hip: <what>: <message>" or an argument error)."""
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        f32p = C.POINTER(C.c_float)
+        u32p = C.POINTER(C.c_uint32)
+        u64p = C.POINTER(C.c_uint64)
+        sig = {
+            "ta3h_create": (C.c_int, [C.POINTER(vp)]),
+            "ta3h_destroy": (None, [vp]),
+            "ta3h_last_error": (C.c_char_p, [vp]),
+            "ta3h_num_params": (C.c_int, [C.c_int]),
+            "ta3h_set_model": (C.c_int, [vp, C.c_int]),
+            "ta3h_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint32]),
+            "ta3h_eval": (C.c_int, [vp, f32p, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32, f32p]),
+            "ta3h_stage": (C.c_int, [vp, f32p, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
+            "ta3h_launch": (C.c_int, [vp]),
+            "ta3h_sync": (C.c_int, [vp]),
+            "ta3h_fetch": (C.c_int, [vp, f32p, u32p]),
+            "ta3h_last_kernel_ms": (C.c_int, [vp, f32p]),
+            "ta3h_fetch_counters": (C.c_int, [vp, u64p]),
+            "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
+            "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
+            "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
+            "ta3h_tables": (None, [C.POINTER(C.c_int8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                   C.POINTER(C.c_uint8)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def num_params(kind: int = MODEL_V11) -> int:
+    return lib().ta3h_num_params(kind)
+
+
+class ModelEvaluator:
+    """One evaluation context (a HIP stream + grow-only device buffers); not thread-safe, like the
+    reference's ModelEvaluator (model_eval.hpp:21-26)."""
+
+    def __init__(self, model: int = MODEL_V11):
+        self._h = C.c_void_p()
+        if lib().ta3h_create(C.byref(self._h)) != 0:
+            raise HipError("ta3h_create failed")
+        self.model = model
+        self._check(lib().ta3h_set_model(self._h, model))
+
+    def close(self):
+        if self._h:
+            lib().ta3h_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise HipError(lib().ta3h_last_error(self._h).decode())
+
+    @property
+    def num_params(self) -> int:
+        return num_params(self.model)
+
+    def set_launch(self, waves_per_block: int = 4, persistent: bool = True, grid_blocks: int = 0):
+        self._check(lib().ta3h_set_launch(self._h, waves_per_block, int(persistent), grid_blocks))
+
+    def _args(self, weights, num_models, seeds):
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        s = np.ascontiguousarray(seeds, dtype=np.uint64).reshape(-1)
+        if num_models is None:
+            num_models = 1
+        P = self.num_params
+        if w.size != num_models * P:
+            raise HipError(f"eval: weights.size() must equal num_models * NUM_PARAMS ({num_models} * {P})")
+        return w, int(num_models), s
+
+    def eval(self, weights, num_models_or_seeds, seeds=None, max_moves=None):
+        """eval(weights, num_models, seeds, max_moves) or eval(weights, seeds, max_moves)."""
+        if max_moves is None:  # single-model overload (model_eval.hpp:54-60)
+            num_models, seeds, max_moves = 1, num_models_or_seeds, seeds
+        else:
+            num_models = num_models_or_seeds
+        w, m, s = self._args(weights, num_models, seeds)
+        out = np.zeros(m * s.size, dtype=np.float32)
+        self._check(lib().ta3h_eval(self._h, _ptr(w, C.c_float), m, self.num_params, _ptr(s, C.c_uint64), s.size,
+                                    int(max_moves), _ptr(out, C.c_float)))
+        return out
+
+    # --- device steps, for callers that keep inputs resident and time the device work ---------------
+    def stage(self, weights, num_models, seeds, max_moves):
+        w, m, s = self._args(weights, num_models, seeds)
+        self._items = m * s.size
+        self._check(lib().ta3h_stage(self._h, _ptr(w, C.c_float), m, self.num_params, _ptr(s, C.c_uint64), s.size,
+                                     int(max_moves)))
+
+    def launch(self):
+        self._check(lib().ta3h_launch(self._h))
+
+    def sync(self):
+        self._check(lib().ta3h_sync(self._h))
+
+    def fetch(self):
+        fit = np.zeros(self._items, dtype=np.float32)
+        pl = np.zeros(self._items, dtype=np.uint32)
+        self._check(lib().ta3h_fetch(self._h, _ptr(fit, C.c_float), _ptr(pl, C.c_uint32)))
+        return fit, pl
+
+    def fetch_counters(self):
+        """per-item (candidates fitted, candidates legal) of the last launch"""
+        out = np.zeros(2 * self._items, dtype=np.uint64)
+        self._check(lib().ta3h_fetch_counters(self._h, _ptr(out, C.c_uint64)))
+        return out.reshape(-1, 2)
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_float(0)
+        self._check(lib().ta3h_last_kernel_ms(self._h, C.byref(v)))
+        return float(v.value)
+
+    def launch_info(self):
+        g, w, c = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        self._check(lib().ta3h_launch_info(self._h, C.byref(g), C.byref(w), C.byref(c)))
+        return {"grid_blocks": g.value, "waves_per_block": w.value, "num_cus": c.value}
+
+
+def expand_seeds(seed: int, games: int) -> np.ndarray:
+    """gpu::simulate's seed expansion: seeds[g] = xoshiro256**{seed}() in order (model_eval.hpp:92-95)."""
+    mask = (1 << 64) - 1
+    s = seed & mask
+    st = []
+    for _ in range(4):  # splitmix64 seeding (xoshiro256ss.hpp:99-107)
+        s = (s + 0x9E3779B97F4A7C15) & mask
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        st.append(z ^ (z >> 31))
+    out = []
+    for _ in range(games):
+        r = (((st[1] * 5) & mask) << 7 | ((st[1] * 5) & mask) >> 57) & mask
+        out.append((r * 9) & mask)
+        t = (st[1] << 17) & mask
+        st[2] ^= st[0]
+        st[3] ^= st[1]
+        st[1] ^= st[2]
+        st[0] ^= st[3]
+        st[2] ^= t
+        st[3] = ((st[3] << 45) | (st[3] >> 19)) & mask
+    return np.array(out, dtype=np.uint64)
+
+
+def simulate(evaluator: ModelEvaluator, model_weights, num_models: int, games: int, seed: int, max_moves: int):
+    """Batched simulate (model_eval.hpp:84-103): num_models models over `games` shared games seeded from `seed`."""
+    return evaluator.eval(model_weights, num_models, expand_seeds(seed, games), max_moves)
+
+
+def device_bits_canary() -> bool:
+    ok = C.c_int(0)
+    if lib().ta3h_device_bits_canary(C.byref(ok)) != 0:
+        raise HipError("device_bits_canary: HIP error")
+    return bool(ok.value)
+
+
+def feature_norms(n: int) -> np.ndarray:
+    out = np.zeros(4 * n, dtype=np.float32)
+    if lib().ta3h_feature_norms(_ptr(out, C.c_float), n) != 0:
+        raise HipError("feature_norms: HIP error")
+    return out.reshape(n, 4)
+
+
+def tables():
+    """Host copy of the compile-time tables: ({piece: [(o, x), ...]}, trees[2][4] lists of node dicts)."""
+    ox = np.zeros(162 * 2, dtype=np.int8)
+    off = np.zeros(8, dtype=np.uint8)
+    tree = np.zeros(2 * 4 * 16 * 6, dtype=np.uint8)
+    cnt = np.zeros(8, dtype=np.uint8)
+    lib().ta3h_tables(_ptr(ox, C.c_int8), _ptr(off, C.c_uint8), _ptr(tree, C.c_uint8), _ptr(cnt, C.c_uint8))
+    lut = {p: [(int(ox[2 * i]), int(ox[2 * i + 1])) for i in range(off[p], off[p + 1])] for p in range(7)}
+    trees = []
+    for h in range(2):
+        layers = []
+        for l in range(4):
+            layer = []
+            for n in range(int(cnt[h * 4 + l])):
+                v = tree[((h * 4 + l) * 16 + n) * 6:((h * 4 + l) * 16 + n) * 6 + 6]
+                layer.append(dict(slot=int(v[0]), parent=int(v[1]), child_begin=int(v[2]), child_end=int(v[3]),
+                                  root_hold=bool(v[4]), held_slot=int(v[5])))
+            layers.append(layer)
+        trees.append(layers)
+    return lut, trees

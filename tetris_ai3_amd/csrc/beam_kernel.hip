@@ -1,0 +1,845 @@
+// Beam-search evaluator kernels for CDNA4 (gfx950), wave64.
+//
+// One wavefront plays one (model, game) item at a time and pulls the next item from a device work
+// queue when its game ends, so uneven, early-terminating games keep every wave busy. Per move it runs
+// the depth-4 beam search of the reference (ta3_ai/incl/ta3/ai/search/beam.hpp:483-570):
+//   roots (<= 2 x 34 candidates) -> top-16 -> level 1 (<= 16 x 2 x 34) -> top-32
+//   -> level 2 (<= 32 x 2 x 34) -> top-32 -> leaves (<= 32 x 2 x 34) -> argmax -> commit.
+// Each stage is a compacted candidate space: segment s = 2k + c is (kept node k, child c); its
+// placements are laid out back to back, one candidate per lane, so pieces with 9, 17 or 34
+// placements all fill the 64 lanes. The slot / order id of a candidate is s * 34 + i, which is the
+// reference's slot numbering (beam.hpp:296, :360, :423).
+//
+// Exact top-K ("max score, lowest slot on ties", beam.hpp:244-265) is kept as a lane-distributed
+// sorted list of unique 64-bit keys (score << 32 | ~slot), merged chunk by chunk with in-wave bitonic
+// networks; a chunk with no key above the current K-th is skipped.
+//
+// Floating point: every multiply and add of the value network is rounded separately (__fmul_rn /
+// __fadd_rn) in the reference order, divisions and square roots are IEEE correctly rounded, so leaf
+// values are bit-identical to the reference host build.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace ta3h {
+namespace {
+
+__constant__ Tables c_tab = TABLES;
+
+constexpr int K0 = 16, K1 = 32, K2 = 32;        // beam widths (variation_sequences.hpp:22)
+constexpr uint32_t LEAF_ORDERS = K2 * 2 * 34;   // leaf_orders (beam.hpp:188)
+constexpr uint32_t SPAN = LEAF_ORDERS + K0;     // order_span (beam.hpp:196-198)
+constexpr int COMMITTED = 64;                   // board index of the committed board in wave LDS
+
+// ---------------------------------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------------------------------
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// orders this wave's LDS writes before its later LDS reads (all lanes of a wave run in lockstep)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ inline uint32_t shfl_xor_u32(uint32_t v, int m) { return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m)); }
+__device__ inline uint32_t shfl_u32(uint32_t v, int src) { return static_cast<uint32_t>(__shfl(static_cast<int>(v), src)); }
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    uint32_t const lo = shfl_xor_u32(static_cast<uint32_t>(v), m);
+    uint32_t const hi = shfl_xor_u32(static_cast<uint32_t>(v >> 32), m);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ inline uint64_t shfl_u64(uint64_t v, int src) {
+    uint32_t const lo = shfl_u32(static_cast<uint32_t>(v), src);
+    uint32_t const hi = shfl_u32(static_cast<uint32_t>(v >> 32), src);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ inline uint64_t readlane_u64(uint64_t v, int src) {
+    uint32_t const lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), src);
+    uint32_t const hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), src);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ inline uint32_t wave_excl_scan(uint32_t x) {
+    uint32_t const lane = lane_id();
+    uint32_t v = x;
+#pragma unroll
+    for(int d = 1; d < 64; d <<= 1) {
+        uint32_t const t = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), d));
+        if(lane >= static_cast<uint32_t>(d)) v += t;
+    }
+    return v - x;
+}
+
+__device__ inline uint32_t wave_or_u32(uint32_t x) {
+#pragma unroll
+    for(int m = 32; m > 0; m >>= 1) x |= shfl_xor_u32(x, m);
+    return x;
+}
+
+__device__ inline uint64_t wave_or_u64(uint64_t x) {
+#pragma unroll
+    for(int m = 32; m > 0; m >>= 1) x |= shfl_xor_u64(x, m);
+    return x;
+}
+
+__device__ inline uint64_t wave_max_u64(uint64_t x) {
+#pragma unroll
+    for(int m = 32; m > 0; m >>= 1) {
+        uint64_t const p = shfl_xor_u64(x, m);
+        x = p > x ? p : x;
+    }
+    return x;
+}
+
+// descending bitonic sort of one key per lane (64 keys)
+__device__ inline uint64_t bitonic_sort_desc(uint64_t v) {
+    uint32_t const lane = lane_id();
+#pragma unroll
+    for(int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for(int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t const p = shfl_xor_u64(v, j);
+            bool const takeMax = ((lane & j) == 0) == ((lane & k) == 0);
+            uint64_t const mx = v > p ? v : p;
+            uint64_t const mn = v > p ? p : v;
+            v = takeMax ? mx : mn;
+        }
+    }
+    return v;
+}
+
+// Running exact top-K (K <= 32): lanes 0..K-1 hold the best keys seen so far in descending order,
+// other lanes hold 0. Folds one chunk of 64 candidate keys (0 = no candidate) into the list.
+template<int K>
+__device__ inline uint64_t topk_merge(uint64_t running, uint64_t key) {
+    uint64_t const thr = readlane_u64(running, K - 1);
+    if(__ballot(key > thr) == 0) return running;
+    uint32_t const lane = lane_id();
+    uint64_t const sorted = bitonic_sort_desc(key);
+    uint64_t const mirrored = shfl_u64(sorted, static_cast<int>(63 - lane));
+    uint64_t v = lane < 32 ? running : mirrored; // descending then ascending: a bitonic sequence
+#pragma unroll
+    for(int j = 32; j > 0; j >>= 1) {
+        uint64_t const p = shfl_xor_u64(v, j);
+        uint64_t const mx = v > p ? v : p;
+        uint64_t const mn = v > p ? p : v;
+        v = (lane & j) == 0 ? mx : mn;
+    }
+    return lane < static_cast<uint32_t>(K) ? v : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// bit helpers (semantics of ta3_sim/incl/ta3/sim/utility/bits.hpp:27-58)
+// ---------------------------------------------------------------------------------------------------
+__device__ inline int ctz_nz(uint32_t v) { return __builtin_ctz(v); }          // v != 0
+__device__ inline int bit_width(uint32_t v) { return 32 - __clz(static_cast<int>(v)); } // __clz(0) == 32
+__device__ inline int popc(uint32_t v) { return __popc(v); }
+
+__device__ inline uint32_t ordered_bits(float v) { // beam.hpp:204-207
+    uint32_t const u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// engine state (one per wave, in LDS); the serial engine steps run on lane 0
+// ---------------------------------------------------------------------------------------------------
+struct EngineState {
+    uint32_t cols[WIDTH];
+    uint32_t queue[6];
+    uint32_t held;
+    uint32_t gameOver;
+    uint32_t bag[7];
+    uint32_t bagPos;
+    uint64_t rng[4];
+    // stats_v4 fields read by score_v1 (ta3_ai/incl/ta3/ai/models/metrics/tetris_stats_v4.hpp:8-15)
+    uint32_t pieces;
+    uint32_t clears[5];
+    uint64_t sumMaxHeight;
+    uint64_t sumHoleDepths;
+    uint32_t placements;
+};
+
+struct alignas(16) WaveLds {
+    uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
+    uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:91-113)
+    uint32_t score[COMMITTED + 1];            // ordered_bits score of the kept node
+    uint32_t info[COMMITTED + 1];             // tree node | root rank << 8
+    uint32_t segOff[64];                      // exclusive prefix offsets of the stage's segments
+    uint32_t segInfo[64];                     // parent board | piece << 8 | heldIsI << 11 | child << 12
+    uint32_t rootSlot[32];
+    uint32_t slotPiece[8];                    // pieces per abstract window slot (beam.hpp:226-237)
+    uint32_t treeIdx;
+    EngineState eng;
+};
+
+__device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:112-124
+    uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
+    uint64_t const t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = (s[3] << 45) | (s[3] >> 19);
+    return r;
+}
+
+__device__ inline uint32_t draw_piece(EngineState& e) { // tetris_engine.hpp:116-129
+    if(e.bagPos > 0) return e.bag[--e.bagPos];
+    for(uint32_t i = 0; i < 7; ++i) e.bag[i] = i;
+    for(uint32_t i = 6; i > 0; --i) {
+        uint32_t const j = static_cast<uint32_t>(rng_next(e.rng) % (i + 1));
+        uint32_t const t = e.bag[i];
+        e.bag[i] = e.bag[j];
+        e.bag[j] = t;
+    }
+    e.bagPos = 6;
+    return e.bag[6];
+}
+
+__device__ inline uint32_t column_of(uint32_t cols, int col0, int x) { // board2.hpp:165-168
+    uint32_t const d = static_cast<uint32_t>(x - col0);
+    return d < 4 ? (cols >> (4 * d)) & 0xFu : 0u;
+}
+
+__device__ inline void update_game_over(EngineState& e) { // tetris_engine.hpp:159-161, board2.hpp:236-256
+    uint32_t const t = e.queue[0];
+    uint32_t const cols = c_tab.shapeCols[t][0];
+    int const col0 = 3 + c_tab.shapeLeft[t][0];
+    int const w = (bit_width(cols) + 3) / 4;
+    bool ok = col0 >= 0 && col0 + w <= WIDTH;
+    uint32_t hit = 0;
+    for(int x = 0; x < WIDTH; ++x) hit |= e.cols[x] & column_of(cols, col0, x);
+    e.gameOver = (ok && hit == 0) ? 0u : 1u;
+}
+
+__device__ inline void advance_queue(EngineState& e) { // tetris_engine.hpp:131-135
+    for(int i = 1; i < 6; ++i) e.queue[i - 1] = e.queue[i];
+    e.queue[5] = draw_piece(e);
+}
+
+__device__ void engine_reset(EngineState& e, uint64_t seed) { // tetris_engine.hpp:103-114
+    for(int x = 0; x < WIDTH; ++x) e.cols[x] = 0;
+    e.held = NO_PIECE;
+    e.gameOver = 0;
+    e.bagPos = 0;
+    for(int i = 0; i < 4; ++i) { // splitmix64 seeding (xoshiro256ss.hpp:99-107)
+        seed += 0x9e3779b97f4a7c15ULL;
+        uint64_t z = seed;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        e.rng[i] = z ^ (z >> 31);
+    }
+    for(int i = 0; i < 6; ++i) e.queue[i] = draw_piece(e);
+    update_game_over(e);
+    e.pieces = 0;
+    for(int k = 0; k < 5; ++k) e.clears[k] = 0;
+    e.sumMaxHeight = 0;
+    e.sumHoleDepths = 0;
+    e.placements = 0;
+}
+
+__device__ inline int clear_full_rows(uint32_t* cols) { // board2.hpp:333-342 with :218-234
+    uint32_t full = FIELD;
+    for(int x = 0; x < WIDTH; ++x) full &= cols[x];
+    if(full == 0) return 0;
+    int const n = popc(full);
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t col = cols[x], rows = full;
+        for(int i = 0; i < n; ++i) {
+            uint32_t const row = rows & (~rows + 1);
+            col = ((col & (row - 1)) << 1) | (col & ~((row << 1) - 1));
+            rows ^= row;
+        }
+        cols[x] = col;
+    }
+    return n;
+}
+
+// hard drop of the current piece; -1 = DIED (tetris_engine.hpp:137-146)
+__device__ int engine_place(EngineState& e, int o, int x) {
+    uint32_t const t = e.queue[0];
+    uint32_t const cols = c_tab.shapeCols[t][o];
+    int const col0 = x + c_tab.shapeLeft[t][o];
+    int minDrop = HEIGHT; // dropDistance at y = 0 (board2.hpp:262-275)
+    for(int c = 0; c < WIDTH; ++c) {
+        uint32_t const nib = column_of(cols, col0, c);
+        int const low = bit_width(nib) - 1;
+        uint32_t const mask = ~((1u << (low + 1)) - 1);
+        int const drop = ctz_nz((e.cols[c] & mask) | FLOOR) - low - 1;
+        if(nib != 0) minDrop = drop < minDrop ? drop : minDrop;
+    }
+    for(int c = 0; c < WIDTH; ++c) e.cols[c] |= column_of(cols, col0, c) << minDrop;
+    int const cleared = clear_full_rows(e.cols);
+    advance_queue(e);
+    update_game_over(e);
+    return e.gameOver ? -1 : cleared;
+}
+
+__device__ void engine_hold(EngineState& e) { // tetris_engine.hpp:148-157
+    if(e.held == NO_PIECE) {
+        e.held = e.queue[0];
+        advance_queue(e);
+    } else {
+        uint32_t const t = e.held;
+        e.held = e.queue[0];
+        e.queue[0] = t;
+    }
+    update_game_over(e);
+}
+
+// stats_v4::advance for the fields score_v1 reads (tetris_stats.hpp:157-162, metrics.hpp:94-127)
+__device__ void stats_advance(EngineState& e, int cleared) {
+    ++e.pieces;
+    ++e.clears[cleared];
+    int maxH = 0, hd = 0;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const col = e.cols[x];
+        int const h = HEIGHT - ctz_nz(col | FLOOR);
+        maxH = h > maxH ? h : maxH;
+        int const n = popc(col);
+        int s = 0;
+        for(int y = 0; y < HEIGHT; ++y) s += ((col >> y) & 1u) ? y : 0;
+        hd += (HEIGHT - 1) * n + n * (n + 1) / 2 - (s + n * n);
+    }
+    e.sumMaxHeight += static_cast<uint64_t>(maxH);
+    e.sumHoleDepths += static_cast<uint64_t>(hd);
+}
+
+__device__ float final_score(EngineState const& e) { // score_v1.hpp:20-34; model_eval_host.cpp:51
+    if(e.pieces == 0) return -1.0e6f;
+    double const payouts[5] = {0.0, 0.2, 0.8, 2.0, 10.0};
+    double bonus = 0.0;
+    for(int k = 1; k < 5; ++k) bonus = __dadd_rn(bonus, __dmul_rn(payouts[k], static_cast<double>(e.clears[k])));
+    double const pieces = static_cast<double>(e.pieces);
+    double const survival = __dmul_rn(0.02, pieces);
+    double const avgMaxH = __ddiv_rn(static_cast<double>(e.sumMaxHeight), pieces);
+    double const over = __dadd_rn(avgMaxH, -12.0);
+    double const heightPenalty = __dmul_rn(over > 0.0 ? over : 0.0, 1.0);
+    double const avgHd = __ddiv_rn(static_cast<double>(e.sumHoleDepths), pieces);
+    double const holesPenalty = __dmul_rn(avgHd, 1.5);
+    double s = __dadd_rn(bonus, survival);
+    s = __dsub_rn(s, heightPenalty);
+    s = __dsub_rn(s, holesPenalty);
+    return static_cast<float>(s);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// candidate evaluation: apply (search.hpp:294-302) + features + value network
+// ---------------------------------------------------------------------------------------------------
+__device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::max<float>(0, v)
+__device__ inline float mad_rn(float acc, float a, float b) { return __fadd_rn(acc, __fmul_rn(a, b)); }
+
+// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347)
+__device__ inline float value_v11(float const* __restrict__ w, float const (&f)[16]) {
+    float out = w[96];
+#pragma unroll
+    for(int o = 0; o < 8; ++o) {
+        float v = w[80 + o];
+#pragma unroll
+        for(int i = 0; i < 10; ++i) v = mad_rn(v, f[i], w[o * 10 + i]);
+        out = mad_rn(out, relu(v), w[88 + o]);
+    }
+    return out;
+}
+
+// 16 -> 32 ReLU -> 32 ReLU -> 1 on the device layout: W1[32][16], b1[32], W2 transposed [32 in][32 out],
+// b2[32], W3[32], b3. Layer 2 accumulates in ascending input order, the order of a row-wise dense().
+constexpr int F16_W1 = 0, F16_B1 = 512, F16_W2T = 544, F16_B2 = 1568, F16_W3 = 1600, F16_B3 = 1632;
+
+__device__ inline float value_f16(float const* __restrict__ w, float const (&f)[16]) {
+    float acc[32];
+#pragma unroll
+    for(int j = 0; j < 32; ++j) acc[j] = w[F16_B2 + j];
+#pragma unroll 2
+    for(int o = 0; o < 32; ++o) {
+        float v = w[F16_B1 + o];
+#pragma unroll
+        for(int i = 0; i < 16; ++i) v = mad_rn(v, f[i], w[F16_W1 + o * 16 + i]);
+        float const h = relu(v);
+#pragma unroll
+        for(int j = 0; j < 32; ++j) acc[j] = mad_rn(acc[j], h, w[F16_W2T + o * 32 + j]);
+    }
+    float out = w[F16_B3];
+#pragma unroll
+    for(int j = 0; j < 32; ++j) out = mad_rn(out, relu(acc[j]), w[F16_W3 + j]);
+    return out;
+}
+
+// set-bit index planes for 23 rows: plane b has bit y set when bit b of y is set
+constexpr uint32_t PLANE0 = 0x2AAAAAu, PLANE1 = 0x4CCCCCu, PLANE2 = 0x70F0F0u, PLANE3 = 0x00FF00u, PLANE4 = 0x7F0000u;
+
+template<int KIND>
+__device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, bool heldIsI,
+                                   float const* __restrict__ w) {
+    int h[WIDTH];
+    int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const col = b[x];
+        h[x] = HEIGHT - ctz_nz(col | FLOOR);
+        int const n = popc(col);
+        holes += h[x] - n;
+        maxH = h[x] > maxH ? h[x] : maxH;
+        int const s = popc(col & PLANE0) + 2 * popc(col & PLANE1) + 4 * popc(col & PLANE2)
+            + 8 * popc(col & PLANE3) + 16 * popc(col & PLANE4);
+        hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - (s + n * n);
+        if(x > 0) {
+            int const d = h[x] - h[x - 1];
+            sv += d * d;
+            if(KIND == MODEL_F16) bump += d < 0 ? -d : d;
+        }
+        if(KIND == MODEL_F16) {
+            minH = h[x] < minH ? h[x] : minH;
+            agg += h[x];
+            uint32_t const yc = col | FLOOR;
+            ytr += popc((yc ^ (yc >> 1)) & FIELD);
+        }
+    }
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        int const lim = h[x] + 3;
+        bool const l = x == 0 ? true : h[x - 1] >= lim;
+        bool const r = x == WIDTH - 1 ? true : h[x + 1] >= lim;
+        bool const edgeL = x == 0 ? h[1] >= lim : true;
+        bool const edgeR = x == WIDTH - 1 ? h[WIDTH - 2] >= lim : true;
+        wells += (l && r && edgeL && edgeR) ? 1 : 0;
+    }
+    float f[16];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
+    f[4] = __fdiv_rn(sv > 0 ? __fsqrt_rn(static_cast<float>(sv)) : 0.0f, 100.0f);
+    f[5] = __fdiv_rn(static_cast<float>(holes), 20.0f);
+    f[6] = __fdiv_rn(static_cast<float>(hd), 200.0f);
+    f[7] = static_cast<float>(maxH);
+    f[8] = heldIsI ? 1.0f : 0.0f;
+    f[9] = __fdiv_rn(static_cast<float>(wells), 5.0f);
+    if constexpr(KIND == MODEL_F16) {
+        uint32_t const c0 = clears & 0xFFu, c1 = (clears >> 8) & 0xFFu, c2 = (clears >> 16) & 0xFFu,
+                       c3 = clears >> 24;
+        int const lines = static_cast<int>(c0 + 2 * c1 + 3 * c2 + 4 * c3);
+        int const cscore = -static_cast<int>(c0) - 2 * static_cast<int>(c1) - 3 * static_cast<int>(c2)
+            + 10 * static_cast<int>(c3);
+        f[10] = __fdiv_rn(static_cast<float>(agg), 200.0f);
+        f[11] = __fdiv_rn(static_cast<float>(ytr), 40.0f);
+        f[12] = __fdiv_rn(static_cast<float>(bump), 100.0f);
+        f[13] = __fmul_rn(static_cast<float>(lines), 0.1f);
+        f[14] = __fmul_rn(static_cast<float>(cscore), 0.1f);
+        f[15] = static_cast<float>(maxH - minH);
+        return value_f16(w, f);
+    } else {
+        return value_v11(w, f);
+    }
+}
+
+// fit (board2.hpp:306-320) + placed (:322-327) + clearLines (:333-342) on a register board.
+// returns false when the piece collides at spawn; otherwise nb = resting board after clears.
+__device__ inline bool apply_placement(uint32_t const* __restrict__ parent, LutEntry e, uint32_t (&nb)[WIDTH],
+                                       int& cleared) {
+    uint32_t const cols = e.cols;
+    int const col0 = e.col0;
+    uint32_t hit = 0;
+    int minDrop = HEIGHT;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const bc = parent[x];
+        nb[x] = bc;
+        uint32_t const nib = column_of(cols, col0, x);
+        hit |= bc & nib;
+        int const low = bit_width(nib) - 1;
+        uint32_t const mask = ~((1u << (low + 1)) - 1);
+        int const drop = ctz_nz((bc & mask) | FLOOR) - low - 1;
+        minDrop = (nib != 0 && drop < minDrop) ? drop : minDrop;
+    }
+    if(hit != 0) return false;
+    uint32_t full = FIELD;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        nb[x] |= column_of(cols, col0, x) << minDrop;
+        full &= nb[x];
+    }
+    cleared = 0;
+    if(full != 0) {
+        int const n = popc(full);
+        cleared = n;
+#pragma unroll
+        for(int x = 0; x < WIDTH; ++x) {
+            uint32_t col = nb[x], rows = full;
+            for(int i = 0; i < n; ++i) {
+                uint32_t const row = rows & (~rows + 1);
+                col = ((col & (row - 1)) << 1) | (col & ~((row << 1) - 1));
+                rows ^= row;
+            }
+            nb[x] = col;
+        }
+    }
+    return true;
+}
+
+__device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t::added (model_defs.hpp:108-110)
+    return cleared == 0 ? h : h + (1u << (8 * (cleared - 1)));
+}
+
+__device__ inline bool held_is_i(WaveLds const& L, uint8_t heldSlot) {
+    return heldSlot != HELD_NONE && L.slotPiece[heldSlot] == PIECE_I;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// one stage of the beam over a compacted candidate space
+// ---------------------------------------------------------------------------------------------------
+enum StageKind { STAGE_ROOT = 0, STAGE_MID = 1, STAGE_LEAF = 2 };
+
+// Builds the stage's segment table in LDS. Segment s = 2k + c: child c of kept node k of buffer `cur`
+// (roots: s = root group, parent = the committed board). Returns the candidate count.
+template<int STAGE>
+__device__ inline uint32_t build_segments(WaveLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur) {
+    uint32_t const s = lane_id();
+    uint32_t n = 0, info = 0;
+    if(STAGE == STAGE_ROOT) {
+        if(s < c_tab.treeCount[ti][0]) {
+            TreeNode const node = c_tab.tree[ti][0][s];
+            uint32_t const piece = L.slotPiece[node.slot];
+            n = c_tab.lutOff[piece + 1] - c_tab.lutOff[piece];
+            info = COMMITTED | (piece << 8) | (held_is_i(L, node.heldSlot) ? 1u << 11 : 0u) | (s << 12);
+        }
+    } else {
+        uint32_t const k = s >> 1, c = s & 1u;
+        if(k < keptCount) {
+            uint32_t const pn = L.info[cur * 32 + k] & 0xFFu;
+            TreeNode const pnode = c_tab.tree[ti][layer - 1][pn];
+            if(c < static_cast<uint32_t>(pnode.childEnd - pnode.childBegin)) {
+                uint32_t const child = pnode.childBegin + c;
+                TreeNode const cn = c_tab.tree[ti][layer][child];
+                uint32_t const piece = L.slotPiece[cn.slot];
+                n = c_tab.lutOff[piece + 1] - c_tab.lutOff[piece];
+                info = (cur * 32 + k) | (piece << 8) | (held_is_i(L, cn.heldSlot) ? 1u << 11 : 0u) | (child << 12);
+            }
+        }
+    }
+    uint32_t const off = wave_excl_scan(n);
+    L.segOff[s] = off;
+    L.segInfo[s] = info;
+    uint32_t const total = __builtin_amdgcn_readlane(off + n, 63);
+    wave_sync();
+    return total;
+}
+
+
+// Scores every candidate of a stage (roots, a middle level or the leaves).
+//  ROOT / MID: returns the exact top-K keys (score << 32 | ~slot) of the legal candidates with a
+//    nonzero score (select_top, beam.hpp:244-265); MID also ORs into *childMask the kept nodes that
+//    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
+//  LEAF: returns the best (score, order) key over all legal leaves and ORs the segments with at least
+//    one legal placement into *legalSeg (expand_leaves, beam.hpp:407-443).
+template<int KIND, int STAGE, int K>
+__device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, float const* __restrict__ w, uint32_t* childMask,
+                                     uint64_t* legalSeg, uint32_t& nLegal) {
+    uint32_t const lane = lane_id();
+    uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
+    uint32_t cm = 0;
+    uint64_t ls = 0;
+    for(uint32_t base = 0; base < total; base += 64) {
+        uint32_t const f = base + lane;
+        uint32_t s = 0;
+#pragma unroll
+        for(uint32_t step = 32; step > 0; step >>= 1)
+            if(L.segOff[s + step] <= f) s += step;
+        uint32_t const info = L.segInfo[s];
+        uint32_t const i = f - L.segOff[s];
+        uint32_t const piece = (info >> 8) & 7u;
+        uint32_t const pb = info & 0x7Fu;
+        bool legal = false;
+        uint32_t score = 0;
+        if(f < total) {
+            LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
+            uint32_t nb[WIDTH];
+            int cleared = 0;
+            legal = apply_placement(&L.boards[pb * WIDTH], e, nb, cleared);
+            if(legal) {
+                uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[pb], cleared);
+                score = ordered_bits(leaf_value<KIND>(nb, clears, ((info >> 11) & 1u) != 0, w));
+            }
+        }
+        nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
+        uint32_t const slot = s * 34 + i;
+        if constexpr(STAGE == STAGE_LEAF) {
+            uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
+            acc = key > acc ? key : acc;
+            ls |= legal ? (1ull << s) : 0ull;
+        } else {
+            if(STAGE == STAGE_MID) cm |= legal ? (1u << (s >> 1)) : 0u;
+            uint64_t const key = (legal && score != 0) ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0;
+            acc = topk_merge<K>(acc, key);
+        }
+    }
+    if(STAGE == STAGE_MID) *childMask = wave_or_u32(cm);
+    if(STAGE == STAGE_LEAF) *legalSeg = wave_or_u64(ls);
+    return acc;
+}
+
+// Re-applies the selected placement of rank j (lane j < count) to its parent and writes kept node j of
+// frontier buffer `dst` (seed_frontier / advance_frontier, beam.hpp:307-329, 379-403).
+__device__ inline void write_frontier(WaveLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
+                                      uint32_t dst, bool root) {
+    uint32_t const j = lane_id();
+    if(j >= count) return;
+    uint32_t const slot = 0xFFFFFFFFu - static_cast<uint32_t>(key);
+    uint32_t const seg = slot / 34, i = slot % 34;
+    uint32_t parentBoard, node, rank, parentClears;
+    if(root) {
+        parentBoard = COMMITTED;
+        node = seg;
+        rank = j;
+        parentClears = 0;
+    } else {
+        uint32_t const k = seg >> 1, c = seg & 1u;
+        parentBoard = src * 32 + k;
+        uint32_t const pinfo = L.info[parentBoard];
+        node = c_tab.tree[ti][layer - 1][pinfo & 0xFFu].childBegin + c;
+        rank = pinfo >> 8;
+        parentClears = L.clears[parentBoard];
+    }
+    uint32_t const piece = L.slotPiece[c_tab.tree[ti][layer][node].slot];
+    LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
+    uint32_t nb[WIDTH];
+    int cleared = 0;
+    apply_placement(&L.boards[parentBoard * WIDTH], e, nb, cleared);
+    uint32_t const d = dst * 32 + j;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = nb[x];
+    L.clears[d] = hist_added(parentClears, cleared);
+    L.score[d] = static_cast<uint32_t>(key >> 32);
+    L.info[d] = node | (rank << 8);
+}
+
+// One move of the beamed search plus its commit (search_move_beam, beam.hpp:483-570; the eval_host loop
+// body, model_eval_host.cpp:37-47). Returns false when the game ends (no move, or a top-out).
+template<int KIND>
+__device__ bool play_move(WaveLds& L, float const* __restrict__ w, uint64_t& nApply, uint64_t& nLegal) {
+    uint32_t legalCount = 0, applyCount = 0;
+    uint32_t const lane = lane_id();
+    // gather the committed state (beam.hpp:503-505)
+    if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
+    if(lane == 0) {
+        L.clears[COMMITTED] = 0;
+        L.slotPiece[0] = L.eng.held;
+        for(int i = 0; i < 5; ++i) L.slotPiece[1 + i] = L.eng.queue[i];
+        L.treeIdx = L.eng.held == NO_PIECE ? 1u : 0u;
+    }
+    wave_sync();
+    uint32_t const ti = uniform(L.treeIdx);
+
+    // stage 0: roots -> top K0
+    uint32_t total = build_segments<STAGE_ROOT>(L, ti, 0, 0, 0);
+    applyCount += total;
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0>(L, total, w, nullptr, nullptr, legalCount);
+    uint32_t count = __popcll(__ballot(top != 0));
+    nApply += applyCount;
+    nLegal += legalCount;
+    if(count == 0) return false;
+    applyCount = legalCount = 0;
+    if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
+    write_frontier(L, top, count, ti, 0, 0, 0, true);
+    wave_sync();
+
+    uint64_t bestKey = 0;
+    uint32_t cur = 0;
+    // middle levels 1 and 2 -> top K1, K2
+#pragma unroll
+    for(int level = 1; level <= 2; ++level) {
+        total = build_segments<STAGE_MID>(L, ti, level, count, cur);
+        applyCount += total;
+        uint32_t childMask = 0;
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1>(L, total, w, &childMask, nullptr, legalCount)
+                         : run_stage<KIND, STAGE_MID, K2>(L, total, w, &childMask, nullptr, legalCount);
+        // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
+        uint64_t fb = 0;
+        if(lane < count && ((childMask >> lane) & 1u) == 0) {
+            uint32_t const rank = L.info[cur * 32 + lane] >> 8;
+            fb = (static_cast<uint64_t>(L.score[cur * 32 + lane]) << 32) | (SPAN - (LEAF_ORDERS + rank));
+        }
+        fb = wave_max_u64(fb);
+        bestKey = fb > bestKey ? fb : bestKey;
+        uint32_t const next = __popcll(__ballot(top != 0));
+        write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false);
+        wave_sync();
+        cur ^= 1u;
+        count = next;
+    }
+
+    // leaves
+    total = build_segments<STAGE_LEAF>(L, ti, 3, count, cur);
+    applyCount += total;
+    uint64_t legalSeg = 0;
+    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1>(L, total, w, nullptr, &legalSeg, legalCount);
+    nApply += applyCount;
+    nLegal += legalCount;
+    {
+        // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
+        uint32_t const s = lane;
+        uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - L.segOff[s];
+        uint64_t fb = 0;
+        if(n != 0 && ((legalSeg >> s) & 1ull) == 0) {
+            uint32_t const k = s >> 1;
+            fb = (static_cast<uint64_t>(L.score[cur * 32 + k]) << 32) | (SPAN - s * 34);
+        }
+        leaf = fb > leaf ? fb : leaf;
+    }
+    leaf = wave_max_u64(leaf);
+    bestKey = leaf > bestKey ? leaf : bestKey;
+
+    // decode_move (beam.hpp:450-470) and commit on lane 0
+    uint32_t const order = SPAN - static_cast<uint32_t>(bestKey);
+    if(lane == 0) {
+        uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
+        uint32_t const rootSlot = L.rootSlot[rank];
+        TreeNode const rn = c_tab.tree[ti][0][rootSlot / 34];
+        uint32_t const piece = L.slotPiece[rn.slot];
+        LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + rootSlot % 34];
+        int const o = e.ox & 3, x = (e.ox >> 2) - 2;
+        EngineState& eng = L.eng;
+        if(rn.rootHold) engine_hold(eng);
+        int const cleared = engine_place(eng, o, x);
+        ++eng.placements;
+        if(cleared >= 0) stats_advance(eng, cleared);
+        else eng.gameOver = 1;
+    }
+    wave_sync();
+    return uniform(L.eng.gameOver) == 0;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------------------
+template<int KIND, int WPB>
+__global__ __launch_bounds__(64 * WPB) void beam_eval_kernel(EvalArgs a) {
+    __shared__ WaveLds lds[WPB];
+    uint32_t const wv = threadIdx.x >> 6;
+    uint32_t const lane = lane_id();
+    WaveLds& L = lds[wv];
+    uint32_t item = a.persistent ? 0u : blockIdx.x * WPB + wv;
+    for(;;) {
+        if(a.persistent) {
+            uint32_t got = 0;
+            if(lane == 0) got = atomicAdd(a.queue, 1u);
+            item = uniform(got);
+        }
+        if(item >= a.items) break;
+        uint32_t const m = item / a.games, g = item % a.games;
+        float const* __restrict__ w = a.weights + static_cast<uint64_t>(m) * a.weightStride;
+        if(lane == 0) engine_reset(L.eng, a.seeds[g]);
+        wave_sync();
+        uint64_t nApply = 0, nLegal = 0;
+        for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
+            if(uniform(L.eng.gameOver) != 0) break;
+            if(!play_move<KIND>(L, w, nApply, nLegal)) break;
+        }
+        if(lane == 0) {
+            a.fitness[item] = final_score(L.eng);
+            if(a.placements) a.placements[item] = L.eng.placements;
+            if(a.counters) {
+                a.counters[2 * static_cast<uint64_t>(item)] = nApply;
+                a.counters[2 * static_cast<uint64_t>(item) + 1] = nLegal;
+            }
+        }
+        wave_sync();
+        if(!a.persistent) break;
+    }
+}
+
+__global__ void reset_counter_kernel(uint32_t* q) {
+    if(threadIdx.x == 0) q[0] = 0;
+}
+
+// Device self-check of the bit intrinsics the board code relies on (device_canary.hpp:16); writes 1 on
+// success. Mirrors bits.hpp:27-58: ctz(0) = 32, bit_width(0) = 0.
+__global__ void bits_canary_kernel(uint32_t* ok) {
+    if(threadIdx.x != 0) return;
+    bool good = true;
+    uint32_t const vals[6] = {1u, 2u, 0x80000000u, 0x00F000u, 0x7FFFFFu, 0xFFFFFFFFu};
+    int const ctzs[6] = {0, 1, 31, 12, 0, 0};
+    int const pops[6] = {1, 1, 1, 4, 23, 32};
+    int const widths[6] = {1, 2, 32, 16, 23, 32};
+    for(int i = 0; i < 6; ++i) {
+        good = good && ctz_nz(vals[i]) == ctzs[i] && popc(vals[i]) == pops[i] && bit_width(vals[i]) == widths[i];
+    }
+    good = good && bit_width(0u) == 0 && ctz_nz(0u | FLOOR) == 23;
+    ok[0] = good ? 1u : 0u;
+}
+
+// Evaluates the normalised features that go through division / square root over their whole integer
+// domain, so a test can compare them bit for bit with host IEEE arithmetic.
+__global__ void feature_norm_kernel(float* out, uint32_t n) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    float const v = static_cast<float>(i);
+    out[4 * i + 0] = __fdiv_rn(i > 0 ? __fsqrt_rn(v) : 0.0f, 100.0f);
+    out[4 * i + 1] = __fdiv_rn(v, 20.0f);
+    out[4 * i + 2] = __fdiv_rn(v, 200.0f);
+    out[4 * i + 3] = __fdiv_rn(v, 5.0f);
+}
+
+template<int KIND>
+hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
+    switch(wpb) {
+        case 1: hipLaunchKernelGGL((beam_eval_kernel<KIND, 1>), dim3(blocks), dim3(64), 0, stream, a); break;
+        case 2: hipLaunchKernelGGL((beam_eval_kernel<KIND, 2>), dim3(blocks), dim3(128), 0, stream, a); break;
+        case 4: hipLaunchKernelGGL((beam_eval_kernel<KIND, 4>), dim3(blocks), dim3(256), 0, stream, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
+    if(a.persistent) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
+        hipError_t const e = hipGetLastError();
+        if(e != hipSuccess) return e;
+    }
+    switch(kind) {
+        case MODEL_V11: return launch_kind<MODEL_V11>(a, wpb, blocks, stream);
+        case MODEL_F16: return launch_kind<MODEL_F16>(a, wpb, blocks, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+namespace {
+template<int KIND>
+hipError_t occupancy_kind(int wpb, int* n) {
+    switch(wpb) {
+        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 1>, 64, 0);
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 2>, 128, 0);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 4>, 256, 0);
+        default: return hipErrorInvalidValue;
+    }
+}
+} // namespace
+
+hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU) {
+    switch(kind) {
+        case MODEL_V11: return occupancy_kind<MODEL_V11>(wavesPerBlock, blocksPerCU);
+        case MODEL_F16: return occupancy_kind<MODEL_F16>(wavesPerBlock, blocksPerCU);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream) {
+    hipLaunchKernelGGL(bits_canary_kernel, dim3(1), dim3(64), 0, stream, ok);
+    return hipGetLastError();
+}
+
+hipError_t launch_feature_norms(float* out, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(feature_norm_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, out, n);
+    return hipGetLastError();
+}
+
+} // namespace ta3h

@@ -1,0 +1,33 @@
+// Launch interface between the host driver and the evaluator kernels (no device code here).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ta3h {
+
+enum ModelKind : int {
+    MODEL_V11 = 0, // reference default value network: 10 features -> 8 ReLU -> 1 (97 params)
+    MODEL_F16 = 1, // 16 features -> 32 ReLU -> 32 ReLU -> 1 (1633 params)
+};
+
+struct EvalArgs {
+    float const* weights;     // device layout per model, weightStride floats apart
+    uint64_t const* seeds;    // one per game
+    float* fitness;           // [items], model-major (item = m * games + g)
+    uint32_t* placements;     // [items] engine.place calls per item (may be null)
+    uint64_t* counters;       // [items][2]: candidates fitted, candidates legal (may be null)
+    uint32_t* queue;          // work-queue counter (persistent mode)
+    uint32_t items;
+    uint32_t games;
+    uint32_t maxMoves;
+    uint32_t weightStride;
+    uint32_t persistent;
+};
+
+hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
+hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU);
+hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream);
+hipError_t launch_feature_norms(float* out, uint32_t n, hipStream_t stream);
+
+} // namespace ta3h
