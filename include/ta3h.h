@@ -88,9 +88,9 @@ int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_
  * bit_width on the device against known answers; *ok = 1 when all match. */
 int ta3h_device_bits_canary(int* ok);
 
-/* Device values of the normalised features for integers 0..n-1: out[4*i + {0,1,2,3}] =
- * {sqrt(i)/100, i/20, i/200, i/5} in float32 (no reference counterpart; lets a test pin the device's
- * division and square root against host IEEE arithmetic). */
+/* Device values of the normalised features for integers 0..n-1, computed as the search computes them:
+ * out[6*i + {0..5}] = {sqrt(i)/100 (i <= 4761, else 0), i/20, i/200, i/5, i/40, i/100} in float32 (no
+ * reference counterpart; lets a test pin the device arithmetic against host IEEE arithmetic). */
 int ta3h_feature_norms(float* out, uint32_t n);
 
 /* Host copy of the compile-time tables, for checks without a GPU: the placement list as

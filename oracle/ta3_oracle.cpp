@@ -564,6 +564,16 @@ void dense(float const* in, int nIn, int nOut, bool relu, float const* w, float*
     }
 }
 
+// Build-defined dense layer of the 16-feature network: the same layout and order as dense() above, but
+// each step is one fused multiply-add, v = fma(in[i], w, v), rounded once (IEEE 754 fusedMultiplyAdd).
+void dense_fma(float const* in, int nIn, int nOut, bool relu, float const* w, float* out) {
+    for(int o = 0; o < nOut; ++o) {
+        float v = w[nIn * nOut + o];
+        for(int i = 0; i < nIn; ++i) v = std::fma(in[i], w[o * nIn + i], v);
+        out[o] = relu ? ((0.0f < v) ? v : 0.0f) : v;
+    }
+}
+
 struct ModelRef {
     int kind;
     float const* w;
@@ -577,12 +587,12 @@ struct ModelRef {
                 dense(h1, 8, 1, false, w + 88, out);
                 return out[0];
             }
-            case TA3O_MODEL_F16: { // 16 -> 32 ReLU -> 32 ReLU -> 1
+            case TA3O_MODEL_F16: { // 16 -> 32 ReLU -> 32 ReLU -> 1, fused multiply-adds
                 float in[16], h1[32], h2[32], out[1];
                 features_f16(clears, b, heldIsI, in);
-                dense(in, 16, 32, true, w, h1);
-                dense(h1, 32, 32, true, w + 544, h2);
-                dense(h2, 32, 1, false, w + 1600, out);
+                dense_fma(in, 16, 32, true, w, h1);
+                dense_fma(h1, 32, 32, true, w + 544, h2);
+                dense_fma(h2, 32, 1, false, w + 1600, out);
                 return out[0];
             }
             case TA3O_MODEL_HEIGHT: { // beam_test.cpp:20-35

@@ -124,14 +124,53 @@ def test_v11_value_matches_numpy_float32():
         assert np.float32(got).tobytes() == want.tobytes()
 
 
-def test_f16_value_matches_numpy_float32():
+def fma32(a, b, c):
+    """IEEE float32 fused multiply-add by exact rational arithmetic: round(a*b + c) once, to nearest even."""
+    from fractions import Fraction
+    x = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    if x == 0:
+        return f32(0.0) if not (np.signbit(f32(a) * f32(b)) and np.signbit(f32(c))) else f32(-0.0)
+    sign = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    e = max(e, -126)                       # subnormal range keeps the exponent of 2^-126
+    q = x / Fraction(2) ** (e - 23)        # 24 significant bits before the point
+    n = q.numerator // q.denominator
+    rem = q - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return f32(sign * float(Fraction(n) * Fraction(2) ** (e - 23)))
+
+
+def test_fma32_reference_rounds_once():
+    # 1 + 2^-24 is a tie in float32; a*b + c computed in float64 then rounded would lose the fused rounding
+    a, b, c = f32(1.0 + 2.0 ** -12), f32(1.0 + 2.0 ** -12), f32(-1.0)
+    assert fma32(a, b, c) == f32(2.0 ** -11 + 2.0 ** -24)
+    assert fma32(f32(3.0), f32(5.0), f32(-1.0)) == f32(14.0)
+
+
+def fma_dense(x, n_in, n_out, relu, w):
+    out = []
+    for o in range(n_out):
+        v = f32(w[n_in * n_out + o])
+        for i in range(n_in):
+            v = fma32(x[i], w[o * n_in + i], v)
+        out.append((v if f32(0) < v else f32(0)) if relu else v)
+    return out
+
+
+def test_f16_value_matches_exact_fma_reference():
+    """the build-defined 16-32-32-1 network: dense layout of utility.hpp:330-347 with each
+    multiply-add fused (one IEEE rounding), checked against exact rational arithmetic"""
     rng = np.random.default_rng(5)
-    for cols in random_boards(15, 2):
+    for cols in random_boards(4, 2):
         w = rng.normal(0, 0.5, 1633).astype(np.float32)
         x = O.features(O.MODEL_F16, 0x01000001, cols, True)
-        h1 = np_dense(x, 16, 32, True, w)
-        h2 = np_dense(h1, 32, 32, True, w[544:])
-        want = np_dense(h2, 32, 1, False, w[1600:])[0]
+        h1 = fma_dense(x, 16, 32, True, w)
+        h2 = fma_dense(h1, 32, 32, True, w[544:])
+        want = fma_dense(h2, 32, 1, False, w[1600:])[0]
         got = O.model_value(O.MODEL_F16, w, 0x01000001, cols, True)
         assert np.float32(got).tobytes() == want.tobytes()
 

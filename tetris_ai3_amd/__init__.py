@@ -202,10 +202,11 @@ def device_bits_canary() -> bool:
 
 
 def feature_norms(n: int) -> np.ndarray:
-    out = np.zeros(4 * n, dtype=np.float32)
+    """device values [i, (sqrt(i)/100, i/20, i/200, i/5, i/40, i/100)] for i < n"""
+    out = np.zeros(6 * n, dtype=np.float32)
     if lib().ta3h_feature_norms(_ptr(out, C.c_float), n) != 0:
         raise HipError("feature_norms: HIP error")
-    return out.reshape(n, 4)
+    return out.reshape(n, 6)
 
 
 def tables():

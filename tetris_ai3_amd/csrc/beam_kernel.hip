@@ -19,6 +19,7 @@
 // values are bit-identical to the reference host build.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -28,6 +29,11 @@ namespace ta3h {
 namespace {
 
 __constant__ Tables c_tab = TABLES;
+
+// sqrt(sv) / 100 for every surface-variance sum sv in [0, 9 * 23^2], from host IEEE arithmetic (the device
+// square root is not correctly rounded)
+constexpr int SV_MAX = 9 * HEIGHT * HEIGHT;
+__constant__ float c_svNorm[SV_MAX + 1];
 
 constexpr int K0 = 16, K1 = 32, K2 = 32;        // beam widths (variation_sequences.hpp:22)
 constexpr uint32_t LEAF_ORDERS = K2 * 2 * 34;   // leaf_orders (beam.hpp:188)
@@ -334,11 +340,24 @@ __device__ float final_score(EngineState const& e) { // score_v1.hpp:20-34; mode
 // ---------------------------------------------------------------------------------------------------
 // candidate evaluation: apply (search.hpp:294-302) + features + value network
 // ---------------------------------------------------------------------------------------------------
+// Weights are read through the constant address space: the per-model base address is wave-uniform, so
+// every weight becomes a scalar load into an SGPR that VALU instructions take as an operand for free.
+using wptr = float const __attribute__((address_space(4)))*;
+
+// Hides the weight base address from loop-invariant code motion: weights are then re-loaded from the
+// scalar cache where they are used instead of being hoisted into (and spilled from) scarce SGPRs.
+__device__ inline wptr opaque(wptr w) {
+    uint64_t v = reinterpret_cast<uint64_t>(w);
+    asm volatile("" : "+s"(v));
+    return reinterpret_cast<wptr>(v);
+}
+
 __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::max<float>(0, v)
 __device__ inline float mad_rn(float acc, float a, float b) { return __fadd_rn(acc, __fmul_rn(a, b)); }
+__device__ inline float fma_rn(float acc, float a, float b) { return __fmaf_rn(a, b, acc); }
 
 // V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347)
-__device__ inline float value_v11(float const* __restrict__ w, float const (&f)[16]) {
+__device__ inline float value_v11(wptr w, float const (&f)[16]) {
     float out = w[96];
 #pragma unroll
     for(int o = 0; o < 8; ++o) {
@@ -351,10 +370,11 @@ __device__ inline float value_v11(float const* __restrict__ w, float const (&f)[
 }
 
 // 16 -> 32 ReLU -> 32 ReLU -> 1 on the device layout: W1[32][16], b1[32], W2 transposed [32 in][32 out],
-// b2[32], W3[32], b3. Layer 2 accumulates in ascending input order, the order of a row-wise dense().
+// b2[32], W3[32], b3. Each step is one fused multiply-add (the network's defined arithmetic); layer 2
+// accumulates in ascending input order, the order of a row-wise dense().
 constexpr int F16_W1 = 0, F16_B1 = 512, F16_W2T = 544, F16_B2 = 1568, F16_W3 = 1600, F16_B3 = 1632;
 
-__device__ inline float value_f16(float const* __restrict__ w, float const (&f)[16]) {
+__device__ inline float value_f16(wptr w, float const (&f)[16]) {
     float acc[32];
 #pragma unroll
     for(int j = 0; j < 32; ++j) acc[j] = w[F16_B2 + j];
@@ -362,23 +382,35 @@ __device__ inline float value_f16(float const* __restrict__ w, float const (&f)[
     for(int o = 0; o < 32; ++o) {
         float v = w[F16_B1 + o];
 #pragma unroll
-        for(int i = 0; i < 16; ++i) v = mad_rn(v, f[i], w[F16_W1 + o * 16 + i]);
+        for(int i = 0; i < 16; ++i) v = fma_rn(v, f[i], w[F16_W1 + o * 16 + i]);
         float const h = relu(v);
 #pragma unroll
-        for(int j = 0; j < 32; ++j) acc[j] = mad_rn(acc[j], h, w[F16_W2T + o * 32 + j]);
+        for(int j = 0; j < 32; ++j) acc[j] = fma_rn(acc[j], h, w[F16_W2T + o * 32 + j]);
     }
     float out = w[F16_B3];
 #pragma unroll
-    for(int j = 0; j < 32; ++j) out = mad_rn(out, relu(acc[j]), w[F16_W3 + j]);
+    for(int j = 0; j < 32; ++j) out = fma_rn(out, relu(acc[j]), w[F16_W3 + j]);
     return out;
 }
+
+// x / c for an integer-valued x: reciprocal multiply plus one FMA correction. Equals the IEEE quotient for
+// every numerator the features can take (checked exhaustively on the device by the feature-norm kernel).
+__device__ inline float div_const(float x, float c, float r) {
+    float const q = __fmul_rn(x, r);
+    float const e = __fmaf_rn(-q, c, x);
+    return __fmaf_rn(e, r, q);
+}
+__device__ inline float div20(int x) { return div_const(static_cast<float>(x), 20.0f, 1.0f / 20.0f); }
+__device__ inline float div200(int x) { return div_const(static_cast<float>(x), 200.0f, 1.0f / 200.0f); }
+__device__ inline float div5(int x) { return div_const(static_cast<float>(x), 5.0f, 1.0f / 5.0f); }
+__device__ inline float div40(int x) { return div_const(static_cast<float>(x), 40.0f, 1.0f / 40.0f); }
+__device__ inline float div100(int x) { return div_const(static_cast<float>(x), 100.0f, 1.0f / 100.0f); }
 
 // set-bit index planes for 23 rows: plane b has bit y set when bit b of y is set
 constexpr uint32_t PLANE0 = 0x2AAAAAu, PLANE1 = 0x4CCCCCu, PLANE2 = 0x70F0F0u, PLANE3 = 0x00FF00u, PLANE4 = 0x7F0000u;
 
 template<int KIND>
-__device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, bool heldIsI,
-                                   float const* __restrict__ w) {
+__device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, bool heldIsI, wptr w) {
     int h[WIDTH];
     int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
 #pragma unroll
@@ -415,21 +447,21 @@ __device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, 
     float f[16];
 #pragma unroll
     for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
-    f[4] = __fdiv_rn(sv > 0 ? __fsqrt_rn(static_cast<float>(sv)) : 0.0f, 100.0f);
-    f[5] = __fdiv_rn(static_cast<float>(holes), 20.0f);
-    f[6] = __fdiv_rn(static_cast<float>(hd), 200.0f);
+    f[4] = c_svNorm[sv];
+    f[5] = div20(holes);
+    f[6] = div200(hd);
     f[7] = static_cast<float>(maxH);
     f[8] = heldIsI ? 1.0f : 0.0f;
-    f[9] = __fdiv_rn(static_cast<float>(wells), 5.0f);
+    f[9] = div5(wells);
     if constexpr(KIND == MODEL_F16) {
         uint32_t const c0 = clears & 0xFFu, c1 = (clears >> 8) & 0xFFu, c2 = (clears >> 16) & 0xFFu,
                        c3 = clears >> 24;
         int const lines = static_cast<int>(c0 + 2 * c1 + 3 * c2 + 4 * c3);
         int const cscore = -static_cast<int>(c0) - 2 * static_cast<int>(c1) - 3 * static_cast<int>(c2)
             + 10 * static_cast<int>(c3);
-        f[10] = __fdiv_rn(static_cast<float>(agg), 200.0f);
-        f[11] = __fdiv_rn(static_cast<float>(ytr), 40.0f);
-        f[12] = __fdiv_rn(static_cast<float>(bump), 100.0f);
+        f[10] = div200(agg);
+        f[11] = div40(ytr);
+        f[12] = div100(bump);
         f[13] = __fmul_rn(static_cast<float>(lines), 0.1f);
         f[14] = __fmul_rn(static_cast<float>(cscore), 0.1f);
         f[15] = static_cast<float>(maxH - minH);
@@ -539,13 +571,14 @@ __device__ inline uint32_t build_segments(WaveLds& L, uint32_t ti, int layer, ui
 //  LEAF: returns the best (score, order) key over all legal leaves and ORs the segments with at least
 //    one legal placement into *legalSeg (expand_leaves, beam.hpp:407-443).
 template<int KIND, int STAGE, int K>
-__device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, float const* __restrict__ w, uint32_t* childMask,
+__device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, wptr w, uint32_t* childMask,
                                      uint64_t* legalSeg, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
     uint32_t cm = 0;
     uint64_t ls = 0;
     for(uint32_t base = 0; base < total; base += 64) {
+        wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per chunk
         uint32_t const f = base + lane;
         uint32_t s = 0;
 #pragma unroll
@@ -564,7 +597,7 @@ __device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, float const* __
             legal = apply_placement(&L.boards[pb * WIDTH], e, nb, cleared);
             if(legal) {
                 uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[pb], cleared);
-                score = ordered_bits(leaf_value<KIND>(nb, clears, ((info >> 11) & 1u) != 0, w));
+                score = ordered_bits(leaf_value<KIND>(nb, clears, ((info >> 11) & 1u) != 0, wl));
             }
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
@@ -622,7 +655,7 @@ __device__ inline void write_frontier(WaveLds& L, uint64_t key, uint32_t count, 
 // One move of the beamed search plus its commit (search_move_beam, beam.hpp:483-570; the eval_host loop
 // body, model_eval_host.cpp:37-47). Returns false when the game ends (no move, or a top-out).
 template<int KIND>
-__device__ bool play_move(WaveLds& L, float const* __restrict__ w, uint64_t& nApply, uint64_t& nLegal) {
+__device__ bool play_move(WaveLds& L, wptr w, uint64_t& nApply, uint64_t& nLegal) {
     uint32_t legalCount = 0, applyCount = 0;
     uint32_t const lane = lane_id();
     // gather the committed state (beam.hpp:503-505)
@@ -721,7 +754,7 @@ __device__ bool play_move(WaveLds& L, float const* __restrict__ w, uint64_t& nAp
 template<int KIND, int WPB>
 __global__ __launch_bounds__(64 * WPB) void beam_eval_kernel(EvalArgs a) {
     __shared__ WaveLds lds[WPB];
-    uint32_t const wv = threadIdx.x >> 6;
+    uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     WaveLds& L = lds[wv];
     uint32_t item = a.persistent ? 0u : blockIdx.x * WPB + wv;
@@ -732,8 +765,9 @@ __global__ __launch_bounds__(64 * WPB) void beam_eval_kernel(EvalArgs a) {
             item = uniform(got);
         }
         if(item >= a.items) break;
-        uint32_t const m = item / a.games, g = item % a.games;
-        float const* __restrict__ w = a.weights + static_cast<uint64_t>(m) * a.weightStride;
+        // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
+        uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
+        wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
         if(lane == 0) engine_reset(L.eng, a.seeds[g]);
         wave_sync();
         uint64_t nApply = 0, nLegal = 0;
@@ -774,16 +808,18 @@ __global__ void bits_canary_kernel(uint32_t* ok) {
     ok[0] = good ? 1u : 0u;
 }
 
-// Evaluates the normalised features that go through division / square root over their whole integer
-// domain, so a test can compare them bit for bit with host IEEE arithmetic.
+// Evaluates the normalised features exactly as the search computes them, over integer numerators
+// 0..n-1, so a test can compare them bit for bit with host IEEE arithmetic.
 __global__ void feature_norm_kernel(float* out, uint32_t n) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
-    float const v = static_cast<float>(i);
-    out[4 * i + 0] = __fdiv_rn(i > 0 ? __fsqrt_rn(v) : 0.0f, 100.0f);
-    out[4 * i + 1] = __fdiv_rn(v, 20.0f);
-    out[4 * i + 2] = __fdiv_rn(v, 200.0f);
-    out[4 * i + 3] = __fdiv_rn(v, 5.0f);
+    int const v = static_cast<int>(i);
+    out[6 * i + 0] = i <= static_cast<uint32_t>(SV_MAX) ? c_svNorm[i] : 0.0f;
+    out[6 * i + 1] = div20(v);
+    out[6 * i + 2] = div200(v);
+    out[6 * i + 3] = div5(v);
+    out[6 * i + 4] = div40(v);
+    out[6 * i + 5] = div100(v);
 }
 
 template<int KIND>
@@ -832,12 +868,25 @@ hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU) {
     }
 }
 
+hipError_t upload_constants() {
+    // fsqrt (v_board_metrics.hpp:23-37) then the /100 normaliser (:229), both IEEE on the host
+    float lut[SV_MAX + 1];
+    for(int sv = 0; sv <= SV_MAX; ++sv) {
+        float const x = static_cast<float>(sv);
+        float const r = x <= 0.0f ? 0.0f : std::sqrt(x);
+        lut[sv] = r / 100.0f;
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_svNorm), lut, sizeof(lut), 0, hipMemcpyHostToDevice);
+}
+
 hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream) {
     hipLaunchKernelGGL(bits_canary_kernel, dim3(1), dim3(64), 0, stream, ok);
     return hipGetLastError();
 }
 
 hipError_t launch_feature_norms(float* out, uint32_t n, hipStream_t stream) {
+    hipError_t const e = upload_constants();
+    if(e != hipSuccess) return e;
     hipLaunchKernelGGL(feature_norm_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, out, n);
     return hipGetLastError();
 }

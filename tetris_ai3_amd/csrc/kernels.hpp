@@ -26,6 +26,8 @@ struct EvalArgs {
 };
 
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
+// uploads the constant tables computed on the host; call once per device before the first launch
+hipError_t upload_constants();
 hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU);
 hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream);
 hipError_t launch_feature_norms(float* out, uint32_t n, hipStream_t stream);

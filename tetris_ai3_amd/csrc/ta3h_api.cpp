@@ -105,6 +105,8 @@ struct ta3h_evaluator {
         if(e != hipSuccess) return fail("create event", e);
         e = hipMalloc(&dQueue, 64);
         if(e != hipSuccess) return fail("malloc queue", e);
+        e = ta3h::upload_constants(); // per process and device; idempotent
+        if(e != hipSuccess) return fail("upload constants", e);
         ready = true;
         return TA3H_OK;
     }
@@ -345,10 +347,10 @@ int ta3h_device_bits_canary(int* ok) {
 int ta3h_feature_norms(float* out, uint32_t n) {
     if(!out || n == 0) return TA3H_E_ARG;
     float* d = nullptr;
-    if(hipMalloc(&d, static_cast<size_t>(n) * 4 * sizeof(float)) != hipSuccess) return TA3H_E_HIP;
+    if(hipMalloc(&d, static_cast<size_t>(n) * 6 * sizeof(float)) != hipSuccess) return TA3H_E_HIP;
     int r = TA3H_OK;
     if(ta3h::launch_feature_norms(d, n, nullptr) != hipSuccess
-       || hipMemcpy(out, d, static_cast<size_t>(n) * 4 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+       || hipMemcpy(out, d, static_cast<size_t>(n) * 6 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         r = TA3H_E_HIP;
     static_cast<void>(hipFree(d));
     return r;
