@@ -27,12 +27,13 @@ sys.path.insert(0, ROOT)
 ISLANDS, POP, GAMES, CAP = 4, 50, 50, 2000
 GEN_SEED = 0x7E7215
 
-# Algorithmic VALU lane-ops per candidate placement of the reference algorithm
-# (10-column board loops, separately rounded multiply and add in the network):
+# Algorithmic VALU lane-ops per candidate placement of the reference algorithm (10-column board loops):
 C_FIT = 120                 # spawn collision + drop distance over 10 columns
 C_PLACE_CLEAR = 45          # lock-in, full-row mask, compaction
 C_FEAT = {0: 355, 1: 425}   # fused column metrics + normalisation (V11 / 16-feature set)
-C_MLP = {0: 184, 1: 3200}   # 88 / 1,633 multiply-adds as separate mul + add, plus ReLUs
+# network: V11 = 88 multiply-adds as separate mul + add (reference rounding) + 8 ReLU;
+# 16-32-32-1 = 1,633 fused multiply-adds + 64 ReLU
+C_MLP = {0: 184, 1: 1697}
 VALU_PEAK_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz (non-packed 32-bit VALU ops)
 
 
@@ -79,7 +80,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--model", default="f16", choices=["f16", "v11"])
-    ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--waves-per-game", type=int, default=0, help="0: the network's tuned default")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -92,7 +93,8 @@ def main():
     kind = T.MODEL_F16 if args.model == "f16" else T.MODEL_V11
     w, M, seeds = workload(kind)
     ev = T.ModelEvaluator(model=kind)
-    ev.set_launch(args.waves_per_block, True, 0)
+    if args.waves_per_game:
+        ev.set_launch(args.waves_per_game, True, 0)
     ev.stage(w, M, seeds, CAP)
 
     for _ in range(args.warmup):

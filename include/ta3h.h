@@ -50,10 +50,10 @@ int ta3h_num_params(int kind);
  * ta3_ai/incl/ta3/ai/model.hpp:44). */
 int ta3h_set_model(ta3h_evaluator* ev, int kind);
 
-/* Launch shape: waves per workgroup (1, 2 or 4), persistent work-queue mode (1) or one wave per item
- * (0), and the persistent grid in workgroups (0 = fill the device at the occupancy limit).
- * Results never depend on these knobs. */
-int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_block, int persistent, uint32_t grid_blocks);
+/* Launch shape: waves cooperating on one game (1, 2 or 4; a workgroup plays one game at a time),
+ * persistent work-queue mode (1) or one workgroup per item (0), and the persistent grid in workgroups
+ * (0 = fill the device at the occupancy limit). Results never depend on these knobs. */
+int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks);
 
 /* Replaces ModelEvaluator::eval(weights, num_models, seeds, max_moves) (model_eval.hpp:46-51;
  * model_eval.cu:97-153): num_models models, model-major weights (model m owns
@@ -78,11 +78,17 @@ int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements)
  * candidates that fit and were scored (the algorithmic work of the search; no reference counterpart). */
 int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 
+/* Optional per-item trace (no reference counterpart): when enabled before ta3h_stage, each launch records
+ * the device wall clock when every (model, game) item starts and ends; fetch writes out[2*item + {0,1}]
+ * and the clock rate. Used to measure load balance over uneven game lengths. */
+int ta3h_set_timeline(ta3h_evaluator* ev, int enabled);
+int ta3h_fetch_timeline(ta3h_evaluator* ev, uint64_t* out, double* ticks_per_second);
+
 /* Device time of the last launch's search kernel in milliseconds (HIP events on the evaluator's stream). */
 int ta3h_last_kernel_ms(ta3h_evaluator* ev, float* ms);
 
-/* Workgroups the last launch used and the number of CUs of the device. */
-int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_per_block, uint32_t* num_cus);
+/* Workgroups the last launch used, waves per game, and the number of CUs of the device. */
+int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_per_game, uint32_t* num_cus);
 
 /* Replaces ta3::gpu::device_bits_canary() (ta3_gpu/incl/ta3/gpu/device_canary.hpp:16): runs ctz / popcount /
  * bit_width on the device against known answers; *ok = 1 when all match. */

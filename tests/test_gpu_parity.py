@@ -120,7 +120,7 @@ def test_edge_cases(ev):
     assert got.tobytes() == want.tobytes()
 
 
-@pytest.mark.parametrize("wpb,persistent,grid", [(1, False, 0), (2, True, 0), (4, True, 3), (4, False, 0), (1, True, 1)])
+@pytest.mark.parametrize("wpb,persistent,grid", [(1, False, 0), (2, True, 0), (4, True, 3), (4, False, 0), (1, True, 1), (2, False, 0), (4, True, 0)])
 def test_launch_knobs_do_not_change_results(wpb, persistent, grid):
     rng = np.random.default_rng(7)
     ws = rng.normal(0, 0.5, (6, 97)).astype(np.float32)
@@ -151,3 +151,20 @@ def test_config0_cpu_runnable_case_bit_exact():
         assert fit.tobytes() == host.tobytes()
         assert (pl == host_pl).all()
         e.close()
+
+
+def test_cpp_dropin_header_caller(tmp_path):
+    """a C++ caller of include/ta3h/model_eval.hpp (the reference's class shape) passes the reference's
+    parity_test.cpp acceptance pattern against the oracle's fitness"""
+    import subprocess
+    exe = os.path.join(HERE, "cpp", "model_eval_test")
+    assert os.path.exists(exe), "build() compiles tests/cpp/model_eval_test"
+    rng = np.random.default_rng(100)
+    ws = np.concatenate([rng.uniform(-2, 2, 97).astype(np.float32) for _ in range(3)])
+    want, _ = O.eval_host_mt(O.MODEL_V11, ws, 3, [1, 2, 3, 4], 200)
+    wf, ef = tmp_path / "w.f32", tmp_path / "e.f32"
+    ws.tofile(wf)
+    want.astype(np.float32).tofile(ef)
+    say("case: C++ drop-in header caller")
+    r = subprocess.run([exe, str(wf), str(ef)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout, r.stderr)

@@ -53,7 +53,7 @@ def test_argument_errors_without_gpu():
         ev.eval(np.zeros(96, dtype=np.float32), 1, [1], 10)
     # zero (model, game) items: an empty result, no launch (model_eval.cu:104-107)
     assert ev.eval(np.zeros(97, dtype=np.float32), 1, np.zeros(0, dtype=np.uint64), 10).size == 0
-    with pytest.raises(T.HipError, match="waves_per_block"):
+    with pytest.raises(T.HipError, match="waves_per_game"):
         ev.set_launch(3)
     with pytest.raises(T.HipError, match="unknown model"):
         T.ModelEvaluator(model=7)

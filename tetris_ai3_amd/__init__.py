@@ -51,6 +51,8 @@ def lib():
             "ta3h_fetch": (C.c_int, [vp, f32p, u32p]),
             "ta3h_last_kernel_ms": (C.c_int, [vp, f32p]),
             "ta3h_fetch_counters": (C.c_int, [vp, u64p]),
+            "ta3h_set_timeline": (C.c_int, [vp, C.c_int]),
+            "ta3h_fetch_timeline": (C.c_int, [vp, u64p, C.POINTER(C.c_double)]),
             "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
             "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
@@ -83,6 +85,7 @@ class ModelEvaluator:
             raise HipError("ta3h_create failed")
         self.model = model
         self._check(lib().ta3h_set_model(self._h, model))
+        self.set_launch(DEFAULT_WAVES_PER_GAME.get(model, 2), True, 0)
 
     def close(self):
         if self._h:
@@ -103,8 +106,10 @@ class ModelEvaluator:
     def num_params(self) -> int:
         return num_params(self.model)
 
-    def set_launch(self, waves_per_block: int = 4, persistent: bool = True, grid_blocks: int = 0):
-        self._check(lib().ta3h_set_launch(self._h, waves_per_block, int(persistent), grid_blocks))
+    def set_launch(self, waves_per_game: int = 2, persistent: bool = True, grid_blocks: int = 0):
+        """waves cooperating on one game (1, 2 or 4), persistent work queue or one workgroup per item,
+        persistent grid size in workgroups (0 = occupancy limit). Results never depend on these."""
+        self._check(lib().ta3h_set_launch(self._h, waves_per_game, int(persistent), grid_blocks))
 
     def _args(self, weights, num_models, seeds):
         w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
@@ -153,6 +158,17 @@ class ModelEvaluator:
         self._check(lib().ta3h_fetch_counters(self._h, _ptr(out, C.c_uint64)))
         return out.reshape(-1, 2)
 
+    def set_timeline(self, enabled: bool = True):
+        self._check(lib().ta3h_set_timeline(self._h, int(enabled)))
+
+    def fetch_timeline(self):
+        """per-item (start, end) in seconds since the first start of the last launch"""
+        out = np.zeros(2 * self._items, dtype=np.uint64)
+        rate = C.c_double(0)
+        self._check(lib().ta3h_fetch_timeline(self._h, _ptr(out, C.c_uint64), C.byref(rate)))
+        t = out.reshape(-1, 2).astype(np.float64)
+        return (t - t[:, 0].min()) / rate.value
+
     def last_kernel_ms(self) -> float:
         v = C.c_float(0)
         self._check(lib().ta3h_last_kernel_ms(self._h, C.byref(v)))
@@ -161,7 +177,29 @@ class ModelEvaluator:
     def launch_info(self):
         g, w, c = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
         self._check(lib().ta3h_launch_info(self._h, C.byref(g), C.byref(w), C.byref(c)))
-        return {"grid_blocks": g.value, "waves_per_block": w.value, "num_cus": c.value}
+        return {"grid_blocks": g.value, "waves_per_game": w.value, "num_cus": c.value}
+
+
+# launch shapes measured fastest on MI355X per network (full CMA-ES generation, see autotune())
+DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4}
+
+
+def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
+             candidates=((1, 0), (2, 0), (4, 0))):
+    """Times one launch per launch shape (waves per game, grid) on the given batch and keeps the fastest
+    on `evaluator`. Shapes only change speed, never results; returns {shape: kernel_ms}."""
+    timings = {}
+    evaluator.stage(weights, num_models, seeds, max_moves)
+    for wpg, grid in candidates:
+        evaluator.set_launch(wpg, True, grid)
+        evaluator.stage(weights, num_models, seeds, max_moves)
+        evaluator.launch()
+        evaluator.sync()
+        timings[(wpg, grid)] = evaluator.last_kernel_ms()
+    best = min(timings, key=timings.get)
+    evaluator.set_launch(best[0], True, best[1])
+    evaluator.stage(weights, num_models, seeds, max_moves)
+    return timings
 
 
 def expand_seeds(seed: int, games: int) -> np.ndarray:

@@ -1,7 +1,9 @@
 // Beam-search evaluator kernels for CDNA4 (gfx950), wave64.
 //
-// One wavefront plays one (model, game) item at a time and pulls the next item from a device work
-// queue when its game ends, so uneven, early-terminating games keep every wave busy. Per move it runs
+// A workgroup of G waves (G = 1, 2 or 4; one per SIMD) plays one (model, game) item at a time and pulls
+// the next item from a device work queue when its game ends, so uneven, early-terminating games keep the
+// device busy. The G waves split every stage's candidates and merge their exact partial top-K lists
+// through LDS, which cuts the per-move latency that bounds the longest (2,000-piece) games. Per move it runs
 // the depth-4 beam search of the reference (ta3_ai/incl/ta3/ai/search/beam.hpp:483-570):
 //   roots (<= 2 x 34 candidates) -> top-16 -> level 1 (<= 16 x 2 x 34) -> top-32
 //   -> level 2 (<= 32 x 2 x 34) -> top-32 -> leaves (<= 32 x 2 x 34) -> argmax -> commit.
@@ -10,7 +12,7 @@
 // placements all fill the 64 lanes. The slot / order id of a candidate is s * 34 + i, which is the
 // reference's slot numbering (beam.hpp:296, :360, :423).
 //
-// Exact top-K ("max score, lowest slot on ties", beam.hpp:244-265) is kept as a lane-distributed
+// Exact top-K ("max score, lowest slot on ties", beam.hpp:244-265) is kept per wave as a lane-distributed
 // sorted list of unique 64-bit keys (score << 32 | ~slot), merged chunk by chunk with in-wave bitonic
 // networks; a chunk with no key above the current K-th is skipped.
 //
@@ -172,7 +174,7 @@ struct EngineState {
     uint32_t placements;
 };
 
-struct alignas(16) WaveLds {
+struct alignas(16) GameLds {
     uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
     uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:91-113)
     uint32_t score[COMMITTED + 1];            // ordered_bits score of the kept node
@@ -182,6 +184,12 @@ struct alignas(16) WaveLds {
     uint32_t rootSlot[32];
     uint32_t slotPiece[8];                    // pieces per abstract window slot (beam.hpp:226-237)
     uint32_t treeIdx;
+    uint32_t stageTotal;
+    uint32_t item;
+    uint64_t partTop[4][32];                  // per-wave top-K lists of a stage
+    uint64_t partBest[4];                     // per-wave best leaf keys
+    uint64_t partMask[4];                     // per-wave kept-node / segment legality masks
+    uint32_t partLegal[4];                    // per-wave legal-candidate counts of the item
     EngineState eng;
 };
 
@@ -519,7 +527,7 @@ __device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t
     return cleared == 0 ? h : h + (1u << (8 * (cleared - 1)));
 }
 
-__device__ inline bool held_is_i(WaveLds const& L, uint8_t heldSlot) {
+__device__ inline bool held_is_i(GameLds const& L, uint8_t heldSlot) {
     return heldSlot != HELD_NONE && L.slotPiece[heldSlot] == PIECE_I;
 }
 
@@ -531,7 +539,7 @@ enum StageKind { STAGE_ROOT = 0, STAGE_MID = 1, STAGE_LEAF = 2 };
 // Builds the stage's segment table in LDS. Segment s = 2k + c: child c of kept node k of buffer `cur`
 // (roots: s = root group, parent = the committed board). Returns the candidate count.
 template<int STAGE>
-__device__ inline uint32_t build_segments(WaveLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur) {
+__device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur) {
     uint32_t const s = lane_id();
     uint32_t n = 0, info = 0;
     if(STAGE == STAGE_ROOT) {
@@ -564,20 +572,26 @@ __device__ inline uint32_t build_segments(WaveLds& L, uint32_t ti, int layer, ui
 }
 
 
-// Scores every candidate of a stage (roots, a middle level or the leaves).
+// Synchronises the G waves that cooperate on one game (one workgroup); a lone wave needs no barrier.
+template<int G>
+__device__ inline void group_sync() {
+    if constexpr(G == 1) wave_sync();
+    else __syncthreads();
+}
+
+// Scores this wave's share of a stage's candidates (chunks wv, wv + G, ...).
 //  ROOT / MID: returns the exact top-K keys (score << 32 | ~slot) of the legal candidates with a
-//    nonzero score (select_top, beam.hpp:244-265); MID also ORs into *childMask the kept nodes that
+//    nonzero score (select_top, beam.hpp:244-265); MID also returns in *mask the kept nodes that
 //    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
-//  LEAF: returns the best (score, order) key over all legal leaves and ORs the segments with at least
-//    one legal placement into *legalSeg (expand_leaves, beam.hpp:407-443).
-template<int KIND, int STAGE, int K>
-__device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, wptr w, uint32_t* childMask,
-                                     uint64_t* legalSeg, uint32_t& nLegal) {
+//  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
+//    least one legal placement (expand_leaves, beam.hpp:407-443).
+template<int KIND, int STAGE, int K, int G>
+__device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_t wv, uint64_t* mask,
+                                     uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
-    uint32_t cm = 0;
-    uint64_t ls = 0;
-    for(uint32_t base = 0; base < total; base += 64) {
+    uint64_t ms = 0;
+    for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per chunk
         uint32_t const f = base + lane;
         uint32_t s = 0;
@@ -605,21 +619,72 @@ __device__ inline uint64_t run_stage(WaveLds& L, uint32_t total, wptr w, uint32_
         if constexpr(STAGE == STAGE_LEAF) {
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
             acc = key > acc ? key : acc;
-            ls |= legal ? (1ull << s) : 0ull;
+            ms |= legal ? (1ull << s) : 0ull;
         } else {
-            if(STAGE == STAGE_MID) cm |= legal ? (1u << (s >> 1)) : 0u;
+            if(STAGE == STAGE_MID) ms |= legal ? (1ull << (s >> 1)) : 0ull;
             uint64_t const key = (legal && score != 0) ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0;
             acc = topk_merge<K>(acc, key);
         }
     }
-    if(STAGE == STAGE_MID) *childMask = wave_or_u32(cm);
-    if(STAGE == STAGE_LEAF) *legalSeg = wave_or_u64(ls);
+    if(STAGE != STAGE_ROOT) *mask = wave_or_u64(ms);
     return acc;
+}
+
+// merges two lane-distributed descending lists (lanes 0..31, zero padded) into the top K of their union
+template<int K>
+__device__ inline uint64_t merge_sorted(uint64_t a, uint64_t b) {
+    uint32_t const lane = lane_id();
+    uint64_t const mirrored = shfl_u64(b, static_cast<int>(63 - lane));
+    uint64_t v = lane < 32 ? a : mirrored; // descending then ascending: a bitonic sequence
+#pragma unroll
+    for(int j = 32; j > 0; j >>= 1) {
+        uint64_t const p = shfl_xor_u64(v, j);
+        uint64_t const mx = v > p ? v : p;
+        uint64_t const mn = v > p ? p : v;
+        v = (lane & j) == 0 ? mx : mn;
+    }
+    return lane < static_cast<uint32_t>(K) ? v : 0;
+}
+
+// Combines the G waves' partial results of a stage; every wave returns the same values. The union of
+// the per-wave exact top-K lists holds the global top-K, so the merged list is exact.
+template<int G, int K, bool TOPK>
+__device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& mask, uint32_t wv) {
+    if constexpr(G == 1) {
+        return part;
+    } else {
+        uint32_t const lane = lane_id();
+        if constexpr(TOPK) {
+            if(lane < 32) L.partTop[wv][lane] = part;
+        } else {
+            part = wave_max_u64(part);
+            if(lane == 0) L.partBest[wv] = part;
+        }
+        if(lane == 0) L.partMask[wv] = mask;
+        group_sync<G>();
+        uint64_t m = 0;
+#pragma unroll
+        for(int g = 0; g < G; ++g) m |= L.partMask[g];
+        mask = m;
+        if constexpr(TOPK) {
+            uint64_t r = merge_sorted<K>(lane < 32 ? L.partTop[0][lane] : 0, lane < 32 ? L.partTop[1][lane] : 0);
+            if constexpr(G == 4) {
+                uint64_t const r2 = merge_sorted<K>(lane < 32 ? L.partTop[2][lane] : 0, lane < 32 ? L.partTop[3][lane] : 0);
+                r = merge_sorted<K>(r, r2);
+            }
+            return r;
+        } else {
+            uint64_t best = 0;
+#pragma unroll
+            for(int g = 0; g < G; ++g) best = L.partBest[g] > best ? L.partBest[g] : best;
+            return best;
+        }
+    }
 }
 
 // Re-applies the selected placement of rank j (lane j < count) to its parent and writes kept node j of
 // frontier buffer `dst` (seed_frontier / advance_frontier, beam.hpp:307-329, 379-403).
-__device__ inline void write_frontier(WaveLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
+__device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
                                       uint32_t dst, bool root) {
     uint32_t const j = lane_id();
     if(j >= count) return;
@@ -652,74 +717,89 @@ __device__ inline void write_frontier(WaveLds& L, uint64_t key, uint32_t count, 
     L.info[d] = node | (rank << 8);
 }
 
+// wave 0 builds the stage's segment table; every wave gets the candidate count
+template<int STAGE, int G>
+__device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur,
+                                          uint32_t wv) {
+    if(wv == 0) {
+        uint32_t const total = build_segments<STAGE>(L, ti, layer, keptCount, cur);
+        if(lane_id() == 0) L.stageTotal = total;
+    }
+    group_sync<G>();
+    return uniform(L.stageTotal);
+}
+
 // One move of the beamed search plus its commit (search_move_beam, beam.hpp:483-570; the eval_host loop
-// body, model_eval_host.cpp:37-47). Returns false when the game ends (no move, or a top-out).
-template<int KIND>
-__device__ bool play_move(WaveLds& L, wptr w, uint64_t& nApply, uint64_t& nLegal) {
-    uint32_t legalCount = 0, applyCount = 0;
+// body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
+// ends (no move, or a top-out).
+template<int KIND, int G>
+__device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     // gather the committed state (beam.hpp:503-505)
-    if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
-    if(lane == 0) {
-        L.clears[COMMITTED] = 0;
-        L.slotPiece[0] = L.eng.held;
-        for(int i = 0; i < 5; ++i) L.slotPiece[1 + i] = L.eng.queue[i];
-        L.treeIdx = L.eng.held == NO_PIECE ? 1u : 0u;
+    if(wv == 0) {
+        if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
+        if(lane == 0) {
+            L.clears[COMMITTED] = 0;
+            L.slotPiece[0] = L.eng.held;
+            for(int i = 0; i < 5; ++i) L.slotPiece[1 + i] = L.eng.queue[i];
+            L.treeIdx = L.eng.held == NO_PIECE ? 1u : 0u;
+        }
     }
-    wave_sync();
+    group_sync<G>();
     uint32_t const ti = uniform(L.treeIdx);
 
     // stage 0: roots -> top K0
-    uint32_t total = build_segments<STAGE_ROOT>(L, ti, 0, 0, 0);
-    applyCount += total;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0>(L, total, w, nullptr, nullptr, legalCount);
+    uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
+    nApply += total;
+    uint64_t mask = 0;
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, total, w, wv, nullptr, nLegal);
+    top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
-    nApply += applyCount;
-    nLegal += legalCount;
     if(count == 0) return false;
-    applyCount = legalCount = 0;
-    if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
-    write_frontier(L, top, count, ti, 0, 0, 0, true);
-    wave_sync();
+    if(wv == 0) {
+        if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
+        write_frontier(L, top, count, ti, 0, 0, 0, true);
+    }
+    group_sync<G>();
 
     uint64_t bestKey = 0;
     uint32_t cur = 0;
     // middle levels 1 and 2 -> top K1, K2
 #pragma unroll
     for(int level = 1; level <= 2; ++level) {
-        total = build_segments<STAGE_MID>(L, ti, level, count, cur);
-        applyCount += total;
-        uint32_t childMask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1>(L, total, w, &childMask, nullptr, legalCount)
-                         : run_stage<KIND, STAGE_MID, K2>(L, total, w, &childMask, nullptr, legalCount);
+        total = stage_segments<STAGE_MID, G>(L, ti, level, count, cur, wv);
+        nApply += total;
+        mask = 0;
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, total, w, wv, &mask, nLegal)
+                         : run_stage<KIND, STAGE_MID, K2, G>(L, total, w, wv, &mask, nLegal);
+        top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
-        if(lane < count && ((childMask >> lane) & 1u) == 0) {
+        if(lane < count && ((mask >> lane) & 1ull) == 0) {
             uint32_t const rank = L.info[cur * 32 + lane] >> 8;
             fb = (static_cast<uint64_t>(L.score[cur * 32 + lane]) << 32) | (SPAN - (LEAF_ORDERS + rank));
         }
         fb = wave_max_u64(fb);
         bestKey = fb > bestKey ? fb : bestKey;
         uint32_t const next = __popcll(__ballot(top != 0));
-        write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false);
-        wave_sync();
+        if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false);
+        group_sync<G>();
         cur ^= 1u;
         count = next;
     }
 
     // leaves
-    total = build_segments<STAGE_LEAF>(L, ti, 3, count, cur);
-    applyCount += total;
-    uint64_t legalSeg = 0;
-    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1>(L, total, w, nullptr, &legalSeg, legalCount);
-    nApply += applyCount;
-    nLegal += legalCount;
+    total = stage_segments<STAGE_LEAF, G>(L, ti, 3, count, cur, wv);
+    nApply += total;
+    mask = 0;
+    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, total, w, wv, &mask, nLegal);
+    leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
         uint32_t const s = lane;
         uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - L.segOff[s];
         uint64_t fb = 0;
-        if(n != 0 && ((legalSeg >> s) & 1ull) == 0) {
+        if(n != 0 && ((mask >> s) & 1ull) == 0) {
             uint32_t const k = s >> 1;
             fb = (static_cast<uint64_t>(L.score[cur * 32 + k]) << 32) | (SPAN - s * 34);
         }
@@ -728,9 +808,9 @@ __device__ bool play_move(WaveLds& L, wptr w, uint64_t& nApply, uint64_t& nLegal
     leaf = wave_max_u64(leaf);
     bestKey = leaf > bestKey ? leaf : bestKey;
 
-    // decode_move (beam.hpp:450-470) and commit on lane 0
+    // decode_move (beam.hpp:450-470) and commit on one lane
     uint32_t const order = SPAN - static_cast<uint32_t>(bestKey);
-    if(lane == 0) {
+    if(wv == 0 && lane == 0) {
         uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
         uint32_t const rootSlot = L.rootSlot[rank];
         TreeNode const rn = c_tab.tree[ti][0][rootSlot / 34];
@@ -744,46 +824,56 @@ __device__ bool play_move(WaveLds& L, wptr w, uint64_t& nApply, uint64_t& nLegal
         if(cleared >= 0) stats_advance(eng, cleared);
         else eng.gameOver = 1;
     }
-    wave_sync();
+    group_sync<G>();
     return uniform(L.eng.gameOver) == 0;
 }
 
 // ---------------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------------
-template<int KIND, int WPB>
-__global__ __launch_bounds__(64 * WPB) void beam_eval_kernel(EvalArgs a) {
-    __shared__ WaveLds lds[WPB];
+// One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
+// next item from the device work queue.
+template<int KIND, int G>
+__global__ __launch_bounds__(64 * G) void beam_eval_kernel(EvalArgs a) {
+    __shared__ GameLds L;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
-    WaveLds& L = lds[wv];
-    uint32_t item = a.persistent ? 0u : blockIdx.x * WPB + wv;
+    uint32_t item = a.persistent ? 0u : blockIdx.x;
     for(;;) {
         if(a.persistent) {
-            uint32_t got = 0;
-            if(lane == 0) got = atomicAdd(a.queue, 1u);
-            item = uniform(got);
+            if(threadIdx.x == 0) L.item = atomicAdd(a.queue, 1u);
+            group_sync<G>();
+            item = uniform(L.item);
         }
         if(item >= a.items) break;
         // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
-        if(lane == 0) engine_reset(L.eng, a.seeds[g]);
-        wave_sync();
-        uint64_t nApply = 0, nLegal = 0;
+        if(threadIdx.x == 0) {
+            if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
+            engine_reset(L.eng, a.seeds[g]);
+        }
+        group_sync<G>();
+        uint64_t nApply = 0;
+        uint32_t nLegal = 0;
         for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
             if(uniform(L.eng.gameOver) != 0) break;
-            if(!play_move<KIND>(L, w, nApply, nLegal)) break;
+            if(!play_move<KIND, G>(L, w, wv, nApply, nLegal)) break;
         }
-        if(lane == 0) {
+        if(lane == 0) L.partLegal[wv] = nLegal;
+        group_sync<G>();
+        if(threadIdx.x == 0) {
             a.fitness[item] = final_score(L.eng);
             if(a.placements) a.placements[item] = L.eng.placements;
             if(a.counters) {
+                uint64_t legal = 0;
+                for(int k = 0; k < G; ++k) legal += L.partLegal[k];
                 a.counters[2 * static_cast<uint64_t>(item)] = nApply;
-                a.counters[2 * static_cast<uint64_t>(item) + 1] = nLegal;
+                a.counters[2 * static_cast<uint64_t>(item) + 1] = legal;
             }
+            if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item) + 1] = wall_clock64();
         }
-        wave_sync();
+        group_sync<G>();
         if(!a.persistent) break;
     }
 }

@@ -17,6 +17,7 @@ struct EvalArgs {
     float* fitness;           // [items], model-major (item = m * games + g)
     uint32_t* placements;     // [items] engine.place calls per item (may be null)
     uint64_t* counters;       // [items][2]: candidates fitted, candidates legal (may be null)
+    uint64_t* timeline;       // [items][2]: device wall clock at item start / end (may be null)
     uint32_t* queue;          // work-queue counter (persistent mode)
     uint32_t items;
     uint32_t games;

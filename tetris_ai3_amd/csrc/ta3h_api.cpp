@@ -64,10 +64,13 @@ struct ta3h_evaluator {
     size_t placementsCap = 0;
     uint64_t* dCounters = nullptr;
     size_t countersCap = 0;
+    uint64_t* dTimeline = nullptr;
+    size_t timelineCap = 0;
+    bool timelineOn = false;
     uint32_t* dQueue = nullptr;
 
     int kind = TA3H_MODEL_V11;
-    int wavesPerBlock = 4;
+    int wavesPerBlock = 2; // waves cooperating on one game
     int persistent = 1;
     uint32_t gridBlocks = 0; // 0: occupancy-derived
     uint32_t lastGrid = 0;
@@ -133,6 +136,7 @@ struct ta3h_evaluator {
         if(dFitness) static_cast<void>(hipFree(dFitness));
         if(dPlacements) static_cast<void>(hipFree(dPlacements));
         if(dCounters) static_cast<void>(hipFree(dCounters));
+        if(dTimeline) static_cast<void>(hipFree(dTimeline));
         if(dQueue) static_cast<void>(hipFree(dQueue));
         if(evStart) static_cast<void>(hipEventDestroy(evStart));
         if(evStop) static_cast<void>(hipEventDestroy(evStop));
@@ -162,11 +166,11 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind) {
     return TA3H_OK;
 }
 
-int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_block, int persistent, uint32_t grid_blocks) {
+int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
-    if(waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
-        return ev->fail_arg("set_launch: waves_per_block must be 1, 2 or 4");
-    ev->wavesPerBlock = waves_per_block;
+    if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4)
+        return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4");
+    ev->wavesPerBlock = waves_per_game;
     ev->persistent = persistent ? 1 : 0;
     ev->gridBlocks = grid_blocks;
     return TA3H_OK;
@@ -199,6 +203,7 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
     if((r = ev->grow(ev->dFitness, ev->fitnessCap, items, "malloc fitness"))) return r;
     if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
     if((r = ev->grow(ev->dCounters, ev->countersCap, 2 * items, "malloc counters"))) return r;
+    if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
     ev->packed.resize(wCount);
     for(uint32_t m = 0; m < num_models; ++m)
         repack(ev->kind, weights + static_cast<size_t>(m) * num_params, ev->packed.data() + static_cast<size_t>(m) * stride);
@@ -226,20 +231,21 @@ int ta3h_launch(ta3h_evaluator* ev) {
     a.fitness = ev->dFitness;
     a.placements = ev->dPlacements;
     a.counters = ev->dCounters;
+    a.timeline = ev->timelineOn ? ev->dTimeline : nullptr;
     a.queue = ev->dQueue;
     a.items = ev->items;
     a.games = ev->games;
     a.maxMoves = ev->maxMoves;
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
-    uint32_t const wpb = static_cast<uint32_t>(ev->wavesPerBlock);
+    // one workgroup (of wavesPerBlock cooperating waves) per game in flight
+    uint32_t const needed = ev->items;
     uint32_t blocks;
-    uint32_t const needed = (ev->items + wpb - 1) / wpb;
     hipError_t e0 = hipSuccess;
     if(ev->persistent) {
         blocks = ev->gridBlocks;
         if(blocks == 0) {
-            // fill every CU up to the kernel's occupancy limit; waves beyond it would only queue
+            // fill every CU up to the kernel's occupancy limit; workgroups beyond it would only queue
             int perCU = 0;
             e0 = ta3h::max_active_blocks(ev->kind, ev->wavesPerBlock, &perCU);
             if(e0 != hipSuccess) return ev->fail("occupancy", e0);
@@ -303,6 +309,34 @@ int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters) {
     return TA3H_OK;
 }
 
+int ta3h_set_timeline(ta3h_evaluator* ev, int enabled) {
+    if(!ev) return TA3H_E_ARG;
+    ev->timelineOn = enabled != 0;
+    ev->staged = false;
+    return TA3H_OK;
+}
+
+int ta3h_fetch_timeline(ta3h_evaluator* ev, uint64_t* out, double* ticks_per_second) {
+    if(!ev || !out) return TA3H_E_ARG;
+    if(!ev->launched || !ev->timelineOn) {
+        ev->err = "fetch_timeline: no launch with the timeline enabled";
+        return TA3H_E_STATE;
+    }
+    if(ticks_per_second) {
+        int khz = 0;
+        hipError_t const e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ev->device);
+        if(e != hipSuccess) return ev->fail("wall clock rate", e);
+        *ticks_per_second = 1000.0 * khz;
+    }
+    if(ev->items == 0) return TA3H_OK;
+    hipError_t e = hipMemcpyAsync(out, ev->dTimeline, 2 * static_cast<size_t>(ev->items) * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy timeline", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("kernel run", e);
+    return TA3H_OK;
+}
+
 int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uint32_t num_params,
               const uint64_t* seeds, uint32_t games, uint32_t max_moves, float* out_fitness) {
     int r = ta3h_stage(ev, weights, num_models, num_params, seeds, games, max_moves);
@@ -322,10 +356,10 @@ int ta3h_last_kernel_ms(ta3h_evaluator* ev, float* ms) {
     return TA3H_OK;
 }
 
-int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_per_block, uint32_t* num_cus) {
+int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_per_game, uint32_t* num_cus) {
     if(!ev) return TA3H_E_ARG;
     if(grid_blocks) *grid_blocks = ev->lastGrid;
-    if(waves_per_block) *waves_per_block = static_cast<uint32_t>(ev->wavesPerBlock);
+    if(waves_per_game) *waves_per_game = static_cast<uint32_t>(ev->wavesPerBlock);
     if(num_cus) *num_cus = ev->numCUs;
     return TA3H_OK;
 }
