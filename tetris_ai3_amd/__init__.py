@@ -181,7 +181,7 @@ class ModelEvaluator:
 
 
 # launch shapes measured fastest on MI355X per network (full CMA-ES generation, see autotune())
-DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4}
+DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4}  # V11 at 5 waves/SIMD, F16 at 4
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,

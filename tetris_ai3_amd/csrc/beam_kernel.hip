@@ -181,6 +181,7 @@ struct alignas(16) GameLds {
     uint32_t info[COMMITTED + 1];             // tree node | root rank << 8
     uint32_t segOff[64];                      // exclusive prefix offsets of the stage's segments
     uint32_t segInfo[64];                     // parent board | piece << 8 | heldIsI << 11 | child << 12
+    uint16_t candSeg[32 * 2 * 34];            // flat candidate index -> segment << 6 | placement index
     uint32_t rootSlot[32];
     uint32_t slotPiece[8];                    // pieces per abstract window slot (beam.hpp:226-237)
     uint32_t treeIdx;
@@ -417,41 +418,45 @@ __device__ inline float div100(int x) { return div_const(static_cast<float>(x), 
 // set-bit index planes for 23 rows: plane b has bit y set when bit b of y is set
 constexpr uint32_t PLANE0 = 0x2AAAAAu, PLANE1 = 0x4CCCCCu, PLANE2 = 0x70F0F0u, PLANE3 = 0x00FF00u, PLANE4 = 0x7F0000u;
 
+// Candidate boards live in LDS, one per lane: column x of the lane's board is col[x * SCR_STRIDE], so the
+// 64 lanes of a wave touch 64 consecutive words (no bank conflicts) and the column loops stay rolled,
+// which keeps the register footprint small.
+constexpr int SCR_STRIDE = 64;
+
 template<int KIND>
-__device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, bool heldIsI, wptr w) {
-    int h[WIDTH];
+__device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool heldIsI, wptr w) {
+    // one streaming pass over the columns (v_board_metrics.hpp:130-225); a column's well flag is settled
+    // once its right neighbour is known
     int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
-#pragma unroll
+    int hPrev2 = 0, hPrev = 0;
+#pragma unroll 2
     for(int x = 0; x < WIDTH; ++x) {
-        uint32_t const col = b[x];
-        h[x] = HEIGHT - ctz_nz(col | FLOOR);
-        int const n = popc(col);
-        holes += h[x] - n;
-        maxH = h[x] > maxH ? h[x] : maxH;
-        int const s = popc(col & PLANE0) + 2 * popc(col & PLANE1) + 4 * popc(col & PLANE2)
-            + 8 * popc(col & PLANE3) + 16 * popc(col & PLANE4);
+        uint32_t const c = col[x * SCR_STRIDE];
+        int const hx = HEIGHT - ctz_nz(c | FLOOR);
+        int const n = popc(c);
+        holes += hx - n;
+        maxH = hx > maxH ? hx : maxH;
+        int const s = popc(c & PLANE0) + 2 * popc(c & PLANE1) + 4 * popc(c & PLANE2)
+            + 8 * popc(c & PLANE3) + 16 * popc(c & PLANE4);
         hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - (s + n * n);
         if(x > 0) {
-            int const d = h[x] - h[x - 1];
+            int const d = hx - hPrev;
             sv += d * d;
             if(KIND == MODEL_F16) bump += d < 0 ? -d : d;
+            int const lim = hPrev + 3; // wells_t (v_board_metrics.hpp:204-225) for column x-1
+            bool const leftOk = x == 1 ? true : hPrev2 >= lim;
+            wells += (leftOk && hx >= lim) ? 1 : 0;
         }
         if(KIND == MODEL_F16) {
-            minH = h[x] < minH ? h[x] : minH;
-            agg += h[x];
-            uint32_t const yc = col | FLOOR;
+            minH = hx < minH ? hx : minH;
+            agg += hx;
+            uint32_t const yc = c | FLOOR;
             ytr += popc((yc ^ (yc >> 1)) & FIELD);
         }
+        hPrev2 = hPrev;
+        hPrev = hx;
     }
-#pragma unroll
-    for(int x = 0; x < WIDTH; ++x) {
-        int const lim = h[x] + 3;
-        bool const l = x == 0 ? true : h[x - 1] >= lim;
-        bool const r = x == WIDTH - 1 ? true : h[x + 1] >= lim;
-        bool const edgeL = x == 0 ? h[1] >= lim : true;
-        bool const edgeR = x == WIDTH - 1 ? h[WIDTH - 2] >= lim : true;
-        wells += (l && r && edgeL && edgeR) ? 1 : 0;
-    }
+    wells += hPrev2 >= hPrev + 3 ? 1 : 0; // column 9 against its single neighbour
     float f[16];
 #pragma unroll
     for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
@@ -479,48 +484,49 @@ __device__ inline float leaf_value(uint32_t const (&b)[WIDTH], uint32_t clears, 
     }
 }
 
-// fit (board2.hpp:306-320) + placed (:322-327) + clearLines (:333-342) on a register board.
-// returns false when the piece collides at spawn; otherwise nb = resting board after clears.
-__device__ inline bool apply_placement(uint32_t const* __restrict__ parent, LutEntry e, uint32_t (&nb)[WIDTH],
-                                       int& cleared) {
-    uint32_t const cols = e.cols;
-    int const col0 = e.col0;
+// fit (board2.hpp:306-320) over the piece's occupied columns only: a column without a piece nibble can
+// neither collide nor bound the drop. Board words past the piece's width are read but never used (their
+// nibble is 0; the reads stay inside the LDS state).
+__device__ inline bool fit_span(uint32_t const* parent, uint32_t cols, int col0, int& minDrop) {
     uint32_t hit = 0;
-    int minDrop = HEIGHT;
+    int md = HEIGHT;
 #pragma unroll
-    for(int x = 0; x < WIDTH; ++x) {
-        uint32_t const bc = parent[x];
-        nb[x] = bc;
-        uint32_t const nib = column_of(cols, col0, x);
+    for(int c = 0; c < 4; ++c) {
+        uint32_t const nib = (cols >> (4 * c)) & 0xFu;
+        uint32_t const bc = parent[col0 + c];
         hit |= bc & nib;
         int const low = bit_width(nib) - 1;
         uint32_t const mask = ~((1u << (low + 1)) - 1);
         int const drop = ctz_nz((bc & mask) | FLOOR) - low - 1;
-        minDrop = (nib != 0 && drop < minDrop) ? drop : minDrop;
+        md = (nib != 0 && drop < md) ? drop : md;
     }
-    if(hit != 0) return false;
+    minDrop = md;
+    return hit == 0;
+}
+
+// placed (board2.hpp:322-327) then clearLines (:333-342, :218-234) into the lane's scratch board;
+// returns the number of rows cleared
+__device__ inline int place_into(uint32_t const* parent, uint32_t cols, int col0, int drop, uint32_t* col) {
     uint32_t full = FIELD;
 #pragma unroll
     for(int x = 0; x < WIDTH; ++x) {
-        nb[x] |= column_of(cols, col0, x) << minDrop;
-        full &= nb[x];
+        uint32_t const v = parent[x] | (column_of(cols, col0, x) << drop);
+        col[x * SCR_STRIDE] = v;
+        full &= v;
     }
-    cleared = 0;
-    if(full != 0) {
-        int const n = popc(full);
-        cleared = n;
-#pragma unroll
-        for(int x = 0; x < WIDTH; ++x) {
-            uint32_t col = nb[x], rows = full;
-            for(int i = 0; i < n; ++i) {
-                uint32_t const row = rows & (~rows + 1);
-                col = ((col & (row - 1)) << 1) | (col & ~((row << 1) - 1));
-                rows ^= row;
-            }
-            nb[x] = col;
+    if(full == 0) return 0;
+    int const n = popc(full);
+#pragma unroll 1
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t c = col[x * SCR_STRIDE], rows = full;
+        for(int i = 0; i < n; ++i) {
+            uint32_t const row = rows & (~rows + 1); // top-most remaining full row
+            c = ((c & (row - 1)) << 1) | (c & ~((row << 1) - 1));
+            rows ^= row;
         }
+        col[x * SCR_STRIDE] = c;
     }
-    return true;
+    return n;
 }
 
 __device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t::added (model_defs.hpp:108-110)
@@ -566,6 +572,7 @@ __device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, ui
     uint32_t const off = wave_excl_scan(n);
     L.segOff[s] = off;
     L.segInfo[s] = info;
+    for(uint32_t i = 0; i < n; ++i) L.candSeg[off + i] = static_cast<uint16_t>((s << 6) | i);
     uint32_t const total = __builtin_amdgcn_readlane(off + n, 63);
     wave_sync();
     return total;
@@ -586,7 +593,7 @@ __device__ inline void group_sync() {
 //  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
 //    least one legal placement (expand_leaves, beam.hpp:407-443).
 template<int KIND, int STAGE, int K, int G>
-__device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_t wv, uint64_t* mask,
+__device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_t wv, uint32_t* col, uint64_t* mask,
                                      uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
@@ -594,24 +601,22 @@ __device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per chunk
         uint32_t const f = base + lane;
-        uint32_t s = 0;
-#pragma unroll
-        for(uint32_t step = 32; step > 0; step >>= 1)
-            if(L.segOff[s + step] <= f) s += step;
-        uint32_t const info = L.segInfo[s];
-        uint32_t const i = f - L.segOff[s];
-        uint32_t const piece = (info >> 8) & 7u;
-        uint32_t const pb = info & 0x7Fu;
         bool legal = false;
-        uint32_t score = 0;
+        uint32_t score = 0, s = 0, i = 0;
         if(f < total) {
+            uint32_t const si = L.candSeg[f];
+            s = si >> 6;
+            i = si & 63u;
+            uint32_t const info = L.segInfo[s];
+            uint32_t const piece = (info >> 8) & 7u;
+            uint32_t const* parent = &L.boards[(info & 0x7Fu) * WIDTH];
             LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
-            uint32_t nb[WIDTH];
-            int cleared = 0;
-            legal = apply_placement(&L.boards[pb * WIDTH], e, nb, cleared);
+            int drop = 0;
+            legal = fit_span(parent, e.cols, e.col0, drop);
             if(legal) {
-                uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[pb], cleared);
-                score = ordered_bits(leaf_value<KIND>(nb, clears, ((info >> 11) & 1u) != 0, wl));
+                int const cleared = place_into(parent, e.cols, e.col0, drop, col);
+                uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
+                score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl));
             }
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
@@ -685,7 +690,7 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
 // Re-applies the selected placement of rank j (lane j < count) to its parent and writes kept node j of
 // frontier buffer `dst` (seed_frontier / advance_frontier, beam.hpp:307-329, 379-403).
 __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
-                                      uint32_t dst, bool root) {
+                                      uint32_t dst, bool root, uint32_t* col) {
     uint32_t const j = lane_id();
     if(j >= count) return;
     uint32_t const slot = 0xFFFFFFFFu - static_cast<uint32_t>(key);
@@ -706,12 +711,13 @@ __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, 
     }
     uint32_t const piece = L.slotPiece[c_tab.tree[ti][layer][node].slot];
     LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
-    uint32_t nb[WIDTH];
-    int cleared = 0;
-    apply_placement(&L.boards[parentBoard * WIDTH], e, nb, cleared);
+    uint32_t const* parent = &L.boards[parentBoard * WIDTH];
+    int drop = 0;
+    fit_span(parent, e.cols, e.col0, drop); // a selected placement always fits
+    int const cleared = place_into(parent, e.cols, e.col0, drop, col);
     uint32_t const d = dst * 32 + j;
 #pragma unroll
-    for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = nb[x];
+    for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = col[x * SCR_STRIDE];
     L.clears[d] = hist_added(parentClears, cleared);
     L.score[d] = static_cast<uint32_t>(key >> 32);
     L.info[d] = node | (rank << 8);
@@ -733,7 +739,7 @@ __device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, ui
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
 // ends (no move, or a top-out).
 template<int KIND, int G>
-__device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uint32_t& nLegal) {
+__device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64_t& nApply, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
@@ -752,13 +758,13 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uin
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
     nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, total, w, wv, nullptr, nLegal);
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, total, w, wv, col, nullptr, nLegal);
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
     if(wv == 0) {
         if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
-        write_frontier(L, top, count, ti, 0, 0, 0, true);
+        write_frontier(L, top, count, ti, 0, 0, 0, true, col);
     }
     group_sync<G>();
 
@@ -770,8 +776,8 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uin
         total = stage_segments<STAGE_MID, G>(L, ti, level, count, cur, wv);
         nApply += total;
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, total, w, wv, &mask, nLegal)
-                         : run_stage<KIND, STAGE_MID, K2, G>(L, total, w, wv, &mask, nLegal);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, total, w, wv, col, &mask, nLegal)
+                         : run_stage<KIND, STAGE_MID, K2, G>(L, total, w, wv, col, &mask, nLegal);
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
@@ -782,7 +788,7 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uin
         fb = wave_max_u64(fb);
         bestKey = fb > bestKey ? fb : bestKey;
         uint32_t const next = __popcll(__ballot(top != 0));
-        if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false);
+        if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false, col);
         group_sync<G>();
         cur ^= 1u;
         count = next;
@@ -792,7 +798,7 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uin
     total = stage_segments<STAGE_LEAF, G>(L, ti, 3, count, cur, wv);
     nApply += total;
     mask = 0;
-    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, total, w, wv, &mask, nLegal);
+    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, total, w, wv, col, &mask, nLegal);
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
@@ -831,13 +837,21 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint64_t& nApply, uin
 // ---------------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------------
+// Waves per SIMD the register allocation is held to: the largest occupancy each network's code fits
+// without spilling registers to memory (V11: <= 102 VGPRs, 16-32-32-1: <= 128).
+template<int KIND>
+constexpr int WAVES_PER_SIMD = KIND == MODEL_V11 ? 5 : 4;
+
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
 template<int KIND, int G>
-__global__ __launch_bounds__(64 * G) void beam_eval_kernel(EvalArgs a) {
+__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND>, 8)))
+void beam_eval_kernel(EvalArgs a) {
     __shared__ GameLds L;
+    __shared__ uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
+    uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
     uint32_t item = a.persistent ? 0u : blockIdx.x;
     for(;;) {
         if(a.persistent) {
@@ -858,7 +872,7 @@ __global__ __launch_bounds__(64 * G) void beam_eval_kernel(EvalArgs a) {
         uint32_t nLegal = 0;
         for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
             if(uniform(L.eng.gameOver) != 0) break;
-            if(!play_move<KIND, G>(L, w, wv, nApply, nLegal)) break;
+            if(!play_move<KIND, G>(L, w, wv, col, nApply, nLegal)) break;
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
         group_sync<G>();
