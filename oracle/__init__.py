@@ -96,6 +96,14 @@ def lib():
             "ta3o_eval_host_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
                                          f32p, u32p, C.POINTER(Counters)]),
             "ta3o_play_trace": (C.c_int, [C.c_int, f32p, C.c_uint64, C.c_uint32, u8p, u32p, f32p]),
+            "ta3o_pruner_num_params": (C.c_int, []),
+            "ta3o_pruner_score": (C.c_float, [f32p, u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_pruner_loss_grad": (C.c_double, [f32p, u32p, C.c_int, C.c_double, C.c_int,
+                                                   C.POINTER(C.c_double)]),
+            "ta3o_eval_host_pruned_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
+                                                f32p, C.c_int, C.c_uint, f32p, u32p, C.POINTER(Counters)]),
+            "ta3o_trace_records": (C.c_uint64, [C.c_int, f32p, C.c_uint64, C.c_uint32, C.c_uint32, u32p,
+                                                C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -325,3 +333,49 @@ def play_trace(kind: int, w, seed: int, max_moves: int):
     n = lib().ta3o_play_trace(kind, _p(w, C.c_float), seed, max_moves, _p(moves, C.c_uint8),
                               _p(boards, C.c_uint32), C.byref(fit))
     return moves[:n * 4].reshape(n, 4), boards[:n * 10].reshape(n, 10), fit.value
+
+
+# --- placement-proposal network (pruner) ----------------------------------------------------------
+def pruner_num_params() -> int:
+    return lib().ta3o_pruner_num_params()
+
+
+def pruner_score(pw, cols, piece, o, x, ply) -> float:
+    pw = _f32(pw)
+    c = np.array(cols, dtype=np.uint32)
+    return lib().ta3o_pruner_score(_p(pw, C.c_float), _p(c, C.c_uint32), piece, o, x, ply)
+
+
+def pruner_loss_grad(pw, samples, pos_weight=1.0, rounded=True, with_grad=True):
+    """(loss, grad float64[P]) of the batch-mean weighted BCE over trace records [n, 11] uint32"""
+    pw = _f32(pw)
+    smp = np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1, 11)
+    g = np.zeros(pruner_num_params(), dtype=np.float64) if with_grad else None
+    loss = lib().ta3o_pruner_loss_grad(_p(pw, C.c_float), _p(smp, C.c_uint32), smp.shape[0], float(pos_weight),
+                                       int(rounded), _p(g, C.c_double) if with_grad else None)
+    return loss, g
+
+
+def eval_host_pruned_mt(kind, w, num_models, seeds, max_moves, pruner_w, k, ply_mask, threads=0):
+    w = _f32(w if len(w) else np.zeros(1))
+    pw = _f32(pruner_w)
+    s = np.ascontiguousarray(seeds, dtype=np.uint64)
+    items = num_models * len(s)
+    out = np.zeros(items, dtype=np.float32)
+    pl = np.zeros(items, dtype=np.uint32)
+    cnt = Counters()
+    if threads <= 0:
+        threads = os.cpu_count() or 1
+    lib().ta3o_eval_host_pruned_mt(kind, _p(w, C.c_float), num_models, _p(s, C.c_uint64), len(s), max_moves,
+                                   threads, _p(pw, C.c_float), k, ply_mask, _p(out, C.c_float), _p(pl, C.c_uint32),
+                                   C.byref(cnt))
+    return out, pl, cnt
+
+
+def trace_records(kind, w, seed, max_moves, every):
+    """all trace records of one full-beam game: uint32 [n, 11]"""
+    w = _f32(w if len(w) else np.zeros(1))
+    n = lib().ta3o_trace_records(kind, _p(w, C.c_float), seed, max_moves, every, None, 0)
+    out = np.zeros(n * 11, dtype=np.uint32)
+    lib().ta3o_trace_records(kind, _p(w, C.c_float), seed, max_moves, every, _p(out, C.c_uint32), n)
+    return out.reshape(-1, 11)

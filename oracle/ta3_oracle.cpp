@@ -697,6 +697,154 @@ struct Counters {
     }
 };
 
+
+// ---------------------------------------------------------------------------------------------------
+// placement-proposal network (build-defined; no reference counterpart)
+// input 265: board slots 24c + r = bit (3 + r) of column c (r < 20; r in [20, 24) always 0), then one-hots:
+// piece (240 + p), rotation (247 + o), column (251 + x + 2), ply (262 + ply - 1). 265 -> 64 ReLU -> 64 ReLU -> 1.
+// layout: W1[64][265], b1[64], W2[64][64], b2[64], W3[64], b3 (21,249 floats). W1 and W2 are used rounded to
+// bfloat16 (round to nearest even), the first hidden layer is rounded to bfloat16 before layer 2.
+// ---------------------------------------------------------------------------------------------------
+constexpr int PR_IN = 265, PR_H = 64;
+constexpr int PR_W1 = 0, PR_B1 = PR_W1 + PR_H * PR_IN, PR_W2 = PR_B1 + PR_H, PR_B2 = PR_W2 + PR_H * PR_H,
+              PR_W3 = PR_B2 + PR_H, PR_B3 = PR_W3 + PR_H, PR_PARAMS = PR_B3 + 1;
+
+inline float bf16_round(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+// active input indices of a candidate (board bits then the four one-hots); returns the count
+int pruner_inputs(Board const& b, int piece, int o, int x, int ply, int* idx) {
+    int n = 0;
+    for(int c = 0; c < WIDTH; ++c)
+        for(int r = 0; r < 20; ++r)
+            if((b.c[c] >> (3 + r)) & 1u) idx[n++] = 24 * c + r;
+    idx[n++] = 240 + piece;
+    idx[n++] = 247 + o;
+    idx[n++] = 251 + x + 2;
+    idx[n++] = 262 + ply - 1;
+    return n;
+}
+
+// inference value with exact (double) accumulation; the device accumulates in float32 in its own order
+float pruner_score(float const* w, Board const& b, int piece, int o, int x, int ply) {
+    int idx[PR_IN];
+    int const n = pruner_inputs(b, piece, o, x, ply, idx);
+    float h1[PR_H];
+    for(int u = 0; u < PR_H; ++u) {
+        double v = w[PR_B1 + u];
+        for(int j = 0; j < n; ++j) v += bf16_round(w[PR_W1 + u * PR_IN + idx[j]]);
+        float const f = static_cast<float>(v);
+        h1[u] = bf16_round(f > 0.0f ? f : 0.0f);
+    }
+    double s = w[PR_B3];
+    for(int v = 0; v < PR_H; ++v) {
+        double a = w[PR_B2 + v];
+        for(int u = 0; u < PR_H; ++u) a += static_cast<double>(h1[u]) * bf16_round(w[PR_W2 + v * PR_H + u]);
+        double const h2 = a > 0.0 ? a : 0.0;
+        s += h2 * w[PR_W3 + v];
+    }
+    return static_cast<float>(s);
+}
+
+struct PrunerRef {
+    float const* w = nullptr; // PR_PARAMS weights; null = no pruning
+    int k = 34;               // placements kept per list at a pruned ply
+    unsigned plyMask = 0;     // bit p: prune ply p (1, 2 = middle levels, 3 = leaves)
+};
+
+// trace record: parent board (10 words) + meta = piece | o << 3 | (x + 2) << 5 | ply << 9 | label << 11
+struct TraceOut {
+    std::vector<uint32_t>* out = nullptr;
+    bool on = false;
+    void add(Board const& b, int piece, int o, int x, int ply, bool label) {
+        for(int c = 0; c < WIDTH; ++c) out->push_back(b.c[c]);
+        out->push_back(static_cast<uint32_t>(piece) | static_cast<uint32_t>(o) << 3 | static_cast<uint32_t>(x + 2) << 5
+                       | static_cast<uint32_t>(ply) << 9 | (label ? 1u << 11 : 0u));
+    }
+};
+
+// keeps the k best legal placements of one list by pruner score (score descending, index ascending)
+void prune_list(PrunerRef const& pr, Board const& parent, int piece, int ply, uint8_t const* legal, int n,
+                uint8_t* keep) {
+    float sc[MAX_PLACEMENTS];
+    int order[MAX_PLACEMENTS];
+    int m = 0;
+    for(int i = 0; i < n; ++i) {
+        keep[i] = 0;
+        if(!legal[i]) continue;
+        Placement const pl = LUT.nth(piece, i);
+        sc[i] = pruner_score(pr.w, parent, piece, pl.o, pl.x, ply);
+        order[m++] = i;
+    }
+    std::stable_sort(order, order + m, [&](int a, int b) { return sc[a] > sc[b]; });
+    for(int j = 0; j < m && j < pr.k; ++j) keep[order[j]] = 1;
+}
+
+// Training reference (float64): batch-mean weighted BCE on the logit and its gradient. `rounded` applies the
+// inference roundings (bfloat16 W1, W2 and first hidden layer) with straight-through gradients; without them
+// the loss is smooth and the gradient can be checked by finite differences.
+// sample = parent board (10 words) + meta (trace record layout); weight = posWeight for label 1, else 1.
+double pruner_loss_grad(float const* w, uint32_t const* samples, int n, double posWeight, bool rounded, double* grad) {
+    auto rw = [&](float v) { return rounded ? static_cast<double>(bf16_round(v)) : static_cast<double>(v); };
+    if(grad) std::fill(grad, grad + PR_PARAMS, 0.0);
+    double loss = 0.0;
+    int idx[PR_IN];
+    double pre1[PR_H], h1[PR_H], pre2[PR_H], h2[PR_H], dpre2[PR_H];
+    for(int t = 0; t < n; ++t) {
+        uint32_t const* rec = samples + 11 * static_cast<size_t>(t);
+        Board b;
+        std::memcpy(b.c, rec, 40);
+        uint32_t const meta = rec[10];
+        int const piece = meta & 7, o = (meta >> 3) & 3, x = static_cast<int>((meta >> 5) & 15) - 2,
+                  ply = (meta >> 9) & 3;
+        double const y = (meta >> 11) & 1u;
+        int const m = pruner_inputs(b, piece, o, x, ply, idx);
+        for(int u = 0; u < PR_H; ++u) {
+            double v = w[PR_B1 + u];
+            for(int j = 0; j < m; ++j) v += rw(w[PR_W1 + u * PR_IN + idx[j]]);
+            pre1[u] = v;
+            double const r = v > 0.0 ? v : 0.0;
+            h1[u] = rounded ? static_cast<double>(bf16_round(static_cast<float>(r))) : r;
+        }
+        double sc = w[PR_B3];
+        for(int v = 0; v < PR_H; ++v) {
+            double a = w[PR_B2 + v];
+            for(int u = 0; u < PR_H; ++u) a += h1[u] * rw(w[PR_W2 + v * PR_H + u]);
+            pre2[v] = a;
+            h2[v] = a > 0.0 ? a : 0.0;
+            sc += h2[v] * w[PR_W3 + v];
+        }
+        double const wt = y > 0.5 ? posWeight : 1.0;
+        double const sp = sc > 0 ? sc + std::log1p(std::exp(-sc)) : std::log1p(std::exp(sc)); // softplus
+        loss += wt * (sp - y * sc) / n;
+        if(!grad) continue;
+        double const sig = 1.0 / (1.0 + std::exp(-sc));
+        double const ds = wt * (sig - y) / n;
+        grad[PR_B3] += ds;
+        for(int v = 0; v < PR_H; ++v) {
+            grad[PR_W3 + v] += ds * h2[v];
+            dpre2[v] = pre2[v] > 0.0 ? ds * w[PR_W3 + v] : 0.0;
+            grad[PR_B2 + v] += dpre2[v];
+            for(int u = 0; u < PR_H; ++u) grad[PR_W2 + v * PR_H + u] += dpre2[v] * h1[u];
+        }
+        for(int u = 0; u < PR_H; ++u) {
+            double dh = 0.0;
+            for(int v = 0; v < PR_H; ++v) dh += dpre2[v] * rw(w[PR_W2 + v * PR_H + u]);
+            double const dp = pre1[u] > 0.0 ? dh : 0.0;
+            grad[PR_B1 + u] += dp;
+            for(int j = 0; j < m; ++j) grad[PR_W1 + u * PR_IN + idx[j]] += dp;
+        }
+    }
+    return loss;
+}
+
 struct BeamWorkspace {
     std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
     std::vector<uint8_t> hasChild;
@@ -705,7 +853,8 @@ struct BeamWorkspace {
 
 // search_move_beam (:483-570) at runtime depth = ks.size() + 1
 SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uint32_t> const& ks,
-                         BeamWorkspace& ws, Counters& cnt) {
+                         BeamWorkspace& ws, Counters& cnt, PrunerRef const& pr = PrunerRef{},
+                         TraceOut trace = TraceOut{}) {
     int const depth = static_cast<int>(ks.size()) + 1;
     if(g.gameOver) return {false, false, 0, 0, 0.0f};
 
@@ -775,9 +924,11 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     ws.topScores.assign(kMax, 0);
     for(int level = 1; level + 1 < depth; ++level) {
         Frontier const& F = ws.kept[cur];
-        // score_mid_level (:337-372)
+        // score_mid_level (:337-372); at a pruned ply only the pruner's top-k legal placements of a list are scored
         ws.midScores.assign(midSlots, 0u);
         ws.hasChild.assign(kMax, 0);
+        bool const pruneHere = pr.w != nullptr && ((pr.plyMask >> level) & 1u) != 0;
+        std::vector<uint8_t> legalMid(midSlots, 0);
         for(uint32_t k = 0; k < keptCount; ++k) {
             Node const& pn = tree.layers[level - 1][F.node[k]];
             for(int ci = pn.childBegin; ci < pn.childEnd; ++ci) {
@@ -785,17 +936,46 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 uint32_t const piece = slot[cn.slot];
                 bool const hI = heldIsI(cn);
                 size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
-                for(int i = 0; i < LUT.count(piece); ++i) {
-                    Step const s = apply(F.board[k], F.clears[k], LUT.nth(piece, i));
+                int const n = LUT.count(piece);
+                uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
+                int dropTo[MAX_PLACEMENTS];
+                for(int i = 0; i < n; ++i) {
+                    Fit const f = fit(F.board[k], LUT.nth(piece, i).shape);
                     ++cnt.applies;
-                    if(!s.legal) continue;
-                    ++cnt.legal;
-                    ws.midScores[base + i] = ordered_bits(model.value(s.clears, s.board, hI));
+                    legal[i] = f.legal ? 1 : 0;
+                    dropTo[i] = f.drop;
+                    keep[i] = legal[i];
+                    legalMid[base + i] = legal[i];
+                }
+                if(pruneHere) prune_list(pr, F.board[k], static_cast<int>(piece), level, legal, n, keep);
+                for(int i = 0; i < n; ++i) {
+                    if(!legal[i]) continue;
                     ws.hasChild[k] = 1;
+                    if(!keep[i]) continue;
+                    ++cnt.legal;
+                    Step const st = apply(F.board[k], F.clears[k], LUT.nth(piece, i));
+                    (void)dropTo;
+                    ws.midScores[base + i] = ordered_bits(model.value(st.clears, st.board, hI));
                 }
             }
         }
         uint32_t const nextCount = select_top(ws.midScores, ks[level], ws.topSlots.data(), ws.topScores.data());
+        if(trace.on) { // every legal candidate of the level; label = it survived the level's top-K
+            std::vector<uint8_t> survived(midSlots, 0);
+            for(uint32_t j = 0; j < nextCount; ++j) survived[ws.topSlots[j]] = 1;
+            for(uint32_t k = 0; k < keptCount; ++k) {
+                Node const& pn = tree.layers[level - 1][F.node[k]];
+                for(int ci = pn.childBegin; ci < pn.childEnd; ++ci) {
+                    uint32_t const piece = slot[tree.layers[level][ci].slot];
+                    size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
+                    for(int i = 0; i < LUT.count(piece); ++i)
+                        if(legalMid[base + i]) {
+                            Placement const pl = LUT.nth(piece, i);
+                            trace.add(F.board[k], static_cast<int>(piece), pl.o, pl.x, level, survived[base + i] != 0);
+                        }
+                }
+            }
+        }
         for(uint32_t k = 0; k < keptCount; ++k) // dead-end kept nodes become depth-`level` leaves (:549-551)
             if(!ws.hasChild[k]) consider(F.score[k], leafOrderCount + F.rootRank[k]);
         // advance_frontier (:379-403)
@@ -817,9 +997,11 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         keptCount = nextCount;
     }
 
-    // leaf stage: expand_leaves (:407-443)
+    // leaf stage: expand_leaves (:407-443); at a pruned ply only the pruner's top-k legal placements are scored
     {
         Frontier const& F = ws.kept[cur];
+        int const leafPly = depth - 1;
+        bool const pruneHere = pr.w != nullptr && ((pr.plyMask >> leafPly) & 1u) != 0;
         for(uint32_t j = 0; j < keptCount; ++j) {
             Node const& ln = tree.layers[depth - 2][F.node[j]];
             for(int ci = ln.childBegin; ci < ln.childEnd; ++ci) {
@@ -827,15 +1009,37 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 uint32_t const piece = slot[cn.slot];
                 bool const hI = heldIsI(cn);
                 uint32_t const base = (j * maxLast + (ci - ln.childBegin)) * MP;
-                bool expanded = false;
-                for(int i = 0; i < LUT.count(piece); ++i) {
-                    Step const s = apply(F.board[j], F.clears[j], LUT.nth(piece, i));
+                int const n = LUT.count(piece);
+                uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
+                for(int i = 0; i < n; ++i) {
+                    legal[i] = fit(F.board[j], LUT.nth(piece, i).shape).legal ? 1 : 0;
+                    keep[i] = legal[i];
                     ++cnt.applies;
-                    if(!s.legal) continue;
-                    ++cnt.legal;
-                    expanded = true;
-                    consider(ordered_bits(model.value(s.clears, s.board, hI)), base + i);
                 }
+                if(pruneHere) prune_list(pr, F.board[j], static_cast<int>(piece), leafPly, legal, n, keep);
+                bool expanded = false;
+                uint64_t listBest = 0;
+                int listArg = -1;
+                for(int i = 0; i < n; ++i) {
+                    if(!legal[i]) continue;
+                    expanded = true;
+                    if(!keep[i]) continue;
+                    ++cnt.legal;
+                    Step const s = apply(F.board[j], F.clears[j], LUT.nth(piece, i));
+                    uint32_t const sc = ordered_bits(model.value(s.clears, s.board, hI));
+                    consider(sc, base + i);
+                    uint64_t const key = pack_key(sc, base + i, orderCount);
+                    if(key > listBest) {
+                        listBest = key;
+                        listArg = i;
+                    }
+                }
+                if(trace.on) // label = the list's best leaf
+                    for(int i = 0; i < n; ++i)
+                        if(legal[i]) {
+                            Placement const pl = LUT.nth(piece, i);
+                            trace.add(F.board[j], static_cast<int>(piece), pl.o, pl.x, leafPly, i == listArg);
+                        }
                 if(!expanded) consider(F.score[j], base);
             }
         }
@@ -960,13 +1164,17 @@ const std::vector<uint32_t> REF_WIDTHS = {16, 32, 32}; // variation_sequences.hp
 
 // eval_host inner loop for one (model, seed) game (model_eval_host.cpp:33-52)
 float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWorkspace& ws, Counters& cnt,
-                uint8_t* moves = nullptr, uint32_t* boards = nullptr, int* nRecorded = nullptr) {
+                uint8_t* moves = nullptr, uint32_t* boards = nullptr, int* nRecorded = nullptr,
+                PrunerRef const& pr = PrunerRef{}, std::vector<uint32_t>* traceOut = nullptr, uint32_t traceEvery = 0) {
     Engine g;
     engine_reset(g, seed);
     Stats stats;
     int rec = 0;
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
-        SearchResult const r = search_beam(g, model, REF_WIDTHS, ws, cnt);
+        TraceOut tr;
+        tr.out = traceOut;
+        tr.on = traceOut != nullptr && traceEvery != 0 && i % traceEvery == 0;
+        SearchResult const r = search_beam(g, model, REF_WIDTHS, ws, cnt, pr, tr);
         if(!r.some) break;
         if(r.hold) engine_hold(g);
         int const cleared = engine_place(g, r.o, r.x);
@@ -1241,6 +1449,63 @@ int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves,
     float const f = play_game(m, seed, max_moves, ws, c, moves, boards, &rec);
     if(fitness) *fitness = f;
     return rec;
+}
+
+int ta3o_pruner_num_params(void) { return PR_PARAMS; }
+
+float ta3o_pruner_score(const float* w, const uint32_t cols[10], int piece, int o, int x, int ply) {
+    return pruner_score(w, board_from(cols), piece, o, x, ply);
+}
+
+double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, double pos_weight, int rounded,
+                             double* grad) {
+    return pruner_loss_grad(w, samples, n, pos_weight, rounded != 0, grad);
+}
+
+void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
+                              uint32_t max_moves, int threads, const float* pruner_w, int k, unsigned ply_mask,
+                              float* out, uint32_t* placements_out, ta3o_counters* counters) {
+    int const P = ta3o_num_params(kind);
+    uint64_t const items = static_cast<uint64_t>(num_models) * games;
+    std::atomic<uint64_t> next{0};
+    if(threads < 1) threads = 1;
+    PrunerRef pr;
+    pr.w = pruner_w;
+    pr.k = k;
+    pr.plyMask = ply_mask;
+    std::vector<Counters> perThread(static_cast<size_t>(threads));
+    auto worker = [&](int tid) {
+        BeamWorkspace ws;
+        for(;;) {
+            uint64_t const item = next.fetch_add(1);
+            if(item >= items) break;
+            uint64_t const mi = item / games, gi = item % games;
+            ModelRef const m{kind, w + mi * static_cast<uint64_t>(P > 0 ? P : 0)};
+            uint64_t const before = perThread[tid].placements;
+            out[item] = play_game(m, seeds[gi], max_moves, ws, perThread[tid], nullptr, nullptr, nullptr, pr);
+            if(placements_out) placements_out[item] = static_cast<uint32_t>(perThread[tid].placements - before);
+        }
+    };
+    std::vector<std::thread> pool;
+    for(int t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for(auto& t : pool) t.join();
+    Counters total;
+    for(auto const& c : perThread) total.add(c);
+    to_counters(total, counters);
+}
+
+uint64_t ta3o_trace_records(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint32_t every,
+                            uint32_t* out, uint64_t cap_records) {
+    ModelRef const m{kind, w};
+    BeamWorkspace ws;
+    Counters c;
+    std::vector<uint32_t> rec;
+    play_game(m, seed, max_moves, ws, c, nullptr, nullptr, nullptr, PrunerRef{}, &rec, every);
+    uint64_t const n = rec.size() / 11;
+    uint64_t const copy = n < cap_records ? n : cap_records;
+    if(out) std::memcpy(out, rec.data(), copy * 11 * sizeof(uint32_t));
+    return n;
 }
 
 } // extern "C"

@@ -113,6 +113,25 @@ void ta3o_eval_host_mt(int kind, const float* w, uint32_t num_models, const uint
 int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint8_t* moves,
                     uint32_t* boards, float* fitness);
 
+/* placement-proposal network (build-defined): 265 inputs -> 64 -> 64 -> 1, 21,249 parameters */
+int ta3o_pruner_num_params(void);
+/* inference value (exact accumulation) for a candidate (parent board, piece, orientation, x, ply 1..3) */
+float ta3o_pruner_score(const float* w, const uint32_t cols[10], int piece, int o, int x, int ply);
+/* batch-mean weighted BCE and its gradient (float64) over trace records (11 words each);
+ * rounded = apply the inference bfloat16 roundings (straight-through) */
+double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, double pos_weight, int rounded,
+                             double* grad);
+/* eval_host_mt with the pruned beam: at plies in ply_mask (bit 1, 2: middle levels, bit 3: leaves) only
+ * the pruner's top-k legal placements of every list are scored by the value network */
+void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
+                              uint32_t max_moves, int threads, const float* pruner_w, int k, unsigned ply_mask,
+                              float* out, uint32_t* placements_out, ta3o_counters* counters);
+/* trace records of one full-beam game: every legal candidate of plies 1..3 of every `every`-th move:
+ * 10 board words + meta (piece | o << 3 | (x+2) << 5 | ply << 9 | label << 11); returns the count
+ * (copies at most cap_records) */
+uint64_t ta3o_trace_records(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint32_t every,
+                            uint32_t* out, uint64_t cap_records);
+
 #ifdef __cplusplus
 }
 #endif
