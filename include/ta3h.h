@@ -105,6 +105,58 @@ int ta3h_feature_norms(float* out, uint32_t n);
  * No reference counterpart. */
 void ta3h_tables(int8_t* lut_ox, uint8_t* lut_off, uint8_t* tree, uint8_t* tree_count);
 
+/* ---- learned pruner (placement-proposal network) and its training traces -------------------------------
+ * No reference counterpart: the reference scores every placement of every kept node (ai/search/beam.hpp:337-372,
+ * 407-443). The network: 265 inputs (board slot 24c + r = bit 3 + r of column c, r < 20; one-hots piece 240 + p,
+ * rotation 247 + o, column 251 + x + 2, ply 262 + ply - 1) -> 64 ReLU -> 64 ReLU -> 1 logit, FP32 master weights
+ * W1[64][265], b1[64], W2[64][64], b2[64], W3[64], b3. Trace record / training sample: 11 words = parent board
+ * (10 columns) + meta = piece | o << 3 | (x + 2) << 5 | ply << 9 | label << 11. */
+
+/* Master parameters of the pruner (21,249). */
+int ta3h_pruner_num_params(void);
+
+/* Grid exponents {e1, e2, e3, e4, d} of the integer inference form of weights w with first-hidden-layer range
+ * act_range (host only). */
+int ta3h_pruner_quant_info(const float* w, float act_range, int* info);
+
+/* Pruned beam for the following launches: at plies whose bit is set in ply_mask (bit 1, 2: middle levels, bit 3:
+ * leaves) only the k (1..34) best legal placements of every (kept node, piece) list by pruner logit are scored by
+ * the value network. w = NULL restores the full beam. k = 34 reproduces the full beam exactly. Pruned launches
+ * run 4 waves per game and record no traces. */
+int ta3h_set_pruner(ta3h_evaluator* ev, const float* w, float act_range, uint32_t k, uint32_t ply_mask);
+
+/* Full-beam traces for the following launches: every legal candidate of plies 1..3 of every `every`-th move
+ * (move index % every == 0) of every game, labelled 1 if the level kept it (plies 1, 2) or it is its list's
+ * best leaf (ply 3). Up to cap_records records are kept per launch, in no particular order. cap_records = 0
+ * turns traces off. Trace launches run 4 waves per game; results are the full beam's. */
+int ta3h_set_trace(ta3h_evaluator* ev, uint64_t cap_records, uint32_t every);
+
+/* Copies min(produced, capacity set by ta3h_set_trace, cap_records) records of the last launch to out
+ * (NULL: none; *copied gets the number) and reports the records produced (which may exceed the capacity; the
+ * excess was dropped). */
+int ta3h_fetch_trace(ta3h_evaluator* ev, uint32_t* out, uint64_t cap_records, uint64_t* produced, uint64_t* copied);
+
+/* Batch-mean weighted binary cross-entropy of n samples (weight pos_weight for label 1) and its gradient with
+ * respect to the master weights, with bfloat16 W1, W2 and first hidden layer in the forward pass
+ * (straight-through). samples = NULL uses the evaluator's trace buffer. */
+int ta3h_pruner_grad(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint32_t n, float pos_weight,
+                     float* grad, float* loss);
+
+/* Adam (beta 0.9, 0.999, eps 1e-8, fresh moments) over `epochs` shuffled passes of n samples in mini-batches
+ * of `batch`, updating w in place; epoch_loss[e] (optional) = mean training loss of epoch e. samples = NULL
+ * trains on the evaluator's trace buffer without a host copy. */
+int ta3h_pruner_train(ta3h_evaluator* ev, float* w, const uint32_t* samples, uint64_t n, uint32_t epochs,
+                      uint32_t batch, float lr, float pos_weight, uint64_t seed, float* epoch_loss);
+
+/* First hidden layer (after ReLU, bfloat16 as trained) of n samples: out[n][64]; used to calibrate
+ * act_range. */
+int ta3h_pruner_hidden(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint32_t n, float* out);
+
+/* Inference logits of n samples as the pruned search computes them (integer grids on BF16 MFMA); NaN for a
+ * record whose (rotation, column) is no placement of its piece. */
+int ta3h_pruner_logits(ta3h_evaluator* ev, const float* w, float act_range, const uint32_t* samples, uint32_t n,
+                       float* out);
+
 #ifdef __cplusplus
 }
 #endif

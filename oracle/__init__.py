@@ -97,11 +97,14 @@ def lib():
                                          f32p, u32p, C.POINTER(Counters)]),
             "ta3o_play_trace": (C.c_int, [C.c_int, f32p, C.c_uint64, C.c_uint32, u8p, u32p, f32p]),
             "ta3o_pruner_num_params": (C.c_int, []),
-            "ta3o_pruner_score": (C.c_float, [f32p, u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_pruner_score": (C.c_float, [f32p, C.c_float, u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ta3o_pruner_scores": (None, [f32p, C.c_float, u32p, C.c_int, f32p]),
+            "ta3o_pruner_quant_info": (None, [f32p, C.c_float, C.POINTER(C.c_int)]),
             "ta3o_pruner_loss_grad": (C.c_double, [f32p, u32p, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(C.c_double)]),
             "ta3o_eval_host_pruned_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
-                                                f32p, C.c_int, C.c_uint, f32p, u32p, C.POINTER(Counters)]),
+                                                f32p, C.c_float, C.c_int, C.c_uint, f32p, u32p,
+                                                C.POINTER(Counters)]),
             "ta3o_trace_records": (C.c_uint64, [C.c_int, f32p, C.c_uint64, C.c_uint32, C.c_uint32, u32p,
                                                 C.c_uint64]),
         }
@@ -340,10 +343,29 @@ def pruner_num_params() -> int:
     return lib().ta3o_pruner_num_params()
 
 
-def pruner_score(pw, cols, piece, o, x, ply) -> float:
+def pruner_score(pw, act_range, cols, piece, o, x, ply) -> float:
+    """integer-grid inference logit of one candidate"""
     pw = _f32(pw)
     c = np.array(cols, dtype=np.uint32)
-    return lib().ta3o_pruner_score(_p(pw, C.c_float), _p(c, C.c_uint32), piece, o, x, ply)
+    return lib().ta3o_pruner_score(_p(pw, C.c_float), float(act_range), _p(c, C.c_uint32), piece, o, x, ply)
+
+
+def pruner_scores(pw, act_range, samples):
+    """integer-grid inference logits of trace records [n, 11]"""
+    pw = _f32(pw)
+    smp = np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1, 11)
+    out = np.zeros(smp.shape[0], dtype=np.float32)
+    lib().ta3o_pruner_scores(_p(pw, C.c_float), float(act_range), _p(smp, C.c_uint32), smp.shape[0],
+                             _p(out, C.c_float))
+    return out
+
+
+def pruner_quant_info(pw, act_range):
+    """grid exponents (e1, e2, e3, e4, d) of the quantized network"""
+    pw = _f32(pw)
+    info = (C.c_int * 5)()
+    lib().ta3o_pruner_quant_info(_p(pw, C.c_float), float(act_range), info)
+    return tuple(info)
 
 
 def pruner_loss_grad(pw, samples, pos_weight=1.0, rounded=True, with_grad=True):
@@ -356,7 +378,7 @@ def pruner_loss_grad(pw, samples, pos_weight=1.0, rounded=True, with_grad=True):
     return loss, g
 
 
-def eval_host_pruned_mt(kind, w, num_models, seeds, max_moves, pruner_w, k, ply_mask, threads=0):
+def eval_host_pruned_mt(kind, w, num_models, seeds, max_moves, pruner_w, act_range, k, ply_mask, threads=0):
     w = _f32(w if len(w) else np.zeros(1))
     pw = _f32(pruner_w)
     s = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -367,7 +389,8 @@ def eval_host_pruned_mt(kind, w, num_models, seeds, max_moves, pruner_w, k, ply_
     if threads <= 0:
         threads = os.cpu_count() or 1
     lib().ta3o_eval_host_pruned_mt(kind, _p(w, C.c_float), num_models, _p(s, C.c_uint64), len(s), max_moves,
-                                   threads, _p(pw, C.c_float), k, ply_mask, _p(out, C.c_float), _p(pl, C.c_uint32),
+                                   threads, _p(pw, C.c_float), float(act_range), k, ply_mask, _p(out, C.c_float),
+                                   _p(pl, C.c_uint32),
                                    C.byref(cnt))
     return out, pl, cnt
 

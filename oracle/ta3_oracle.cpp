@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <memory>
 #include <vector>
 
 namespace {
@@ -732,29 +733,86 @@ int pruner_inputs(Board const& b, int piece, int o, int x, int ply, int* idx) {
     return n;
 }
 
-// inference value with exact (double) accumulation; the device accumulates in float32 in its own order
-float pruner_score(float const* w, Board const& b, int piece, int o, int x, int ply) {
+// Inference form: an integer network held on power-of-two grids (each grid's exponent is the largest that keeps
+// the layer's largest magnitude inside its integer range), so every sum is an exact integer:
+//   N1 = rne(W1 2^e1), Nb1 = rne(b1 2^e1) (|N1| <= 255, |Nb1| <= 2^20), e1 from max |W1| <= 255
+//   P  = sum over active inputs of N1 + Nb1                         (first-layer pre-activation, |P| < 2^21)
+//   c  = 0 if P <= 0, else min(255, round-half-up(P 2^-d)), d = e1 - e2 clamped to [-8, 24], e2 from act_range
+//   N2 = rne(W2 2^e3), Nb2 = rne(b2 2^(e2 + e3)) (|N2| <= 255, |Nb2| <= 2^22), e3 from max |W2| <= 255
+//   h2 = max(0, sum_u c[u] N2[v][u] + Nb2[v])                      (|.| < 2^23)
+//   N3 = rne(W3 2^e4), e4 from max |W3| <= 32767
+//   logit = float(sum_v h2[v] N3[v])                              (exact integer below 2^45, rounded once)
+// b3 shifts every logit equally and is dropped. act_range is the calibrated upper range of the first hidden
+// layer in float units.
+struct PrunerQ {
+    int32_t n1[PR_H][PR_IN];
+    int32_t nb1[PR_H];
+    int32_t n2[PR_H][PR_H];
+    int32_t nb2[PR_H];
+    int32_t n3[PR_H];
+    int d;
+    int e1, e2, e3, e4;
+};
+
+// largest e with m 2^e <= lim (0 when m is 0)
+int grid_exponent(double m, double lim) {
+    if(!(m > 0.0)) return 0;
+    int e = 0;
+    while(std::ldexp(m, e) > lim) --e;
+    while(std::ldexp(m, e + 1) <= lim) ++e;
+    return e;
+}
+
+int32_t rne_scaled(double v, int e, int32_t lim) {
+    double const r = std::nearbyint(std::ldexp(v, e)); // default rounding mode: nearest, ties to even
+    return static_cast<int32_t>(std::min<double>(std::max<double>(r, -lim), lim));
+}
+
+void pruner_quantize(float const* w, float actRange, PrunerQ& q) {
+    double m1 = 0, m2 = 0, m3 = 0;
+    for(int i = 0; i < PR_H * PR_IN; ++i) m1 = std::max(m1, std::fabs(static_cast<double>(w[PR_W1 + i])));
+    for(int i = 0; i < PR_H * PR_H; ++i) m2 = std::max(m2, std::fabs(static_cast<double>(w[PR_W2 + i])));
+    for(int i = 0; i < PR_H; ++i) m3 = std::max(m3, std::fabs(static_cast<double>(w[PR_W3 + i])));
+    q.e1 = grid_exponent(m1, 255.0);
+    q.e2 = grid_exponent(actRange, 255.0);
+    q.e3 = grid_exponent(m2, 255.0);
+    q.e4 = grid_exponent(m3, 32767.0);
+    q.d = std::min(24, std::max(-8, q.e1 - q.e2));
+    for(int u = 0; u < PR_H; ++u) {
+        for(int k = 0; k < PR_IN; ++k) q.n1[u][k] = rne_scaled(w[PR_W1 + u * PR_IN + k], q.e1, 255);
+        q.nb1[u] = rne_scaled(w[PR_B1 + u], q.e1, 1 << 20);
+        for(int i = 0; i < PR_H; ++i) q.n2[u][i] = rne_scaled(w[PR_W2 + u * PR_H + i], q.e3, 255);
+        q.nb2[u] = rne_scaled(w[PR_B2 + u], q.e2 + q.e3, 1 << 22);
+        q.n3[u] = rne_scaled(w[PR_W3 + u], q.e4, 32767);
+    }
+}
+
+int32_t pruner_code(int32_t p, int d) {
+    if(p <= 0) return 0;
+    int64_t c = d > 0 ? (static_cast<int64_t>(p) + (int64_t{1} << (d - 1))) >> d : static_cast<int64_t>(p) << -d;
+    return static_cast<int32_t>(std::min<int64_t>(c, 255));
+}
+
+float pruner_score(PrunerQ const& q, Board const& b, int piece, int o, int x, int ply) {
     int idx[PR_IN];
     int const n = pruner_inputs(b, piece, o, x, ply, idx);
-    float h1[PR_H];
+    int32_t c[PR_H];
     for(int u = 0; u < PR_H; ++u) {
-        double v = w[PR_B1 + u];
-        for(int j = 0; j < n; ++j) v += bf16_round(w[PR_W1 + u * PR_IN + idx[j]]);
-        float const f = static_cast<float>(v);
-        h1[u] = bf16_round(f > 0.0f ? f : 0.0f);
+        int32_t p = q.nb1[u];
+        for(int j = 0; j < n; ++j) p += q.n1[u][idx[j]];
+        c[u] = pruner_code(p, q.d);
     }
-    double s = w[PR_B3];
+    int64_t s = 0;
     for(int v = 0; v < PR_H; ++v) {
-        double a = w[PR_B2 + v];
-        for(int u = 0; u < PR_H; ++u) a += static_cast<double>(h1[u]) * bf16_round(w[PR_W2 + v * PR_H + u]);
-        double const h2 = a > 0.0 ? a : 0.0;
-        s += h2 * w[PR_W3 + v];
+        int64_t a = q.nb2[v];
+        for(int u = 0; u < PR_H; ++u) a += static_cast<int64_t>(c[u]) * q.n2[v][u];
+        if(a > 0) s += a * q.n3[v];
     }
-    return static_cast<float>(s);
+    return static_cast<float>(static_cast<double>(s));
 }
 
 struct PrunerRef {
-    float const* w = nullptr; // PR_PARAMS weights; null = no pruning
+    PrunerQ const* q = nullptr; // quantized network; null = no pruning
     int k = 34;               // placements kept per list at a pruned ply
     unsigned plyMask = 0;     // bit p: prune ply p (1, 2 = middle levels, 3 = leaves)
 };
@@ -780,7 +838,7 @@ void prune_list(PrunerRef const& pr, Board const& parent, int piece, int ply, ui
         keep[i] = 0;
         if(!legal[i]) continue;
         Placement const pl = LUT.nth(piece, i);
-        sc[i] = pruner_score(pr.w, parent, piece, pl.o, pl.x, ply);
+        sc[i] = pruner_score(*pr.q, parent, piece, pl.o, pl.x, ply);
         order[m++] = i;
     }
     std::stable_sort(order, order + m, [&](int a, int b) { return sc[a] > sc[b]; });
@@ -927,7 +985,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         // score_mid_level (:337-372); at a pruned ply only the pruner's top-k legal placements of a list are scored
         ws.midScores.assign(midSlots, 0u);
         ws.hasChild.assign(kMax, 0);
-        bool const pruneHere = pr.w != nullptr && ((pr.plyMask >> level) & 1u) != 0;
+        bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> level) & 1u) != 0;
         std::vector<uint8_t> legalMid(midSlots, 0);
         for(uint32_t k = 0; k < keptCount; ++k) {
             Node const& pn = tree.layers[level - 1][F.node[k]];
@@ -1001,7 +1059,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     {
         Frontier const& F = ws.kept[cur];
         int const leafPly = depth - 1;
-        bool const pruneHere = pr.w != nullptr && ((pr.plyMask >> leafPly) & 1u) != 0;
+        bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> leafPly) & 1u) != 0;
         for(uint32_t j = 0; j < keptCount; ++j) {
             Node const& ln = tree.layers[depth - 2][F.node[j]];
             for(int ci = ln.childBegin; ci < ln.childEnd; ++ci) {
@@ -1453,8 +1511,31 @@ int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves,
 
 int ta3o_pruner_num_params(void) { return PR_PARAMS; }
 
-float ta3o_pruner_score(const float* w, const uint32_t cols[10], int piece, int o, int x, int ply) {
-    return pruner_score(w, board_from(cols), piece, o, x, ply);
+float ta3o_pruner_score(const float* w, float act_range, const uint32_t cols[10], int piece, int o, int x, int ply) {
+    auto q = std::make_unique<PrunerQ>();
+    pruner_quantize(w, act_range, *q);
+    return pruner_score(*q, board_from(cols), piece, o, x, ply);
+}
+
+void ta3o_pruner_scores(const float* w, float act_range, const uint32_t* samples, int n, float* out) {
+    auto q = std::make_unique<PrunerQ>();
+    pruner_quantize(w, act_range, *q);
+    for(int t = 0; t < n; ++t) {
+        uint32_t const* rec = samples + 11 * static_cast<size_t>(t);
+        uint32_t const meta = rec[10];
+        out[t] = pruner_score(*q, board_from(rec), meta & 7, (meta >> 3) & 3, static_cast<int>((meta >> 5) & 15) - 2,
+                              (meta >> 9) & 3);
+    }
+}
+
+void ta3o_pruner_quant_info(const float* w, float act_range, int* info) {
+    auto q = std::make_unique<PrunerQ>();
+    pruner_quantize(w, act_range, *q);
+    info[0] = q->e1;
+    info[1] = q->e2;
+    info[2] = q->e3;
+    info[3] = q->e4;
+    info[4] = q->d;
 }
 
 double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, double pos_weight, int rounded,
@@ -1463,14 +1544,18 @@ double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, dou
 }
 
 void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
-                              uint32_t max_moves, int threads, const float* pruner_w, int k, unsigned ply_mask,
-                              float* out, uint32_t* placements_out, ta3o_counters* counters) {
+                              uint32_t max_moves, int threads, const float* pruner_w, float act_range, int k,
+                              unsigned ply_mask, float* out, uint32_t* placements_out, ta3o_counters* counters) {
     int const P = ta3o_num_params(kind);
     uint64_t const items = static_cast<uint64_t>(num_models) * games;
     std::atomic<uint64_t> next{0};
     if(threads < 1) threads = 1;
+    auto q = std::make_unique<PrunerQ>();
     PrunerRef pr;
-    pr.w = pruner_w;
+    if(pruner_w) {
+        pruner_quantize(pruner_w, act_range, *q);
+        pr.q = q.get();
+    }
     pr.k = k;
     pr.plyMask = ply_mask;
     std::vector<Counters> perThread(static_cast<size_t>(threads));

@@ -1,7 +1,8 @@
 """Pins the oracle's placement-proposal network (pruner), its training gradient and the pruned beam.
 
 The pruner is build-defined (no reference counterpart): 265 inputs (200 board bits in 240 slots, one-hot
-piece / rotation / column / ply) -> 64 -> 64 -> 1, bfloat16 W1, W2 and first hidden layer at inference.
+piece / rotation / column / ply) -> 64 -> 64 -> 1. Training rounds W1, W2 and the first hidden layer to
+bfloat16; inference runs an integer network on power-of-two grids (8-bit W1, W2 and first hidden layer).
 """
 import numpy as np
 import pytest
@@ -36,33 +37,82 @@ def inputs(cols, piece, o, x, ply):
     return xs
 
 
-def np_score(w, cols, piece, o, x, ply):
-    W1 = bf16(w[:64 * 265]).reshape(64, 265).astype(np.float64)
-    b1 = w[64 * 265:64 * 266].astype(np.float64)
+def np_float_score(w, xs_list):
+    """dense float64 forward pass of the unrounded network, one row per input vector"""
+    w = np.asarray(w, dtype=np.float64)
+    W1 = w[:64 * 265].reshape(64, 265)
+    b1 = w[64 * 265:64 * 266]
     o2 = 64 * 266
-    W2 = bf16(w[o2:o2 + 4096]).reshape(64, 64).astype(np.float64)
-    b2 = w[o2 + 4096:o2 + 4160].astype(np.float64)
-    W3 = w[o2 + 4160:o2 + 4224].astype(np.float64)
+    W2 = w[o2:o2 + 4096].reshape(64, 64)
+    b2 = w[o2 + 4096:o2 + 4160]
+    W3 = w[o2 + 4160:o2 + 4224]
     b3 = float(w[o2 + 4224])
-    pre1 = b1 + W1 @ inputs(cols, piece, o, x, ply)
-    h1 = bf16(np.maximum(pre1.astype(np.float32), 0)).astype(np.float64)
-    h2 = np.maximum(b2 + W2 @ h1, 0)
-    return b3 + h2 @ W3
+    X = np.asarray(xs_list)
+    h1 = np.maximum(X @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ W3 + b3, h1
 
 
 def test_param_count_and_layout():
     assert O.pruner_num_params() == P == 64 * 265 + 64 + 64 * 64 + 64 + 64 + 1
 
 
-def test_score_matches_dense_numpy():
-    w = random_pruner(1)
-    rng = np.random.default_rng(2)
-    for _ in range(20):
+def random_candidates(seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
         cols = rng.integers(0, 1 << 23, 10).astype(np.uint32)
-        piece, o, x, ply = int(rng.integers(0, 7)), int(rng.integers(0, 4)), int(rng.integers(-2, 9)), int(rng.integers(1, 4))
-        got = O.pruner_score(w, cols, piece, o, x, ply)
-        want = np_score(w, cols, piece, o, x, ply)
-        assert abs(got - want) <= 1e-5 * max(1.0, abs(want))
+        out.append((cols, int(rng.integers(0, 7)), int(rng.integers(0, 4)), int(rng.integers(-2, 9)),
+                    int(rng.integers(1, 4))))
+    return out
+
+
+def test_integer_inference_tracks_the_float_network():
+    """the quantized logit, scaled back by 2^-(e2+e3+e4) and shifted by b3, is the dense float64 network up to
+    8-bit quantization error, and ranks candidates the same way"""
+    w = random_pruner(1)
+    cands = random_candidates(2, 300)
+    ref, h1 = np_float_score(w, [inputs(*c) for c in cands])
+    act = float(np.max(h1)) * 1.01  # no hidden unit clips
+    e1, e2, e3, e4, d = O.pruner_quant_info(w, act)
+    assert 2.0 ** e1 * np.abs(w[:64 * 265]).max() <= 255 < 2.0 ** (e1 + 1) * np.abs(w[:64 * 265]).max()
+    assert d == e1 - e2
+    got = np.array([O.pruner_score(w, act, *c) for c in cands], dtype=np.float64)
+    assert (got == np.round(got)).all()  # integer-valued logits
+    approx = got * 2.0 ** -(e2 + e3 + e4) + float(w[-1])
+    err = np.abs(approx - ref)
+    # 8-bit grids: steps of max|W|/255 per weight and act/255 per hidden activation
+    assert err.max() <= 0.1 * ref.std() and err.mean() <= 0.025 * ref.std(), (err.max(), err.mean(), ref.std())
+    rank_ref = np.argsort(np.argsort(ref))
+    rank_got = np.argsort(np.argsort(approx))
+    assert np.corrcoef(rank_ref, rank_got)[0, 1] > 0.995
+
+
+def test_integer_inference_is_scale_invariant():
+    """power-of-two rescaling of a layer moves its grid exponent and leaves every logit bit-identical"""
+    w = random_pruner(3)
+    cands = random_candidates(4, 40)
+    act = 2.0
+    base = [O.pruner_score(w, act, *c) for c in cands]
+    w2 = w.copy()
+    w2[:64 * 266] *= 4.0                           # W1 and b1: the first hidden layer grows 4x ...
+    w2[64 * 266:64 * 266 + 4096] *= 0.25           # ... and W2 takes it back: the same float function
+    assert [O.pruner_score(w2, 4 * act, *c) for c in cands] == base
+    w3 = w.copy()
+    w3[64 * 266 + 4160:64 * 266 + 4224] *= 0.5  # W3
+    assert [O.pruner_score(w3, act, *c) for c in cands] == base
+    e = O.pruner_quant_info(w2, 4 * act)
+    assert e[0] == O.pruner_quant_info(w, act)[0] - 2
+
+
+def test_batch_scores_match_single():
+    w = random_pruner(5)
+    recs = small_batch(6, 30)
+    got = O.pruner_scores(w, 1.5, recs)
+    for t in range(30):
+        m = int(recs[t, 10])
+        want = O.pruner_score(w, 1.5, recs[t, :10], m & 7, (m >> 3) & 3, ((m >> 5) & 15) - 2, (m >> 9) & 3)
+        assert got[t] == want
 
 
 def small_batch(seed=3, n=40):
@@ -109,7 +159,7 @@ def test_pruned_beam_with_k34_is_the_full_beam():
     seeds = O.expand_seeds(9, 3)
     full, full_pl = O.eval_host_mt(O.MODEL_V11, ws, 2, seeds, 120)
     pr = random_pruner(7)
-    pruned, pl, _ = O.eval_host_pruned_mt(O.MODEL_V11, ws, 2, seeds, 120, pr, 34, 0b1110)
+    pruned, pl, _ = O.eval_host_pruned_mt(O.MODEL_V11, ws, 2, seeds, 120, pr, 1.0, 34, 0b1110)
     assert pruned.tobytes() == full.tobytes() and (pl == full_pl).all()
 
 
@@ -120,7 +170,7 @@ def test_pruning_cuts_value_evaluations():
     pr = random_pruner(11)
     cnt_full = O.Counters()
     O.eval_host_mt(O.MODEL_V11, ws, 1, seeds, 60, counters=cnt_full)
-    _, _, cnt = O.eval_host_pruned_mt(O.MODEL_V11, ws, 1, seeds, 60, pr, 8, 0b1100)
+    _, _, cnt = O.eval_host_pruned_mt(O.MODEL_V11, ws, 1, seeds, 60, pr, 1.0, 8, 0b1100)
     per_move_full = cnt_full.legal / cnt_full.moves
     per_move = cnt.legal / cnt.moves
     assert per_move < 0.75 * per_move_full

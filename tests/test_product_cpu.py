@@ -57,3 +57,25 @@ def test_argument_errors_without_gpu():
         ev.set_launch(3)
     with pytest.raises(T.HipError, match="unknown model"):
         T.ModelEvaluator(model=7)
+    # pruner arguments are checked before any device work
+    pw = T.pruner_init(0)
+    with pytest.raises(T.HipError, match="k must be"):
+        ev.set_pruner(pw, 2.0, k=0)
+    with pytest.raises(T.HipError, match="ply_mask"):
+        ev.set_pruner(pw, 2.0, k=8, ply_mask=0b10001)
+    with pytest.raises(T.HipError, match="act_range"):
+        ev.set_pruner(pw, 0.0, k=8)
+    with pytest.raises(T.HipError, match="21249"):
+        ev.set_pruner(pw[:-1], 2.0)
+    ev.set_pruner(None, 0.0)  # back to the full beam: no device work
+
+
+def test_pruner_quantization_matches_oracle():
+    """the library's grid exponents for the integer inference form equal the oracle's over weights and
+    activation ranges spanning many binades"""
+    import oracle as O
+    rng = np.random.default_rng(0)
+    for _ in range(100):
+        w = rng.normal(0, 10 ** rng.uniform(-3, 2), T.pruner_num_params()).astype(np.float32)
+        act = float(10 ** rng.uniform(-3, 3))
+        assert T.pruner_quant_info(w, act) == O.pruner_quant_info(w, act)

@@ -56,6 +56,16 @@ def lib():
             "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
             "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
+            "ta3h_pruner_num_params": (C.c_int, []),
+            "ta3h_pruner_quant_info": (C.c_int, [f32p, C.c_float, C.POINTER(C.c_int)]),
+            "ta3h_set_pruner": (C.c_int, [vp, f32p, C.c_float, C.c_uint32, C.c_uint32]),
+            "ta3h_set_trace": (C.c_int, [vp, C.c_uint64, C.c_uint32]),
+            "ta3h_fetch_trace": (C.c_int, [vp, u32p, C.c_uint64, u64p, u64p]),
+            "ta3h_pruner_grad": (C.c_int, [vp, f32p, u32p, C.c_uint32, C.c_float, f32p, f32p]),
+            "ta3h_pruner_train": (C.c_int, [vp, f32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
+                                            C.c_float, C.c_uint64, f32p]),
+            "ta3h_pruner_hidden": (C.c_int, [vp, f32p, u32p, C.c_uint32, f32p]),
+            "ta3h_pruner_logits": (C.c_int, [vp, f32p, C.c_float, u32p, C.c_uint32, f32p]),
             "ta3h_tables": (None, [C.POINTER(C.c_int8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
                                    C.POINTER(C.c_uint8)]),
         }
@@ -178,6 +188,116 @@ class ModelEvaluator:
         g, w, c = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
         self._check(lib().ta3h_launch_info(self._h, C.byref(g), C.byref(w), C.byref(c)))
         return {"grid_blocks": g.value, "waves_per_game": w.value, "num_cus": c.value}
+
+    # --- learned pruner ------------------------------------------------------------------------------
+    def set_pruner(self, pruner_w, act_range: float, k: int = 8, ply_mask: int = 0b1100):
+        """pruned beam for the following launches (ply_mask bit 1, 2: middle levels, bit 3: leaves);
+        pruner_w None restores the full beam"""
+        if pruner_w is None:
+            self._check(lib().ta3h_set_pruner(self._h, None, 1.0, 34, 0))
+            return
+        w = _pruner_w(pruner_w)
+        self._check(lib().ta3h_set_pruner(self._h, _ptr(w, C.c_float), float(act_range), int(k), int(ply_mask)))
+
+    def set_trace(self, cap_records: int, every: int = 1):
+        """full-beam traces of every `every`-th move for the following launches (cap_records 0 = off)"""
+        self._check(lib().ta3h_set_trace(self._h, int(cap_records), int(every)))
+        self._trace_cap = int(cap_records)
+
+    def fetch_trace(self, cap_records: int | None = None):
+        """(records uint32 [n, 11], produced) of the last launch; n <= the capacity set by set_trace"""
+        produced, copied = C.c_uint64(0), C.c_uint64(0)
+        self._check(lib().ta3h_fetch_trace(self._h, None, 0, C.byref(produced), C.byref(copied)))
+        n = min(produced.value, getattr(self, "_trace_cap", 0))
+        if cap_records is not None:
+            n = min(n, int(cap_records))
+        out = np.zeros((n, 11), dtype=np.uint32)
+        if n:
+            self._check(lib().ta3h_fetch_trace(self._h, _ptr(out, C.c_uint32), n, C.byref(produced), C.byref(copied)))
+        return out[:copied.value], produced.value
+
+    def pruner_grad(self, pruner_w, samples, pos_weight: float = 1.0):
+        """(batch-mean loss, gradient float32[21249]) over samples [n, 11] on the device"""
+        w = _pruner_w(pruner_w)
+        smp = _samples(samples)
+        g = np.zeros(pruner_num_params(), dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(lib().ta3h_pruner_grad(self._h, _ptr(w, C.c_float), _ptr(smp, C.c_uint32), smp.shape[0],
+                                           float(pos_weight), _ptr(g, C.c_float), C.byref(loss)))
+        return float(loss.value), g
+
+    def pruner_train(self, pruner_w, samples=None, n: int | None = None, epochs: int = 1, batch: int = 512,
+                     lr: float = 1e-3, pos_weight: float = 1.0, seed: int = 0):
+        """Adam on the device; samples None trains on the last launch's trace buffer (first n records).
+        Returns (new weights, per-epoch mean loss)."""
+        w = _pruner_w(pruner_w).copy()
+        if samples is None:
+            smp_p = None
+            if n is None:
+                raise HipError("pruner_train: n is required when training on the trace buffer")
+        else:
+            smp = _samples(samples)
+            n = smp.shape[0] if n is None else min(int(n), smp.shape[0])
+            smp_p = _ptr(smp, C.c_uint32)
+        losses = np.zeros(max(int(epochs), 1), dtype=np.float32)
+        self._check(lib().ta3h_pruner_train(self._h, _ptr(w, C.c_float), smp_p, int(n), int(epochs), int(batch),
+                                            float(lr), float(pos_weight), int(seed), _ptr(losses, C.c_float)))
+        return w, losses[:int(epochs)]
+
+    def pruner_hidden(self, pruner_w, samples):
+        w = _pruner_w(pruner_w)
+        smp = _samples(samples)
+        out = np.zeros((smp.shape[0], 64), dtype=np.float32)
+        self._check(lib().ta3h_pruner_hidden(self._h, _ptr(w, C.c_float), _ptr(smp, C.c_uint32), smp.shape[0],
+                                             _ptr(out, C.c_float)))
+        return out
+
+    def pruner_logits(self, pruner_w, act_range: float, samples):
+        """inference logits as the pruned search computes them (integer grids on BF16 MFMA)"""
+        w = _pruner_w(pruner_w)
+        smp = _samples(samples)
+        out = np.zeros(smp.shape[0], dtype=np.float32)
+        self._check(lib().ta3h_pruner_logits(self._h, _ptr(w, C.c_float), float(act_range), _ptr(smp, C.c_uint32),
+                                             smp.shape[0], _ptr(out, C.c_float)))
+        return out
+
+
+def pruner_num_params() -> int:
+    return lib().ta3h_pruner_num_params()
+
+
+def _pruner_w(w):
+    a = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+    if a.size != pruner_num_params():
+        raise HipError(f"pruner weights must hold {pruner_num_params()} values")
+    return a
+
+
+def _samples(samples):
+    smp = np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1, 11)
+    if smp.shape[0] == 0:
+        raise HipError("no samples")
+    return smp
+
+
+def pruner_quant_info(pruner_w, act_range: float):
+    """grid exponents (e1, e2, e3, e4, d) of the integer inference form"""
+    w = _pruner_w(pruner_w)
+    info = (C.c_int * 5)()
+    if lib().ta3h_pruner_quant_info(_ptr(w, C.c_float), float(act_range), info) != 0:
+        raise HipError("pruner_quant_info: invalid arguments")
+    return tuple(info)
+
+
+def pruner_init(seed: int = 0) -> np.ndarray:
+    """He-initialised master weights (zero biases)"""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(pruner_num_params(), dtype=np.float32)
+    o = 0
+    for fan_in, fan_out in ((265, 64), (64, 64), (64, 1)):
+        w[o:o + fan_in * fan_out] = rng.normal(0.0, np.sqrt(2.0 / fan_in), fan_in * fan_out)
+        o += fan_in * fan_out + fan_out
+    return w
 
 
 # launch shapes measured fastest on MI355X per network (full CMA-ES generation, see autotune())

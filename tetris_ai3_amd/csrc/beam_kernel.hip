@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "tables.hpp"
@@ -191,6 +192,9 @@ struct alignas(16) GameLds {
     uint64_t partBest[4];                     // per-wave best leaf keys
     uint64_t partMask[4];                     // per-wave kept-node / segment legality masks
     uint32_t partLegal[4];                    // per-wave legal-candidate counts of the item
+    uint32_t survivors[68];                   // traces: slots kept by a middle level (2,176 bits)
+    uint64_t segBest[64];                     // traces: best leaf key of every leaf list
+    uint32_t keptCount;                       // pruned stages: candidates kept for the value network
     EngineState eng;
 };
 
@@ -359,6 +363,14 @@ __device__ inline wptr opaque(wptr w) {
     uint64_t v = reinterpret_cast<uint64_t>(w);
     asm volatile("" : "+s"(v));
     return reinterpret_cast<wptr>(v);
+}
+
+// the same for any wave-uniform pointer
+template<typename T>
+__device__ inline T* opaque_ptr(T* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+s"(v));
+    return reinterpret_cast<T*>(v);
 }
 
 __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::max<float>(0, v)
@@ -592,9 +604,9 @@ __device__ inline void group_sync() {
 //    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
 //  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
 //    least one legal placement (expand_leaves, beam.hpp:407-443).
-template<int KIND, int STAGE, int K, int G>
-__device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_t wv, uint32_t* col, uint64_t* mask,
-                                     uint32_t& nLegal) {
+template<int KIND, int STAGE, int K, int G, bool TRACE = false>
+__device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
+                                     uint64_t* mask, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
     uint64_t ms = 0;
@@ -604,7 +616,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_
         bool legal = false;
         uint32_t score = 0, s = 0, i = 0;
         if(f < total) {
-            uint32_t const si = L.candSeg[f];
+            uint32_t const si = seg[f];
             s = si >> 6;
             i = si & 63u;
             uint32_t const info = L.segInfo[s];
@@ -624,6 +636,9 @@ __device__ inline uint64_t run_stage(GameLds& L, uint32_t total, wptr w, uint32_
         if constexpr(STAGE == STAGE_LEAF) {
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
             acc = key > acc ? key : acc;
+            if constexpr(TRACE) {
+                if(legal) atomicMax(reinterpret_cast<unsigned long long*>(&L.segBest[s]), static_cast<unsigned long long>(key));
+            }
             ms |= legal ? (1ull << s) : 0ull;
         } else {
             if(STAGE == STAGE_MID) ms |= legal ? (1ull << (s >> 1)) : 0ull;
@@ -687,6 +702,223 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------------
+// pruned stages: placement-proposal network on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate)
+// ---------------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 8 board bits -> 8 bfloat16 values 0.0 / 1.0
+__device__ inline bf16x8 bits_bf16(uint32_t byte) {
+    u32x4 v;
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+        v[j] = (((byte >> (2 * j)) & 1u) ? 0x3F80u : 0u) | (((byte >> (2 * j + 1)) & 1u) ? 0x3F800000u : 0u);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ inline bf16x8 load_bf16x8(uint16_t const* p) { return *reinterpret_cast<bf16x8 const*>(p); }
+
+// First hidden layer code of a pre-activation (an exact integer held in FP32): 0 for P <= 0, else
+// min(255, round-half-up(P 2^-d)); d in [-8, 24] so no shift overflows (|P| < 2^21).
+__device__ inline float act_code(float p, int d) {
+    int32_t const P = static_cast<int32_t>(p);
+    if(P <= 0) return 0.0f;
+    int32_t const c = d > 0 ? (P + (1 << (d - 1))) >> d : P << -d;
+    return static_cast<float>(c < 255 ? c : 255);
+}
+
+// One 16 x 16 tile of the board projection, bp[row][unit] = sum over the row board's set inputs k of N1[unit][k]:
+// rows = 16 boards (this lane's A row is `board`, null = an empty row), columns = hidden units 16 nt .. + 15.
+// v_mfma_f32_16x16x32_bf16 fragments: lane l holds A row / B column l % 16 and K elements 8 (l / 16) .. + 7;
+// D element j of lane l is row 4 (l / 16) + j, column l % 16. Every partial sum is an integer below 2^24.
+__device__ inline f32x4 project_tile(PrunerDev const* __restrict__ pr, uint32_t const* board, uint32_t nt) {
+    uint32_t const lane = lane_id(), q = lane >> 4;
+    uint32_t const unit = 16 * nt + (lane & 15u);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+    for(uint32_t ks = 0; ks < 8; ++ks) {
+        PrunerDev const* const pl = opaque_ptr(pr); // keep the weight loads inside the loop
+        uint32_t const k0 = 32 * ks + 8 * q;         // board slots k0 .. k0 + 7 lie in one column
+        uint32_t byte = 0;
+        if(board != nullptr && k0 < 240) {
+            uint32_t const c = k0 / 24, r0 = k0 % 24;
+            byte = (board[c] >> (3 + r0)) & (r0 == 16 ? 0xFu : 0xFFu);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bits_bf16(byte), load_bf16x8(&pl->w1b[unit][k0]), acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Pruner logit of the lane's candidate (column l % 16 of a 16-candidate pass) from its board projection row
+// `bpRow` and placement row `ptRow` (null = an empty column). Lane l covers hidden units {8 (l / 16) .. + 7} and
+// {32 + 8 (l / 16) .. + 7}: exactly its B fragments of layer 2, so no data moves between lanes before the
+// MFMAs. Every lane returns its column's logit.
+__device__ inline float pruner_logit(PrunerDev const* __restrict__ pr, float const* bpRow, float const* ptRow) {
+    uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
+    int const d = pr->actShift;
+    bf16x8 h1a, h1b;
+#pragma unroll
+    for(int j = 0; j < 8; ++j) {
+        float ca = 0.0f, cb = 0.0f;
+        if(ptRow != nullptr) {
+            ca = act_code(bpRow[8 * q + j] + ptRow[8 * q + j], d);
+            cb = act_code(bpRow[32 + 8 * q + j] + ptRow[32 + 8 * q + j], d);
+        }
+        h1a[j] = static_cast<__bf16>(ca);
+        h1b[j] = static_cast<__bf16>(cb);
+    }
+    double part = 0.0;
+#pragma unroll 1
+    for(uint32_t mt = 0; mt < 4; ++mt) { // rolled: one output tile's weights live at a time
+        PrunerDev const* const pl = pr;
+        uint32_t const out = 16 * mt + r16; // A row: output unit
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(&pl->w2b[out][8 * q]), h1a, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(&pl->w2b[out][32 + 8 * q]), h1b, acc, 0, 0, 0);
+#pragma unroll
+        for(int j = 0; j < 4; ++j) {
+            uint32_t const o = 16 * mt + 4 * q + j;
+            float const h2 = acc[j] + pl->b2[o]; // exact: |acc| < 2^22, |Nb2| <= 2^22
+            if(h2 > 0.0f) part = __fma_rn(static_cast<double>(h2), static_cast<double>(pl->w3[o]), part); // exact
+        }
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return static_cast<float>(part);
+}
+
+// Board projection of the frontier's kept nodes: bp[p][u] for p < count (32 x 64, eight tiles).
+template<int G>
+__device__ void pruner_board_projection(GameLds const& L, PrunerDev const* __restrict__ pr, uint32_t cur,
+                                        uint32_t count, float (*bp)[PR_H], uint32_t wv) {
+    uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
+#pragma unroll 1
+    for(uint32_t t = wv; t < 8; t += G) {
+        uint32_t const mt = t >> 2, nt = t & 3u;
+        uint32_t const p = 16 * mt + r16;
+        f32x4 const acc = project_tile(pr, p < count ? &L.boards[(cur * 32 + p) * WIDTH] : nullptr, nt);
+#pragma unroll
+        for(int j = 0; j < 4; ++j) bp[16 * mt + 4 * q + j][16 * nt + r16] = acc[j];
+    }
+}
+
+// Pruner logits of a stage's candidates, 16 per pass; illegal candidates get -inf.
+template<int G>
+__device__ void pruner_scores(GameLds const& L, PrunerDev const* __restrict__ pr, uint32_t total, uint32_t cur,
+                              uint32_t ply, float const (*bp)[PR_H], float* prs, uint32_t wv) {
+    uint32_t const lane = lane_id(), r16 = lane & 15u;
+    uint32_t const groups = (total + 15) / 16;
+    for(uint32_t gi = wv; gi < groups; gi += G) {
+        uint32_t const f = 16 * gi + r16;
+        bool legal = false;
+        float const* bpRow = nullptr;
+        float const* ptRow = nullptr;
+        if(f < total) {
+            uint32_t const si = L.candSeg[f];
+            uint32_t const s = si >> 6, i = si & 63u;
+            uint32_t const info = L.segInfo[s];
+            uint32_t const pb = info & 0x7Fu, piece = (info >> 8) & 7u;
+            uint32_t const li = c_tab.lutOff[piece] + i;
+            LutEntry const e = c_tab.lut[li];
+            int drop = 0;
+            legal = fit_span(&L.boards[pb * WIDTH], e.cols, e.col0, drop);
+            bpRow = bp[pb - cur * 32];
+            ptRow = pr->ptab[ply - 1][li];
+        }
+        float const logit = pruner_logit(pr, bpRow, ptRow);
+        if(lane < 16 && f < total) prs[f] = legal ? logit : -__builtin_huge_valf();
+    }
+}
+
+// Keeps the K best legal placements of every list by pruner logit (logit descending, placement index ascending);
+// returns how many were kept and lists them in `kept` (segment << 6 | placement index, any order).
+template<int G>
+__device__ uint32_t pruner_select(GameLds& L, uint32_t total, float const* prs, uint16_t* kept, uint32_t K, uint32_t wv) {
+    uint32_t const lane = lane_id();
+    if(wv == 0 && lane == 0) L.keptCount = 0;
+    group_sync<G>();
+    for(uint32_t base = wv * 64; base < total; base += 64 * G) {
+        uint32_t const f = base + lane;
+        bool keep = false;
+        uint32_t si = 0;
+        if(f < total) {
+            si = L.candSeg[f];
+            uint32_t const s = si >> 6, i = si & 63u;
+            float const pf = prs[f];
+            if(pf != -__builtin_huge_valf()) {
+                uint32_t const off = L.segOff[s];
+                uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - off;
+                uint32_t rank = 0;
+                for(uint32_t j = 0; j < n; ++j) {
+                    float const pj = prs[off + j];
+                    rank += (pj > pf || (pj == pf && j < i)) ? 1u : 0u;
+                }
+                keep = rank < K;
+            }
+        }
+        uint64_t const bal = __ballot(keep);
+        uint32_t const cnt = static_cast<uint32_t>(__popcll(bal));
+        uint32_t b = 0;
+        if(lane == 0 && cnt != 0) b = atomicAdd(&L.keptCount, cnt);
+        b = uniform(b);
+        if(keep) kept[b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)))] = static_cast<uint16_t>(si);
+    }
+    group_sync<G>();
+    return uniform(L.keptCount);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// full-beam traces for training the pruner
+// ---------------------------------------------------------------------------------------------------
+// every lane of the wave calls this; lanes with `want` append one record
+__device__ inline void emit_record(EvalArgs const& a, bool want, uint32_t const* parent, uint32_t meta) {
+    uint32_t const lane = lane_id();
+    uint64_t const bal = __ballot(want);
+    uint32_t const cnt = static_cast<uint32_t>(__popcll(bal));
+    if(cnt == 0) return;
+    uint32_t b = 0;
+    if(lane == 0) b = atomicAdd(a.traceCount, cnt);
+    b = uniform(b);
+    uint32_t const idx = b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)));
+    if(want && idx < a.traceCap) {
+        uint32_t* r = a.trace + 11ull * idx;
+#pragma unroll
+        for(int x = 0; x < WIDTH; ++x) r[x] = parent[x];
+        r[10] = meta;
+    }
+}
+
+// one record per legal candidate of the stage; label = slot kept by the level (ply 1, 2) or the list's best
+// leaf (ply 3)
+template<int G>
+__device__ void emit_stage(EvalArgs const& a, GameLds const& L, uint32_t total, uint32_t ply, uint32_t wv) {
+    uint32_t const lane = lane_id();
+    for(uint32_t base = wv * 64; base < total; base += 64 * G) {
+        uint32_t const f = base + lane;
+        bool legal = false;
+        uint32_t meta = 0, pb = COMMITTED;
+        if(f < total) {
+            uint32_t const si = L.candSeg[f];
+            uint32_t const s = si >> 6, i = si & 63u;
+            uint32_t const info = L.segInfo[s];
+            pb = info & 0x7Fu;
+            uint32_t const piece = (info >> 8) & 7u;
+            LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
+            int drop = 0;
+            legal = fit_span(&L.boards[pb * WIDTH], e.cols, e.col0, drop);
+            uint32_t const slot = s * 34 + i;
+            bool const label = ply < 3 ? ((L.survivors[slot >> 5] >> (slot & 31u)) & 1u) != 0
+                                       : (SPAN - static_cast<uint32_t>(L.segBest[s])) == slot;
+            meta = piece | static_cast<uint32_t>(e.ox & 3) << 3 | static_cast<uint32_t>(e.ox >> 2) << 5 | ply << 9
+                 | (label ? 1u << 11 : 0u);
+        }
+        emit_record(a, legal, &L.boards[pb * WIDTH], meta);
+    }
+}
+
 // Re-applies the selected placement of rank j (lane j < count) to its parent and writes kept node j of
 // frontier buffer `dst` (seed_frontier / advance_frontier, beam.hpp:307-329, 379-403).
 __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
@@ -735,11 +967,22 @@ __device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, ui
     return uniform(L.stageTotal);
 }
 
+// Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
+// logits share the lanes' candidate-board scratch, which is idle until the value network runs).
+struct PruneLds {
+    float bp[32][PR_H];
+    uint16_t kept[32 * 2 * 34];
+};
+
+enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
+
 // One move of the beamed search plus its commit (search_move_beam, beam.hpp:483-570; the eval_host loop
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
-// ends (no move, or a top-out).
-template<int KIND, int G>
-__device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64_t& nApply, uint32_t& nLegal) {
+// ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
+// pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
+template<int KIND, int G, int MODE>
+__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, uint32_t wv, uint32_t* col,
+                          bool traceMove, uint64_t& nApply, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
@@ -753,12 +996,14 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64
     }
     group_sync<G>();
     uint32_t const ti = uniform(L.treeIdx);
+    // the per-lane candidate-board scratch doubles as the pruner logits of a stage (2,176 floats)
+    float* const prs = reinterpret_cast<float*>(col - lane - wv * (WIDTH * SCR_STRIDE));
 
     // stage 0: roots -> top K0
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
     nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, total, w, wv, col, nullptr, nLegal);
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, L.candSeg, total, w, wv, col, nullptr, nLegal);
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
@@ -775,9 +1020,21 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64
     for(int level = 1; level <= 2; ++level) {
         total = stage_segments<STAGE_MID, G>(L, ti, level, count, cur, wv);
         nApply += total;
+        uint16_t const* seg = L.candSeg;
+        uint32_t evalTotal = total;
+        if constexpr(MODE == MODE_PRUNED) {
+            if((a.pruneMask >> level) & 1u) {
+                pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
+                group_sync<G>();
+                pruner_scores<G>(L, a.pruner, total, cur, static_cast<uint32_t>(level), P->bp, prs, wv);
+                group_sync<G>();
+                evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
+                seg = P->kept;
+            }
+        }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, total, w, wv, col, &mask, nLegal)
-                         : run_stage<KIND, STAGE_MID, K2, G>(L, total, w, wv, col, &mask, nLegal);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal)
+                         : run_stage<KIND, STAGE_MID, K2, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
@@ -788,6 +1045,20 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64
         fb = wave_max_u64(fb);
         bestKey = fb > bestKey ? fb : bestKey;
         uint32_t const next = __popcll(__ballot(top != 0));
+        if constexpr(MODE == MODE_TRACE) {
+            if(traceMove) {
+                if(wv == 0) {
+                    for(uint32_t i = lane; i < 68; i += 64) L.survivors[i] = 0;
+                    wave_sync();
+                    if(lane < next) {
+                        uint32_t const slot = 0xFFFFFFFFu - static_cast<uint32_t>(top);
+                        atomicOr(&L.survivors[slot >> 5], 1u << (slot & 31u));
+                    }
+                }
+                group_sync<G>();
+                emit_stage<G>(a, L, total, static_cast<uint32_t>(level), wv);
+            }
+        }
         if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false, col);
         group_sync<G>();
         cur ^= 1u;
@@ -797,9 +1068,35 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64
     // leaves
     total = stage_segments<STAGE_LEAF, G>(L, ti, 3, count, cur, wv);
     nApply += total;
+    uint16_t const* seg = L.candSeg;
+    uint32_t evalTotal = total;
+    if constexpr(MODE == MODE_PRUNED) {
+        if((a.pruneMask >> 3) & 1u) {
+            pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
+            group_sync<G>();
+            pruner_scores<G>(L, a.pruner, total, cur, 3u, P->bp, prs, wv);
+            group_sync<G>();
+            evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
+            seg = P->kept;
+        }
+    }
     mask = 0;
-    uint64_t leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, total, w, wv, col, &mask, nLegal);
+    uint64_t leaf;
+    if constexpr(MODE == MODE_TRACE) {
+        if(traceMove) {
+            if(wv == 0) L.segBest[lane] = 0;
+            group_sync<G>();
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G, true>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+        } else {
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+        }
+    } else {
+        leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+    }
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
+    if constexpr(MODE == MODE_TRACE) {
+        if(traceMove) emit_stage<G>(a, L, total, 3u, wv);
+    }
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
         uint32_t const s = lane;
@@ -838,17 +1135,21 @@ __device__ bool play_move(GameLds& L, wptr w, uint32_t wv, uint32_t* col, uint64
 // kernels
 // ---------------------------------------------------------------------------------------------------
 // Waves per SIMD the register allocation is held to: the largest occupancy each network's code fits
-// without spilling registers to memory (V11: <= 102 VGPRs, 16-32-32-1: <= 128).
-template<int KIND>
-constexpr int WAVES_PER_SIMD = KIND == MODEL_V11 ? 5 : 4;
+// without spilling registers to memory (V11: <= 102 VGPRs, 16-32-32-1: <= 128). The pruned kernel's LDS
+// caps it at 4 workgroups of 4 waves per CU, so it gets the 128 registers of 4 waves per SIMD.
+template<int KIND, int MODE>
+constexpr int WAVES_PER_SIMD = KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4;
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
-template<int KIND, int G>
-__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND>, 8)))
+template<int KIND, int G, int MODE = MODE_PLAIN>
+__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND, MODE>, 8)))
 void beam_eval_kernel(EvalArgs a) {
+    static_assert(MODE == MODE_PLAIN || G * WIDTH * SCR_STRIDE >= 32 * 2 * 34, "pruner logits share the scratch");
     __shared__ GameLds L;
     __shared__ uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
+    __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
+    PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
@@ -872,7 +1173,8 @@ void beam_eval_kernel(EvalArgs a) {
         uint32_t nLegal = 0;
         for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
             if(uniform(L.eng.gameOver) != 0) break;
-            if(!play_move<KIND, G>(L, w, wv, col, nApply, nLegal)) break;
+            bool const traceMove = MODE == MODE_TRACE && a.traceEvery != 0 && mv % a.traceEvery == 0;
+            if(!play_move<KIND, G, MODE>(a, L, pruneLds, w, wv, col, traceMove, nApply, nLegal)) break;
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
         group_sync<G>();
@@ -928,6 +1230,14 @@ __global__ void feature_norm_kernel(float* out, uint32_t n) {
 
 template<int KIND>
 hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
+    if(a.pruner != nullptr || a.trace != nullptr) { // pruned and trace kernels run 4 waves per game
+        if(wpb != 4) return hipErrorInvalidValue;
+        if(a.pruner != nullptr)
+            hipLaunchKernelGGL((beam_eval_kernel<KIND, 4, MODE_PRUNED>), dim3(blocks), dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL((beam_eval_kernel<KIND, 4, MODE_TRACE>), dim3(blocks), dim3(256), 0, stream, a);
+        return hipGetLastError();
+    }
     switch(wpb) {
         case 1: hipLaunchKernelGGL((beam_eval_kernel<KIND, 1>), dim3(blocks), dim3(64), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((beam_eval_kernel<KIND, 2>), dim3(blocks), dim3(128), 0, stream, a); break;
@@ -937,11 +1247,52 @@ hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t 
     return hipGetLastError();
 }
 
+
+// Integer-grid pruner logits of trace records, 16 per wave (the pruned kernel's projection and layer-2 code on
+// rows taken from memory); a record whose (rotation, column) is no placement of its piece gets NaN.
+__global__ __launch_bounds__(64) void pruner_infer_kernel(PrunerDev const* __restrict__ pr,
+                                                          uint32_t const* __restrict__ samples, uint32_t n,
+                                                          float* __restrict__ out) {
+    __shared__ float bp[16][PR_H];
+    uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
+    uint32_t const base = blockIdx.x * 16;
+    uint32_t const t = base + r16;
+    uint32_t const* rec = t < n ? samples + 11ull * t : nullptr;
+#pragma unroll 1
+    for(uint32_t nt = 0; nt < 4; ++nt) {
+        f32x4 const acc = project_tile(pr, rec, nt);
+#pragma unroll
+        for(int j = 0; j < 4; ++j) bp[4 * q + j][16 * nt + r16] = acc[j];
+    }
+    __syncthreads();
+    float const* ptRow = nullptr;
+    bool found = false;
+    if(rec != nullptr) {
+        uint32_t const meta = rec[10];
+        uint32_t const piece = meta & 7u, ox = ((meta >> 3) & 3u) | (((meta >> 5) & 15u) << 2);
+        uint32_t const ply = (meta >> 9) & 3u;
+        if(piece < 7 && ply >= 1 && ply <= 3) {
+            for(uint32_t li = c_tab.lutOff[piece]; li < c_tab.lutOff[piece + 1]; ++li) {
+                if(c_tab.lut[li].ox == ox) {
+                    found = true;
+                    ptRow = pr->ptab[ply - 1][li];
+                }
+            }
+        }
+    }
+    float const logit = pruner_logit(pr, bp[r16], ptRow);
+    if(lane < 16 && t < n) out[t] = found ? logit : __builtin_nanf("");
+}
 } // namespace
 
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
     if(a.persistent) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
+        hipError_t const e = hipGetLastError();
+        if(e != hipSuccess) return e;
+    }
+    if(a.trace != nullptr) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.traceCount);
         hipError_t const e = hipGetLastError();
         if(e != hipSuccess) return e;
     }
@@ -954,7 +1305,9 @@ hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t block
 
 namespace {
 template<int KIND>
-hipError_t occupancy_kind(int wpb, int* n) {
+hipError_t occupancy_kind(int wpb, int* n, int mode) {
+    if(mode == MODE_PRUNED) return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 4, MODE_PRUNED>, 256, 0);
+    if(mode == MODE_TRACE) return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 4, MODE_TRACE>, 256, 0);
     switch(wpb) {
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 1>, 64, 0);
         case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 2>, 128, 0);
@@ -964,10 +1317,16 @@ hipError_t occupancy_kind(int wpb, int* n) {
 }
 } // namespace
 
-hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU) {
+hipError_t pruner_infer_samples(PrunerDev const* pr, uint32_t const* samples, uint32_t n, float* out, hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pruner_infer_kernel, dim3((n + 15) / 16), dim3(64), 0, s, pr, samples, n, out);
+    return hipGetLastError();
+}
+
+hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksPerCU) {
     switch(kind) {
-        case MODEL_V11: return occupancy_kind<MODEL_V11>(wavesPerBlock, blocksPerCU);
-        case MODEL_F16: return occupancy_kind<MODEL_F16>(wavesPerBlock, blocksPerCU);
+        case MODEL_V11: return occupancy_kind<MODEL_V11>(wavesPerBlock, blocksPerCU, mode);
+        case MODEL_F16: return occupancy_kind<MODEL_F16>(wavesPerBlock, blocksPerCU, mode);
         default: return hipErrorInvalidValue;
     }
 }

@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "pruner.hpp"
+
 namespace ta3h {
 
 enum ModelKind : int {
@@ -24,12 +26,23 @@ struct EvalArgs {
     uint32_t maxMoves;
     uint32_t weightStride;
     uint32_t persistent;
+    // pruned beam: at plies in pruneMask (bit 1, 2 = middle levels, bit 3 = leaves) only the pruner's top pruneK
+    // legal placements of every list are scored by the value network (null pruner = full beam)
+    PrunerDev const* pruner;
+    uint32_t pruneK;
+    uint32_t pruneMask;
+    // full-beam traces: every legal candidate of plies 1..3 of every traceEvery-th move (null = off)
+    uint32_t* trace;      // [traceCap][11] records: parent board (10 words) + meta
+    uint32_t* traceCount; // records produced (may exceed traceCap; the excess is dropped)
+    uint32_t traceCap;
+    uint32_t traceEvery;
 };
 
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
 // uploads the constant tables computed on the host; call once per device before the first launch
 hipError_t upload_constants();
-hipError_t max_active_blocks(int kind, int wavesPerBlock, int* blocksPerCU);
+// mode: 0 full beam, 1 full beam with traces, 2 pruned beam
+hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksPerCU);
 hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream);
 hipError_t launch_feature_norms(float* out, uint32_t n, hipStream_t stream);
 

@@ -10,8 +10,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,36 @@ struct ta3h_evaluator {
     size_t timelineCap = 0;
     bool timelineOn = false;
     uint32_t* dQueue = nullptr;
+
+    // pruned beam
+    ta3h::PrunerDev* dPruner = nullptr;
+    bool prunerOn = false;
+    uint32_t pruneK = 34, pruneMask = 0;
+    // full-beam traces
+    uint32_t* dTrace = nullptr;
+    size_t traceCap = 0; // records
+    uint32_t* dTraceCount = nullptr;
+    uint64_t traceRecords = 0; // records the trace buffer holds after a launch (0 = none requested)
+    uint32_t traceEvery = 0;   // 0 = off
+    // pruner training scratch (grow-only)
+    float* dPw = nullptr;
+    size_t pwCap = 0;
+    float* dAdam = nullptr; // m then v
+    size_t adamCap = 0;
+    uint32_t* dSamples = nullptr;
+    size_t samplesCap = 0;
+    uint32_t* dOrder = nullptr;
+    size_t orderCap = 0;
+    float* dAct = nullptr;
+    size_t actCap = 0;
+    float* dDelta = nullptr;
+    size_t deltaCap = 0;
+    float* dScore = nullptr;
+    size_t scoreCap = 0;
+    float* dLoss = nullptr;
+    size_t lossCap = 0;
+    float* dOut = nullptr;
+    size_t outCap = 0;
 
     int kind = TA3H_MODEL_V11;
     int wavesPerBlock = 2; // waves cooperating on one game
@@ -138,6 +170,11 @@ struct ta3h_evaluator {
         if(dCounters) static_cast<void>(hipFree(dCounters));
         if(dTimeline) static_cast<void>(hipFree(dTimeline));
         if(dQueue) static_cast<void>(hipFree(dQueue));
+        for(void* p : {static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
+                       static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
+                       static_cast<void*>(dOrder), static_cast<void*>(dAct), static_cast<void*>(dDelta),
+                       static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut)})
+            if(p) static_cast<void>(hipFree(p));
         if(evStart) static_cast<void>(hipEventDestroy(evStart));
         if(evStop) static_cast<void>(hipEventDestroy(evStop));
         if(stream) static_cast<void>(hipStreamDestroy(stream));
@@ -238,7 +275,22 @@ int ta3h_launch(ta3h_evaluator* ev) {
     a.maxMoves = ev->maxMoves;
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
-    // one workgroup (of wavesPerBlock cooperating waves) per game in flight
+    int mode = 0, wpb = ev->wavesPerBlock;
+    if(ev->prunerOn) {
+        a.pruner = ev->dPruner;
+        a.pruneK = ev->pruneK;
+        a.pruneMask = ev->pruneMask;
+        mode = 2;
+        wpb = 4; // the pruned and trace kernels run four waves per game
+    } else if(ev->traceEvery != 0) {
+        a.trace = ev->dTrace;
+        a.traceCount = ev->dTraceCount;
+        a.traceCap = static_cast<uint32_t>(ev->traceCap);
+        a.traceEvery = ev->traceEvery;
+        mode = 1;
+        wpb = 4;
+    }
+    // one workgroup (of wpb cooperating waves) per game in flight
     uint32_t const needed = ev->items;
     uint32_t blocks;
     hipError_t e0 = hipSuccess;
@@ -247,7 +299,7 @@ int ta3h_launch(ta3h_evaluator* ev) {
         if(blocks == 0) {
             // fill every CU up to the kernel's occupancy limit; workgroups beyond it would only queue
             int perCU = 0;
-            e0 = ta3h::max_active_blocks(ev->kind, ev->wavesPerBlock, &perCU);
+            e0 = ta3h::max_active_blocks(ev->kind, wpb, mode, &perCU);
             if(e0 != hipSuccess) return ev->fail("occupancy", e0);
             blocks = ev->numCUs * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
         }
@@ -257,7 +309,7 @@ int ta3h_launch(ta3h_evaluator* ev) {
     ev->lastGrid = blocks;
     hipError_t e = hipEventRecord(ev->evStart, ev->stream);
     if(e != hipSuccess) return ev->fail("event record", e);
-    e = ta3h::launch_beam_eval(ev->kind, a, ev->wavesPerBlock, blocks, ev->stream);
+    e = ta3h::launch_beam_eval(ev->kind, a, wpb, blocks, ev->stream);
     if(e != hipSuccess) return ev->fail("kernel launch", e);
     e = hipEventRecord(ev->evStop, ev->stream);
     if(e != hipSuccess) return ev->fail("event record", e);
@@ -359,7 +411,8 @@ int ta3h_last_kernel_ms(ta3h_evaluator* ev, float* ms) {
 int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_per_game, uint32_t* num_cus) {
     if(!ev) return TA3H_E_ARG;
     if(grid_blocks) *grid_blocks = ev->lastGrid;
-    if(waves_per_game) *waves_per_game = static_cast<uint32_t>(ev->wavesPerBlock);
+    if(waves_per_game)
+        *waves_per_game = static_cast<uint32_t>(ev->prunerOn || ev->traceEvery != 0 ? 4 : ev->wavesPerBlock);
     if(num_cus) *num_cus = ev->numCUs;
     return TA3H_OK;
 }
@@ -388,6 +441,262 @@ int ta3h_feature_norms(float* out, uint32_t n) {
         r = TA3H_E_HIP;
     static_cast<void>(hipFree(d));
     return r;
+}
+
+int ta3h_pruner_num_params(void) { return ta3h::PR_PARAMS; }
+
+int ta3h_pruner_quant_info(const float* w, float act_range, int* info) {
+    if(!w || !info || !(act_range > 0.0f) || !std::isfinite(act_range)) return TA3H_E_ARG;
+    auto t = std::make_unique<ta3h::PrunerDev>();
+    ta3h::build_pruner_tables(w, act_range, *t, info);
+    return TA3H_OK;
+}
+
+int ta3h_set_pruner(ta3h_evaluator* ev, const float* w, float act_range, uint32_t k, uint32_t ply_mask) {
+    if(!ev) return TA3H_E_ARG;
+    if(w == nullptr) {
+        ev->prunerOn = false;
+        return TA3H_OK;
+    }
+    if(!(act_range > 0.0f) || !std::isfinite(act_range)) return ev->fail_arg("set_pruner: act_range must be > 0");
+    if(k < 1 || k > 34) return ev->fail_arg("set_pruner: k must be in [1, 34]");
+    if((ply_mask & ~0xEu) != 0) return ev->fail_arg("set_pruner: ply_mask may hold bits 1, 2 and 3 only");
+    for(int i = 0; i < ta3h::PR_PARAMS; ++i)
+        if(!std::isfinite(w[i])) return ev->fail_arg("set_pruner: non-finite weight");
+    int r = ev->ensure_ready();
+    if(r) return r;
+    if(ev->dPruner == nullptr) {
+        hipError_t const e = hipMalloc(&ev->dPruner, sizeof(ta3h::PrunerDev));
+        if(e != hipSuccess) return ev->fail("malloc pruner", e);
+    }
+    auto t = std::make_unique<ta3h::PrunerDev>();
+    ta3h::build_pruner_tables(w, act_range, *t, nullptr);
+    hipError_t e = hipMemcpyAsync(ev->dPruner, t.get(), sizeof(ta3h::PrunerDev), hipMemcpyHostToDevice, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy pruner", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("copy pruner", e);
+    ev->prunerOn = true;
+    ev->pruneK = k;
+    ev->pruneMask = ply_mask;
+    ev->traceEvery = 0; // a pruned search records no traces
+    return TA3H_OK;
+}
+
+int ta3h_set_trace(ta3h_evaluator* ev, uint64_t cap_records, uint32_t every) {
+    if(!ev) return TA3H_E_ARG;
+    if(cap_records == 0 || every == 0) {
+        ev->traceEvery = 0;
+        return TA3H_OK;
+    }
+    if(cap_records > 0xFFFFFFFFull) return ev->fail_arg("set_trace: at most 2^32 - 1 records");
+    int r = ev->ensure_ready();
+    if(r) return r;
+    if((r = ev->grow(ev->dTrace, ev->traceCap, static_cast<size_t>(cap_records) * 11, "malloc trace"))) return r;
+    ev->traceCap = static_cast<size_t>(cap_records); // records (the buffer may be larger)
+    if(ev->dTraceCount == nullptr) {
+        hipError_t const e = hipMalloc(&ev->dTraceCount, 64);
+        if(e != hipSuccess) return ev->fail("malloc trace count", e);
+    }
+    ev->traceEvery = every;
+    ev->prunerOn = false; // traces come from the full beam
+    return TA3H_OK;
+}
+
+int ta3h_fetch_trace(ta3h_evaluator* ev, uint32_t* out, uint64_t cap_records, uint64_t* produced, uint64_t* copied) {
+    if(!ev) return TA3H_E_ARG;
+    if(!ev->launched || ev->traceEvery == 0) {
+        ev->err = "fetch_trace: no launch with traces enabled";
+        return TA3H_E_STATE;
+    }
+    uint32_t count = 0;
+    if(ev->items != 0) {
+        hipError_t e = hipMemcpyAsync(&count, ev->dTraceCount, 4, hipMemcpyDeviceToHost, ev->stream);
+        if(e != hipSuccess) return ev->fail("copy trace count", e);
+        e = hipStreamSynchronize(ev->stream);
+        if(e != hipSuccess) return ev->fail("kernel run", e);
+    }
+    if(produced) *produced = count;
+    uint64_t n = count;
+    if(n > ev->traceCap) n = ev->traceCap;
+    if(n > cap_records) n = cap_records;
+    if(out == nullptr) n = 0;
+    if(copied) *copied = n;
+    if(n) {
+        hipError_t e = hipMemcpyAsync(out, ev->dTrace, n * 11 * sizeof(uint32_t), hipMemcpyDeviceToHost, ev->stream);
+        if(e != hipSuccess) return ev->fail("copy trace", e);
+        e = hipStreamSynchronize(ev->stream);
+        if(e != hipSuccess) return ev->fail("copy trace", e);
+    }
+    return TA3H_OK;
+}
+
+namespace {
+
+// device copy of the master weights and of caller samples (null samples = the evaluator's trace buffer)
+int stage_training(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint64_t n, uint32_t const** dSmp) {
+    int r = ev->ensure_ready();
+    if(r) return r;
+    if((r = ev->grow(ev->dPw, ev->pwCap, ta3h::PR_PARAMS, "malloc pruner weights"))) return r;
+    hipError_t e = hipMemcpyAsync(ev->dPw, w, ta3h::PR_PARAMS * sizeof(float), hipMemcpyHostToDevice, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy pruner weights", e);
+    if(samples == nullptr) {
+        if(ev->dTrace == nullptr || n > ev->traceCap) return ev->fail_arg("pruner: trace buffer holds fewer records");
+        *dSmp = ev->dTrace;
+    } else {
+        if((r = ev->grow(ev->dSamples, ev->samplesCap, static_cast<size_t>(n) * 11, "malloc samples"))) return r;
+        e = hipMemcpyAsync(ev->dSamples, samples, static_cast<size_t>(n) * 11 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ev->stream);
+        if(e != hipSuccess) return ev->fail("copy samples", e);
+        *dSmp = ev->dSamples;
+    }
+    return TA3H_OK;
+}
+
+int grow_batch(ta3h_evaluator* ev, size_t b) {
+    int r;
+    if((r = ev->grow(ev->dAct, ev->actCap, b * 4 * ta3h::PR_H, "malloc activations"))) return r;
+    if((r = ev->grow(ev->dDelta, ev->deltaCap, b * 2 * ta3h::PR_H, "malloc deltas"))) return r;
+    if((r = ev->grow(ev->dScore, ev->scoreCap, b, "malloc dscore"))) return r;
+    return TA3H_OK;
+}
+
+} // namespace
+
+int ta3h_pruner_grad(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint32_t n, float pos_weight,
+                     float* grad, float* loss) {
+    if(!ev || !w || !grad || n == 0) return TA3H_E_ARG;
+    uint32_t const* dSmp = nullptr;
+    int r = stage_training(ev, w, samples, n, &dSmp);
+    if(r) return r;
+    if((r = grow_batch(ev, n))) return r;
+    if((r = ev->grow(ev->dLoss, ev->lossCap, n, "malloc loss"))) return r;
+    if((r = ev->grow(ev->dOut, ev->outCap, ta3h::PR_PARAMS, "malloc gradient"))) return r;
+    hipError_t e = ta3h::pruner_fwd_bwd(ev->dPw, dSmp, nullptr, n, 1.0f / static_cast<float>(n), pos_weight, ev->dAct,
+                                        ev->dDelta, ev->dScore, ev->dLoss, ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner forward/backward", e);
+    e = ta3h::pruner_grad_adam(ev->dPw, nullptr, nullptr, ev->dOut, dSmp, nullptr, n, ev->dAct, ev->dDelta, ev->dScore,
+                               0, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner gradient", e);
+    e = hipMemcpyAsync(grad, ev->dOut, ta3h::PR_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy gradient", e);
+    std::vector<float> l(n);
+    e = hipMemcpyAsync(l.data(), ev->dLoss, n * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy loss", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner gradient", e);
+    if(loss) {
+        double s = 0.0;
+        for(float v : l) s += v;
+        *loss = static_cast<float>(s);
+    }
+    return TA3H_OK;
+}
+
+int ta3h_pruner_train(ta3h_evaluator* ev, float* w, const uint32_t* samples, uint64_t n, uint32_t epochs,
+                      uint32_t batch, float lr, float pos_weight, uint64_t seed, float* epoch_loss) {
+    if(!ev || !w || n == 0 || batch == 0 || !(lr > 0.0f)) return TA3H_E_ARG;
+    if(n > 0xFFFFFFFFull) return ev->fail_arg("pruner_train: at most 2^32 - 1 samples");
+    uint32_t const* dSmp = nullptr;
+    int r = stage_training(ev, w, samples, n, &dSmp);
+    if(r) return r;
+    size_t const b = batch < n ? batch : static_cast<size_t>(n);
+    if((r = grow_batch(ev, b))) return r;
+    if((r = ev->grow(ev->dLoss, ev->lossCap, static_cast<size_t>(n), "malloc loss"))) return r;
+    if((r = ev->grow(ev->dOrder, ev->orderCap, static_cast<size_t>(n), "malloc order"))) return r;
+    if((r = ev->grow(ev->dAdam, ev->adamCap, 2 * static_cast<size_t>(ta3h::PR_PARAMS), "malloc adam"))) return r;
+    hipError_t e = hipMemsetAsync(ev->dAdam, 0, 2 * ta3h::PR_PARAMS * sizeof(float), ev->stream);
+    if(e != hipSuccess) return ev->fail("clear adam", e);
+    float* const m = ev->dAdam;
+    float* const v = ev->dAdam + ta3h::PR_PARAMS;
+    float const beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    std::vector<uint32_t> order(static_cast<size_t>(n));
+    std::vector<float> losses(static_cast<size_t>(n));
+    uint64_t st = seed;
+    auto next = [&st]() { // splitmix64
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    uint64_t step = 0;
+    for(uint32_t ep = 0; ep < epochs; ++ep) {
+        for(size_t i = 0; i < order.size(); ++i) order[i] = static_cast<uint32_t>(i);
+        for(size_t i = order.size() - 1; i > 0; --i) std::swap(order[i], order[next() % (i + 1)]); // Fisher-Yates
+        e = hipMemcpyAsync(ev->dOrder, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ev->stream);
+        if(e != hipSuccess) return ev->fail("copy order", e);
+        for(uint64_t b0 = 0; b0 < n; b0 += b) {
+            uint32_t const nb = static_cast<uint32_t>(n - b0 < b ? n - b0 : b);
+            ++step;
+            e = ta3h::pruner_fwd_bwd(ev->dPw, dSmp, ev->dOrder + b0, nb, 1.0f / static_cast<float>(nb), pos_weight,
+                                     ev->dAct, ev->dDelta, ev->dScore, ev->dLoss + b0, ev->stream);
+            if(e != hipSuccess) return ev->fail("pruner forward/backward", e);
+            float const c1 = 1.0f - static_cast<float>(std::pow(0.9, static_cast<double>(step)));
+            float const c2 = 1.0f - static_cast<float>(std::pow(0.999, static_cast<double>(step)));
+            e = ta3h::pruner_grad_adam(ev->dPw, m, v, nullptr, dSmp, ev->dOrder + b0, nb, ev->dAct, ev->dDelta,
+                                       ev->dScore, 1, lr, beta1, beta2, eps, c1, c2, ev->stream);
+            if(e != hipSuccess) return ev->fail("pruner adam", e);
+        }
+        // the next epoch's order upload must not overtake this epoch's kernels
+        e = hipMemcpyAsync(losses.data(), ev->dLoss, losses.size() * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+        if(e != hipSuccess) return ev->fail("copy loss", e);
+        e = hipStreamSynchronize(ev->stream);
+        if(e != hipSuccess) return ev->fail("pruner training", e);
+        if(epoch_loss) { // mean over samples: each batch stored loss / batch size
+            double s = 0.0;
+            for(uint64_t b0 = 0; b0 < n; b0 += b) {
+                uint64_t const nb = n - b0 < b ? n - b0 : b;
+                double bs = 0.0;
+                for(uint64_t i = b0; i < b0 + nb; ++i) bs += losses[i];
+                s += bs * static_cast<double>(nb);
+            }
+            epoch_loss[ep] = static_cast<float>(s / static_cast<double>(n));
+        }
+    }
+    e = hipMemcpyAsync(w, ev->dPw, ta3h::PR_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy pruner weights", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner training", e);
+    return TA3H_OK;
+}
+
+int ta3h_pruner_hidden(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint32_t n, float* out) {
+    if(!ev || !w || !out || n == 0) return TA3H_E_ARG;
+    uint32_t const* dSmp = nullptr;
+    int r = stage_training(ev, w, samples, n, &dSmp);
+    if(r) return r;
+    if((r = ev->grow(ev->dOut, ev->outCap, static_cast<size_t>(n) * ta3h::PR_H, "malloc hidden"))) return r;
+    hipError_t e = ta3h::pruner_hidden_samples(ev->dPw, dSmp, n, ev->dOut, ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner hidden", e);
+    e = hipMemcpyAsync(out, ev->dOut, static_cast<size_t>(n) * ta3h::PR_H * sizeof(float), hipMemcpyDeviceToHost,
+                       ev->stream);
+    if(e != hipSuccess) return ev->fail("copy hidden", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("pruner hidden", e);
+    return TA3H_OK;
+}
+
+int ta3h_pruner_logits(ta3h_evaluator* ev, const float* w, float act_range, const uint32_t* samples, uint32_t n,
+                       float* out) {
+    if(!ev || !w || !out || n == 0) return TA3H_E_ARG;
+    if(!(act_range > 0.0f) || !std::isfinite(act_range)) return ev->fail_arg("pruner_logits: act_range must be > 0");
+    uint32_t const* dSmp = nullptr;
+    int r = stage_training(ev, w, samples, n, &dSmp);
+    if(r) return r;
+    if((r = ev->grow(ev->dOut, ev->outCap, n, "malloc logits"))) return r;
+    // a private table copy: the evaluator's pruner (if any) stays as set
+    ta3h::PrunerDev* dTab = nullptr;
+    hipError_t e = hipMalloc(&dTab, sizeof(ta3h::PrunerDev));
+    if(e != hipSuccess) return ev->fail("malloc pruner", e);
+    auto t = std::make_unique<ta3h::PrunerDev>();
+    ta3h::build_pruner_tables(w, act_range, *t, nullptr);
+    e = hipMemcpyAsync(dTab, t.get(), sizeof(ta3h::PrunerDev), hipMemcpyHostToDevice, ev->stream);
+    if(e == hipSuccess) e = ta3h::pruner_infer_samples(dTab, dSmp, n, ev->dOut, ev->stream);
+    if(e == hipSuccess) e = hipMemcpyAsync(out, ev->dOut, n * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+    hipError_t const es = hipStreamSynchronize(ev->stream);
+    static_cast<void>(hipFree(dTab));
+    if(e != hipSuccess) return ev->fail("pruner logits", e);
+    if(es != hipSuccess) return ev->fail("pruner logits", es);
+    return TA3H_OK;
 }
 
 void ta3h_tables(int8_t* lut_ox, uint8_t* lut_off, uint8_t* tree, uint8_t* tree_count) {
