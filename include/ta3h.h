@@ -115,8 +115,8 @@ void ta3h_tables(int8_t* lut_ox, uint8_t* lut_off, uint8_t* tree, uint8_t* tree_
 /* Master parameters of the pruner (21,249). */
 int ta3h_pruner_num_params(void);
 
-/* Grid exponents {e1, e2, e3, e4, d} of the integer inference form of weights w with first-hidden-layer range
- * act_range (host only). */
+/* Grid exponents {e1, e2, e3, d} of the inference form of weights w with first-hidden-layer range act_range
+ * (host only): W1 on 2^-e1, W2 on 2^-e3, first hidden layer codes rne(P 2^-d) with P on 2^-e1. */
 int ta3h_pruner_quant_info(const float* w, float act_range, int* info);
 
 /* Pruned beam for the following launches: at plies whose bit is set in ply_mask (bit 1, 2: middle levels, bit 3:
@@ -126,7 +126,7 @@ int ta3h_pruner_quant_info(const float* w, float act_range, int* info);
 int ta3h_set_pruner(ta3h_evaluator* ev, const float* w, float act_range, uint32_t k, uint32_t ply_mask);
 
 /* Full-beam traces for the following launches: every legal candidate of plies 1..3 of every `every`-th move
- * (move index % every == 0) of every game, labelled 1 if the level kept it (plies 1, 2) or it is its list's
+ * of every game (the moves whose index is congruent to the game's seed modulo `every`), labelled 1 if the level kept it (plies 1, 2) or it is its list's
  * best leaf (ply 3). Up to cap_records records are kept per launch, in no particular order. cap_records = 0
  * turns traces off. Trace launches run 4 waves per game; results are the full beam's. */
 int ta3h_set_trace(ta3h_evaluator* ev, uint64_t cap_records, uint32_t every);
@@ -152,8 +152,8 @@ int ta3h_pruner_train(ta3h_evaluator* ev, float* w, const uint32_t* samples, uin
  * act_range. */
 int ta3h_pruner_hidden(ta3h_evaluator* ev, const float* w, const uint32_t* samples, uint32_t n, float* out);
 
-/* Inference logits of n samples as the pruned search computes them (integer grids on BF16 MFMA); NaN for a
- * record whose (rotation, column) is no placement of its piece. */
+/* Inference logits of n samples as the pruned search computes them (layers 1-2 on integer grids on BF16 MFMA,
+ * layer 3 FP32 in a fixed order); NaN for a record whose (rotation, column) is no placement of its piece. */
 int ta3h_pruner_logits(ta3h_evaluator* ev, const float* w, float act_range, const uint32_t* samples, uint32_t n,
                        float* out);
 

@@ -361,9 +361,9 @@ def pruner_scores(pw, act_range, samples):
 
 
 def pruner_quant_info(pw, act_range):
-    """grid exponents (e1, e2, e3, e4, d) of the quantized network"""
+    """grid exponents (e1, e2, e3, d) of the quantized network"""
     pw = _f32(pw)
-    info = (C.c_int * 5)()
+    info = (C.c_int * 4)()
     lib().ta3o_pruner_quant_info(_p(pw, C.c_float), float(act_range), info)
     return tuple(info)
 

@@ -733,25 +733,26 @@ int pruner_inputs(Board const& b, int piece, int o, int x, int ply, int* idx) {
     return n;
 }
 
-// Inference form: an integer network held on power-of-two grids (each grid's exponent is the largest that keeps
-// the layer's largest magnitude inside its integer range), so every sum is an exact integer:
+// Inference form: layers 1 and 2 are an integer network on power-of-two grids (each grid's exponent is the
+// largest that keeps the layer's largest magnitude inside its integer range), so every sum is an exact integer
+// below 2^24 whatever the summation order; layer 3 is float32 in a fixed order:
 //   N1 = rne(W1 2^e1), Nb1 = rne(b1 2^e1) (|N1| <= 255, |Nb1| <= 2^20), e1 from max |W1| <= 255
-//   P  = sum over active inputs of N1 + Nb1                         (first-layer pre-activation, |P| < 2^21)
-//   c  = 0 if P <= 0, else min(255, round-half-up(P 2^-d)), d = e1 - e2 clamped to [-8, 24], e2 from act_range
-//   N2 = rne(W2 2^e3), Nb2 = rne(b2 2^(e2 + e3)) (|N2| <= 255, |Nb2| <= 2^22), e3 from max |W2| <= 255
-//   h2 = max(0, sum_u c[u] N2[v][u] + Nb2[v])                      (|.| < 2^23)
-//   N3 = rne(W3 2^e4), e4 from max |W3| <= 32767
-//   logit = float(sum_v h2[v] N3[v])                              (exact integer below 2^45, rounded once)
-// b3 shifts every logit equally and is dropped. act_range is the calibrated upper range of the first hidden
+//   P  = sum over active inputs of N1 + Nb1                      (first-layer pre-activation, |P| < 2^21)
+//   c  = min(255, max(0, rne(P 2^-d))), d = e1 - e2 clamped to [-8, 15], e2 from act_range <= 255
+//   N2 = rne(W2 2^e3), e3 from max |W2| <= 255; Nb2 = rne(b2 2^(e1 - d + e3)), |Nb2| <= 2^22
+//   h2 = max(0, Nb2[v] + sum_u c[u] N2[v][u])                    (exact, |.| < 2^23)
+//   logit = (p0 + p1) + (p2 + p3), p_q = fmaf chain over v = 16 t + 4 q + j (t = 0..3, then j = 0..3) of
+//           h2[v] W3[v], starting from 0                         (float32; W3 unquantized)
+// b3 shifts every logit alike and is not used. act_range is the calibrated upper range of the first hidden
 // layer in float units.
 struct PrunerQ {
     int32_t n1[PR_H][PR_IN];
     int32_t nb1[PR_H];
     int32_t n2[PR_H][PR_H];
     int32_t nb2[PR_H];
-    int32_t n3[PR_H];
+    float w3[PR_H];
     int d;
-    int e1, e2, e3, e4;
+    int e1, e2, e3;
 };
 
 // largest e with m 2^e <= lim (0 when m is 0)
@@ -769,28 +770,25 @@ int32_t rne_scaled(double v, int e, int32_t lim) {
 }
 
 void pruner_quantize(float const* w, float actRange, PrunerQ& q) {
-    double m1 = 0, m2 = 0, m3 = 0;
+    double m1 = 0, m2 = 0;
     for(int i = 0; i < PR_H * PR_IN; ++i) m1 = std::max(m1, std::fabs(static_cast<double>(w[PR_W1 + i])));
     for(int i = 0; i < PR_H * PR_H; ++i) m2 = std::max(m2, std::fabs(static_cast<double>(w[PR_W2 + i])));
-    for(int i = 0; i < PR_H; ++i) m3 = std::max(m3, std::fabs(static_cast<double>(w[PR_W3 + i])));
     q.e1 = grid_exponent(m1, 255.0);
     q.e2 = grid_exponent(actRange, 255.0);
     q.e3 = grid_exponent(m2, 255.0);
-    q.e4 = grid_exponent(m3, 32767.0);
-    q.d = std::min(24, std::max(-8, q.e1 - q.e2));
+    q.d = std::min(15, std::max(-8, q.e1 - q.e2));
     for(int u = 0; u < PR_H; ++u) {
         for(int k = 0; k < PR_IN; ++k) q.n1[u][k] = rne_scaled(w[PR_W1 + u * PR_IN + k], q.e1, 255);
         q.nb1[u] = rne_scaled(w[PR_B1 + u], q.e1, 1 << 20);
         for(int i = 0; i < PR_H; ++i) q.n2[u][i] = rne_scaled(w[PR_W2 + u * PR_H + i], q.e3, 255);
-        q.nb2[u] = rne_scaled(w[PR_B2 + u], q.e2 + q.e3, 1 << 22);
-        q.n3[u] = rne_scaled(w[PR_W3 + u], q.e4, 32767);
+        q.nb2[u] = rne_scaled(w[PR_B2 + u], q.e1 - q.d + q.e3, 1 << 22);
+        q.w3[u] = w[PR_W3 + u];
     }
 }
 
 int32_t pruner_code(int32_t p, int d) {
-    if(p <= 0) return 0;
-    int64_t c = d > 0 ? (static_cast<int64_t>(p) + (int64_t{1} << (d - 1))) >> d : static_cast<int64_t>(p) << -d;
-    return static_cast<int32_t>(std::min<int64_t>(c, 255));
+    double const c = std::nearbyint(std::ldexp(static_cast<double>(p), -d)); // exact scaling, then rne
+    return static_cast<int32_t>(std::min(255.0, std::max(0.0, c)));
 }
 
 float pruner_score(PrunerQ const& q, Board const& b, int piece, int o, int x, int ply) {
@@ -802,13 +800,22 @@ float pruner_score(PrunerQ const& q, Board const& b, int piece, int o, int x, in
         for(int j = 0; j < n; ++j) p += q.n1[u][idx[j]];
         c[u] = pruner_code(p, q.d);
     }
-    int64_t s = 0;
+    float h2[PR_H];
     for(int v = 0; v < PR_H; ++v) {
         int64_t a = q.nb2[v];
         for(int u = 0; u < PR_H; ++u) a += static_cast<int64_t>(c[u]) * q.n2[v][u];
-        if(a > 0) s += a * q.n3[v];
+        h2[v] = static_cast<float>(a > 0 ? a : 0);
     }
-    return static_cast<float>(static_cast<double>(s));
+    float p[4];
+    for(int qq = 0; qq < 4; ++qq) {
+        p[qq] = 0.0f;
+        for(int t = 0; t < 4; ++t)
+            for(int j = 0; j < 4; ++j) {
+                int const v = 16 * t + 4 * qq + j;
+                p[qq] = std::fmaf(h2[v], q.w3[v], p[qq]);
+            }
+    }
+    return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
 struct PrunerRef {
@@ -1231,7 +1238,8 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
         TraceOut tr;
         tr.out = traceOut;
-        tr.on = traceOut != nullptr && traceEvery != 0 && i % traceEvery == 0;
+        // traced moves: index congruent to the game seed modulo traceEvery (spreads them over games)
+        tr.on = traceOut != nullptr && traceEvery != 0 && i % traceEvery == seed % traceEvery;
         SearchResult const r = search_beam(g, model, REF_WIDTHS, ws, cnt, pr, tr);
         if(!r.some) break;
         if(r.hold) engine_hold(g);
@@ -1534,8 +1542,7 @@ void ta3o_pruner_quant_info(const float* w, float act_range, int* info) {
     info[0] = q->e1;
     info[1] = q->e2;
     info[2] = q->e3;
-    info[3] = q->e4;
-    info[4] = q->d;
+    info[3] = q->d;
 }
 
 double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, double pos_weight, int rounded,

@@ -120,7 +120,7 @@ int ta3o_pruner_num_params(void);
 float ta3o_pruner_score(const float* w, float act_range, const uint32_t cols[10], int piece, int o, int x, int ply);
 /* the same for n trace records */
 void ta3o_pruner_scores(const float* w, float act_range, const uint32_t* samples, int n, float* out);
-/* grid exponents of the quantized network: info = {e1, e2, e3, e4, d} */
+/* grid exponents of the quantized network: info = {e1, e2, e3, d} */
 void ta3o_pruner_quant_info(const float* w, float act_range, int* info);
 /* batch-mean weighted BCE and its gradient (float64) over trace records (11 words each);
  * rounded = apply the inference bfloat16 roundings (straight-through) */
@@ -131,7 +131,8 @@ double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, dou
 void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
                               uint32_t max_moves, int threads, const float* pruner_w, float act_range, int k,
                               unsigned ply_mask, float* out, uint32_t* placements_out, ta3o_counters* counters);
-/* trace records of one full-beam game: every legal candidate of plies 1..3 of every `every`-th move:
+/* trace records of one full-beam game: every legal candidate of plies 1..3 of the moves whose index is
+ * congruent to seed modulo `every`:
  * 10 board words + meta (piece | o << 3 | (x+2) << 5 | ply << 9 | label << 11); returns the count
  * (copies at most cap_records) */
 uint64_t ta3o_trace_records(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint32_t every,

@@ -281,9 +281,9 @@ def _samples(samples):
 
 
 def pruner_quant_info(pruner_w, act_range: float):
-    """grid exponents (e1, e2, e3, e4, d) of the integer inference form"""
+    """grid exponents (e1, e2, e3, d) of the integer inference form"""
     w = _pruner_w(pruner_w)
-    info = (C.c_int * 5)()
+    info = (C.c_int * 4)()
     if lib().ta3h_pruner_quant_info(_ptr(w, C.c_float), float(act_range), info) != 0:
         raise HipError("pruner_quant_info: invalid arguments")
     return tuple(info)

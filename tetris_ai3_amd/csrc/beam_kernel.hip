@@ -721,20 +721,13 @@ __device__ inline bf16x8 bits_bf16(uint32_t byte) {
 
 __device__ inline bf16x8 load_bf16x8(uint16_t const* p) { return *reinterpret_cast<bf16x8 const*>(p); }
 
-// First hidden layer code of a pre-activation (an exact integer held in FP32): 0 for P <= 0, else
-// min(255, round-half-up(P 2^-d)); d in [-8, 24] so no shift overflows (|P| < 2^21).
-__device__ inline float act_code(float p, int d) {
-    int32_t const P = static_cast<int32_t>(p);
-    if(P <= 0) return 0.0f;
-    int32_t const c = d > 0 ? (P + (1 << (d - 1))) >> d : P << -d;
-    return static_cast<float>(c < 255 ? c : 255);
-}
-
 // One 16 x 16 tile of the board projection, bp[row][unit] = sum over the row board's set inputs k of N1[unit][k]:
-// rows = 16 boards (this lane's A row is `board`, null = an empty row), columns = hidden units 16 nt .. + 15.
-// v_mfma_f32_16x16x32_bf16 fragments: lane l holds A row / B column l % 16 and K elements 8 (l / 16) .. + 7;
-// D element j of lane l is row 4 (l / 16) + j, column l % 16. Every partial sum is an integer below 2^24.
-__device__ inline f32x4 project_tile(PrunerDev const* __restrict__ pr, uint32_t const* board, uint32_t nt) {
+// rows = 16 boards (this lane's A row is `board`; rowValid = false gives an empty row), columns = hidden units
+// 16 nt .. + 15. v_mfma_f32_16x16x32_bf16 fragments: lane l holds A row / B column l % 16 and K elements
+// 8 (l / 16) .. + 7; D element j of lane l is row 4 (l / 16) + j, column l % 16. Every partial sum is an integer
+// below 2^24.
+__device__ inline f32x4 project_tile(PrunerDev const* __restrict__ pr, uint32_t const* board, bool rowValid,
+                                     uint32_t nt) {
     uint32_t const lane = lane_id(), q = lane >> 4;
     uint32_t const unit = 16 * nt + (lane & 15u);
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -742,52 +735,80 @@ __device__ inline f32x4 project_tile(PrunerDev const* __restrict__ pr, uint32_t 
     for(uint32_t ks = 0; ks < 8; ++ks) {
         PrunerDev const* const pl = opaque_ptr(pr); // keep the weight loads inside the loop
         uint32_t const k0 = 32 * ks + 8 * q;         // board slots k0 .. k0 + 7 lie in one column
-        uint32_t byte = 0;
-        if(board != nullptr && k0 < 240) {
-            uint32_t const c = k0 / 24, r0 = k0 % 24;
-            byte = (board[c] >> (3 + r0)) & (r0 == 16 ? 0xFu : 0xFFu);
-        }
+        uint32_t const c = k0 < 240 ? k0 / 24 : 0, r0 = k0 % 24;
+        uint32_t const colw = board[c];
+        uint32_t byte = (colw >> (3 + r0)) & (r0 == 16 ? 0xFu : 0xFFu);
+        byte = rowValid && k0 < 240 ? byte : 0u;
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bits_bf16(byte), load_bf16x8(&pl->w1b[unit][k0]), acc, 0, 0, 0);
     }
     return acc;
 }
 
-// Pruner logit of the lane's candidate (column l % 16 of a 16-candidate pass) from its board projection row
-// `bpRow` and placement row `ptRow` (null = an empty column). Lane l covers hidden units {8 (l / 16) .. + 7} and
-// {32 + 8 (l / 16) .. + 7}: exactly its B fragments of layer 2, so no data moves between lanes before the
-// MFMAs. Every lane returns its column's logit.
-__device__ inline float pruner_logit(PrunerDev const* __restrict__ pr, float const* bpRow, float const* ptRow) {
+// First hidden layer codes of one candidate column for this lane's MFMA B fragments (hidden units
+// {8 (l / 16) .. + 7} and {32 + 8 (l / 16) .. + 7}) from its scaled board projection row and placement row (both
+// always readable; valid = false gives an empty column). rne(P 2^-d) clamped to [0, 255] is exact
+// (P 2^-d = projection + ptab exactly).
+__device__ inline void pruner_codes(float const* bpRow, float const* ptRow, bool valid, bf16x8& h1a, bf16x8& h1b) {
+    uint32_t const q = lane_id() >> 4;
+    f32x4 const b0 = *reinterpret_cast<f32x4 const*>(bpRow + 8 * q);
+    f32x4 const b1 = *reinterpret_cast<f32x4 const*>(bpRow + 8 * q + 4);
+    f32x4 const b2 = *reinterpret_cast<f32x4 const*>(bpRow + 32 + 8 * q);
+    f32x4 const b3 = *reinterpret_cast<f32x4 const*>(bpRow + 32 + 8 * q + 4);
+    f32x4 const p0 = *reinterpret_cast<f32x4 const*>(ptRow + 8 * q);
+    f32x4 const p1 = *reinterpret_cast<f32x4 const*>(ptRow + 8 * q + 4);
+    f32x4 const p2 = *reinterpret_cast<f32x4 const*>(ptRow + 32 + 8 * q);
+    f32x4 const p3 = *reinterpret_cast<f32x4 const*>(ptRow + 32 + 8 * q + 4);
+    float const hi = valid ? 255.0f : 0.0f; // an empty column clamps to 0
+#pragma unroll
+    for(int j = 0; j < 4; ++j) {
+        h1a[j] = static_cast<__bf16>(__builtin_amdgcn_fmed3f(__builtin_rintf(b0[j] + p0[j]), 0.0f, hi));
+        h1a[4 + j] = static_cast<__bf16>(__builtin_amdgcn_fmed3f(__builtin_rintf(b1[j] + p1[j]), 0.0f, hi));
+        h1b[j] = static_cast<__bf16>(__builtin_amdgcn_fmed3f(__builtin_rintf(b2[j] + p2[j]), 0.0f, hi));
+        h1b[4 + j] = static_cast<__bf16>(__builtin_amdgcn_fmed3f(__builtin_rintf(b3[j] + p3[j]), 0.0f, hi));
+    }
+}
+
+// Layers 2 and 3 for T candidate columns (MFMA B operands h1a/h1b[c]): the A fragments of output tile t
+// (rows 16 t .. + 15) and its bias / W3 slice are loaded once for all T columns, and the next tile's are in flight
+// while this tile's MFMAs run. Returns in every lane of column c its logit (p0 + p1) + (p2 + p3), p_q the FMA
+// chain of lane quarter q over v = 16 t + 4 q + j.
+template<int T>
+__device__ inline void pruner_layers23(PrunerDev const* __restrict__ pr, bf16x8 const (&h1a)[T],
+                                       bf16x8 const (&h1b)[T], float (&logit)[T]) {
     uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
-    int const d = pr->actShift;
-    bf16x8 h1a, h1b;
+    float part[T];
 #pragma unroll
-    for(int j = 0; j < 8; ++j) {
-        float ca = 0.0f, cb = 0.0f;
-        if(ptRow != nullptr) {
-            ca = act_code(bpRow[8 * q + j] + ptRow[8 * q + j], d);
-            cb = act_code(bpRow[32 + 8 * q + j] + ptRow[32 + 8 * q + j], d);
-        }
-        h1a[j] = static_cast<__bf16>(ca);
-        h1b[j] = static_cast<__bf16>(cb);
-    }
-    double part = 0.0;
+    for(int c = 0; c < T; ++c) part[c] = 0.0f;
+    bf16x8 const* fr = reinterpret_cast<bf16x8 const*>(&pr->w2f[r16][q][0]);
+    f32x4 const* bw = reinterpret_cast<f32x4 const*>(&pr->bw[0][q][0]);
+    constexpr uint32_t FR_STRIDE = 16 * 4 * 16 * 2 / 16; // bf16x8 per output tile (16 rows)
+    constexpr uint32_t BW_STRIDE = 4 * 8 * 4 / 16;      // f32x4 per output tile
+    bf16x8 a0 = fr[0], a1 = fr[1];
+    f32x4 bias = bw[0], w3 = bw[1];
 #pragma unroll 1
-    for(uint32_t mt = 0; mt < 4; ++mt) { // rolled: one output tile's weights live at a time
-        PrunerDev const* const pl = pr;
-        uint32_t const out = 16 * mt + r16; // A row: output unit
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(&pl->w2b[out][8 * q]), h1a, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(&pl->w2b[out][32 + 8 * q]), h1b, acc, 0, 0, 0);
+    for(uint32_t t = 0; t < 4; ++t) {
+        uint32_t const nt = t < 3 ? t + 1 : t; // prefetch the next tile (the last re-reads its own)
+        bf16x8 const na0 = fr[nt * FR_STRIDE], na1 = fr[nt * FR_STRIDE + 1];
+        f32x4 const nbias = bw[nt * BW_STRIDE], nw3 = bw[nt * BW_STRIDE + 1];
 #pragma unroll
-        for(int j = 0; j < 4; ++j) {
-            uint32_t const o = 16 * mt + 4 * q + j;
-            float const h2 = acc[j] + pl->b2[o]; // exact: |acc| < 2^22, |Nb2| <= 2^22
-            if(h2 > 0.0f) part = __fma_rn(static_cast<double>(h2), static_cast<double>(pl->w3[o]), part); // exact
+        for(int c = 0; c < T; ++c) {
+            f32x4 acc = bias; // D rows 4 q + j start at Nb2
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, h1a[c], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, h1b[c], acc, 0, 0, 0);
+#pragma unroll
+            for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(__builtin_fmaxf(acc[j], 0.0f), w3[j], part[c]);
         }
+        a0 = na0;
+        a1 = na1;
+        bias = nbias;
+        w3 = nw3;
     }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    return static_cast<float>(part);
+#pragma unroll
+    for(int c = 0; c < T; ++c) {
+        part[c] += __shfl_xor(part[c], 16); // (p0 + p1), (p2 + p3)
+        part[c] += __shfl_xor(part[c], 32); // (p0 + p1) + (p2 + p3)
+        logit[c] = part[c];
+    }
 }
 
 // Board projection of the frontier's kept nodes: bp[p][u] for p < count (32 x 64, eight tiles).
@@ -799,42 +820,93 @@ __device__ void pruner_board_projection(GameLds const& L, PrunerDev const* __res
     for(uint32_t t = wv; t < 8; t += G) {
         uint32_t const mt = t >> 2, nt = t & 3u;
         uint32_t const p = 16 * mt + r16;
-        f32x4 const acc = project_tile(pr, p < count ? &L.boards[(cur * 32 + p) * WIDTH] : nullptr, nt);
+        f32x4 const acc = project_tile(pr, &L.boards[(cur * 32 + (p < count ? p : 0u)) * WIDTH], p < count, nt);
+        float const sc = pr->projScale; // a power of two: exact
 #pragma unroll
-        for(int j = 0; j < 4; ++j) bp[16 * mt + 4 * q + j][16 * nt + r16] = acc[j];
+        for(int j = 0; j < 4; ++j) bp[16 * mt + 4 * q + j][16 * nt + r16] = acc[j] * sc;
     }
 }
 
-// Pruner logits of a stage's candidates, 16 per pass; illegal candidates get -inf.
+// pruner logits are stored list-major: list s occupies prs[36 s .. 36 s + 35] (16-byte aligned rows)
+constexpr uint32_t PRS_ROW = 36;
+
+// Legality of a stage's candidates, one per lane: prs[36 s + i] = -inf for illegal placement i of list s; the
+// legal ones are listed (any order, as list << 6 | placement index) in `legalList`. Returns their count.
 template<int G>
-__device__ void pruner_scores(GameLds const& L, PrunerDev const* __restrict__ pr, uint32_t total, uint32_t cur,
-                              uint32_t ply, float const (*bp)[PR_H], float* prs, uint32_t wv) {
-    uint32_t const lane = lane_id(), r16 = lane & 15u;
-    uint32_t const groups = (total + 15) / 16;
-    for(uint32_t gi = wv; gi < groups; gi += G) {
-        uint32_t const f = 16 * gi + r16;
+__device__ uint32_t pruner_legal(GameLds& L, uint32_t total, float* prs, uint16_t* legalList, uint32_t wv) {
+    uint32_t const lane = lane_id();
+    if(wv == 0 && lane == 0) L.keptCount = 0;
+    group_sync<G>();
+    for(uint32_t base = wv * 64; base < total; base += 64 * G) {
+        uint32_t const f = base + lane;
         bool legal = false;
-        float const* bpRow = nullptr;
-        float const* ptRow = nullptr;
         if(f < total) {
             uint32_t const si = L.candSeg[f];
+            uint32_t const info = L.segInfo[si >> 6];
+            uint32_t const piece = (info >> 8) & 7u;
+            LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + (si & 63u)];
+            int drop = 0;
+            legal = fit_span(&L.boards[(info & 0x7Fu) * WIDTH], e.cols, e.col0, drop);
+            if(!legal) prs[(si >> 6) * PRS_ROW + (si & 63u)] = -__builtin_huge_valf();
+        }
+        uint64_t const bal = __ballot(legal);
+        uint32_t const cnt = static_cast<uint32_t>(__popcll(bal));
+        uint32_t b = 0;
+        if(lane == 0 && cnt != 0) b = atomicAdd(&L.keptCount, cnt);
+        b = uniform(b);
+        if(legal)
+            legalList[b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)))] =
+                static_cast<uint16_t>(L.candSeg[f]);
+    }
+    group_sync<G>();
+    return uniform(L.keptCount);
+}
+
+// first placement-list entry of each piece (lutOff), one byte per piece
+__device__ inline uint32_t lut_base(uint32_t piece) {
+    constexpr uint64_t packed = [] {
+        uint64_t v = 0;
+        for(int p = 0; p < NUM_PIECES; ++p) v |= static_cast<uint64_t>(TABLES.lutOff[p]) << (8 * p);
+        return v;
+    }();
+    return static_cast<uint32_t>(packed >> (8 * piece)) & 0xFFu;
+}
+
+// Pruner logits of the listed legal candidates, 32 per pass (two MFMA column tiles share the weight loads):
+// prs[36 s + i] = logit.
+template<int G>
+__device__ void pruner_scores(GameLds const& L, PrunerDev const* __restrict__ pr, uint16_t const* legalList,
+                              uint32_t nLegal, uint32_t cur, uint32_t ply, float const (*bp)[PR_H], float* prs,
+                              uint32_t wv) {
+    constexpr int T = 2;
+    uint32_t const lane = lane_id(), r16 = lane & 15u;
+    uint32_t const groups = (nLegal + 16 * T - 1) / (16 * T);
+    for(uint32_t gi = wv; gi < groups; gi += G) {
+        bf16x8 h1a[T], h1b[T];
+        uint32_t f[T];
+        bool valid[T];
+#pragma unroll
+        for(int c = 0; c < T; ++c) {
+            uint32_t const t = 16 * (T * gi + c) + r16;
+            valid[c] = t < nLegal;
+            uint32_t const si = legalList[valid[c] ? t : 0u]; // nLegal > 0 here
             uint32_t const s = si >> 6, i = si & 63u;
             uint32_t const info = L.segInfo[s];
-            uint32_t const pb = info & 0x7Fu, piece = (info >> 8) & 7u;
-            uint32_t const li = c_tab.lutOff[piece] + i;
-            LutEntry const e = c_tab.lut[li];
-            int drop = 0;
-            legal = fit_span(&L.boards[pb * WIDTH], e.cols, e.col0, drop);
-            bpRow = bp[pb - cur * 32];
-            ptRow = pr->ptab[ply - 1][li];
+            f[c] = s * PRS_ROW + i;
+            pruner_codes(bp[(info & 0x7Fu) - cur * 32], pr->ptab[ply - 1][lut_base((info >> 8) & 7u) + i], valid[c],
+                         h1a[c], h1b[c]);
         }
-        float const logit = pruner_logit(pr, bpRow, ptRow);
-        if(lane < 16 && f < total) prs[f] = legal ? logit : -__builtin_huge_valf();
+        float logit[T];
+        pruner_layers23<T>(pr, h1a, h1b, logit);
+#pragma unroll
+        for(int c = 0; c < T; ++c)
+            if(lane < 16 && valid[c]) prs[f[c]] = logit[c];
     }
 }
 
 // Keeps the K best legal placements of every list by pruner logit (logit descending, placement index ascending);
-// returns how many were kept and lists them in `kept` (segment << 6 | placement index, any order).
+// returns how many were kept and lists them in `kept` (list << 6 | placement index, any order). A lane reads its
+// whole list's logits with nine 16-byte loads (lanes of one list read the same addresses).
 template<int G>
 __device__ uint32_t pruner_select(GameLds& L, uint32_t total, float const* prs, uint16_t* kept, uint32_t K, uint32_t wv) {
     uint32_t const lane = lane_id();
@@ -847,14 +919,19 @@ __device__ uint32_t pruner_select(GameLds& L, uint32_t total, float const* prs, 
         if(f < total) {
             si = L.candSeg[f];
             uint32_t const s = si >> 6, i = si & 63u;
-            float const pf = prs[f];
+            f32x4 const* row = reinterpret_cast<f32x4 const*>(prs + s * PRS_ROW);
+            float const pf = prs[s * PRS_ROW + i];
             if(pf != -__builtin_huge_valf()) {
-                uint32_t const off = L.segOff[s];
-                uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - off;
+                uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - L.segOff[s];
                 uint32_t rank = 0;
-                for(uint32_t j = 0; j < n; ++j) {
-                    float const pj = prs[off + j];
-                    rank += (pj > pf || (pj == pf && j < i)) ? 1u : 0u;
+#pragma unroll
+                for(uint32_t v = 0; v < PRS_ROW / 4; ++v) {
+                    f32x4 const x = row[v];
+#pragma unroll
+                    for(uint32_t e = 0; e < 4; ++e) {
+                        uint32_t const j = 4 * v + e;
+                        rank += (j < n && (x[e] > pf || (x[e] == pf && j < i))) ? 1u : 0u;
+                    }
                 }
                 keep = rank < K;
             }
@@ -969,7 +1046,7 @@ __device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, ui
 
 // Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
 // logits share the lanes' candidate-board scratch, which is idle until the value network runs).
-struct PruneLds {
+struct alignas(16) PruneLds {
     float bp[32][PR_H];
     uint16_t kept[32 * 2 * 34];
 };
@@ -1024,9 +1101,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
         uint32_t evalTotal = total;
         if constexpr(MODE == MODE_PRUNED) {
             if((a.pruneMask >> level) & 1u) {
+                uint32_t const nl = pruner_legal<G>(L, total, prs, P->kept, wv);
                 pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
                 group_sync<G>();
-                pruner_scores<G>(L, a.pruner, total, cur, static_cast<uint32_t>(level), P->bp, prs, wv);
+                pruner_scores<G>(L, a.pruner, P->kept, nl, cur, static_cast<uint32_t>(level), P->bp, prs, wv);
                 group_sync<G>();
                 evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
                 seg = P->kept;
@@ -1072,9 +1150,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
     uint32_t evalTotal = total;
     if constexpr(MODE == MODE_PRUNED) {
         if((a.pruneMask >> 3) & 1u) {
+            uint32_t const nl = pruner_legal<G>(L, total, prs, P->kept, wv);
             pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
             group_sync<G>();
-            pruner_scores<G>(L, a.pruner, total, cur, 3u, P->bp, prs, wv);
+            pruner_scores<G>(L, a.pruner, P->kept, nl, cur, 3u, P->bp, prs, wv);
             group_sync<G>();
             evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
             seg = P->kept;
@@ -1145,9 +1224,9 @@ constexpr int WAVES_PER_SIMD = KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4;
 template<int KIND, int G, int MODE = MODE_PLAIN>
 __global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND, MODE>, 8)))
 void beam_eval_kernel(EvalArgs a) {
-    static_assert(MODE == MODE_PLAIN || G * WIDTH * SCR_STRIDE >= 32 * 2 * 34, "pruner logits share the scratch");
+    static_assert(MODE == MODE_PLAIN || G * WIDTH * SCR_STRIDE >= 64 * PRS_ROW, "pruner logits share the scratch");
     __shared__ GameLds L;
-    __shared__ uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
+    __shared__ __attribute__((aligned(16))) uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
     uint32_t const wv = uniform(threadIdx.x >> 6);
@@ -1171,9 +1250,12 @@ void beam_eval_kernel(EvalArgs a) {
         group_sync<G>();
         uint64_t nApply = 0;
         uint32_t nLegal = 0;
+        // traced moves: index congruent to the game seed modulo traceEvery (spreads them over the games)
+        uint32_t const tracePhase =
+            MODE == MODE_TRACE && a.traceEvery != 0 ? uniform(static_cast<uint32_t>(a.seeds[g] % a.traceEvery)) : 0u;
         for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
             if(uniform(L.eng.gameOver) != 0) break;
-            bool const traceMove = MODE == MODE_TRACE && a.traceEvery != 0 && mv % a.traceEvery == 0;
+            bool const traceMove = MODE == MODE_TRACE && a.traceEvery != 0 && mv % a.traceEvery == tracePhase;
             if(!play_move<KIND, G, MODE>(a, L, pruneLds, w, wv, col, traceMove, nApply, nLegal)) break;
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
@@ -1257,17 +1339,18 @@ __global__ __launch_bounds__(64) void pruner_infer_kernel(PrunerDev const* __res
     uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
     uint32_t const base = blockIdx.x * 16;
     uint32_t const t = base + r16;
-    uint32_t const* rec = t < n ? samples + 11ull * t : nullptr;
+    uint32_t const* rec = samples + 11ull * (t < n ? t : 0u);
 #pragma unroll 1
     for(uint32_t nt = 0; nt < 4; ++nt) {
-        f32x4 const acc = project_tile(pr, rec, nt);
+        f32x4 const acc = project_tile(pr, rec, t < n, nt);
+        float const sc = pr->projScale;
 #pragma unroll
-        for(int j = 0; j < 4; ++j) bp[4 * q + j][16 * nt + r16] = acc[j];
+        for(int j = 0; j < 4; ++j) bp[4 * q + j][16 * nt + r16] = acc[j] * sc;
     }
     __syncthreads();
-    float const* ptRow = nullptr;
+    float const* ptRow = pr->ptab[0][0];
     bool found = false;
-    if(rec != nullptr) {
+    if(t < n) {
         uint32_t const meta = rec[10];
         uint32_t const piece = meta & 7u, ox = ((meta >> 3) & 3u) | (((meta >> 5) & 15u) << 2);
         uint32_t const ply = (meta >> 9) & 3u;
@@ -1280,8 +1363,11 @@ __global__ __launch_bounds__(64) void pruner_infer_kernel(PrunerDev const* __res
             }
         }
     }
-    float const logit = pruner_logit(pr, bp[r16], ptRow);
-    if(lane < 16 && t < n) out[t] = found ? logit : __builtin_nanf("");
+    bf16x8 h1a[1], h1b[1];
+    pruner_codes(bp[r16], ptRow, found, h1a[0], h1b[0]);
+    float logit[1];
+    pruner_layers23<1>(pr, h1a, h1b, logit);
+    if(lane < 16 && t < n) out[t] = found ? logit[0] : __builtin_nanf("");
 }
 } // namespace
 

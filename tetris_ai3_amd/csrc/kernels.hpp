@@ -31,7 +31,8 @@ struct EvalArgs {
     PrunerDev const* pruner;
     uint32_t pruneK;
     uint32_t pruneMask;
-    // full-beam traces: every legal candidate of plies 1..3 of every traceEvery-th move (null = off)
+    // full-beam traces: every legal candidate of plies 1..3 of the moves whose index is congruent to the game
+    // seed modulo traceEvery (null = off)
     uint32_t* trace;      // [traceCap][11] records: parent board (10 words) + meta
     uint32_t* traceCount; // records produced (may exceed traceCap; the excess is dropped)
     uint32_t traceCap;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace ta3h {
@@ -10,34 +11,37 @@ inline constexpr int PR_IN = 265, PR_H = 64;
 inline constexpr int PR_W1 = 0, PR_B1 = PR_W1 + PR_H * PR_IN, PR_W2 = PR_B1 + PR_H, PR_B2 = PR_W2 + PR_H * PR_H,
                      PR_W3 = PR_B2 + PR_H, PR_B3 = PR_W3 + PR_H, PR_PARAMS = PR_B3 + 1;
 
-// Inference form of a pruner: an integer network on power-of-two grids, so every sum the device forms is an
-// exact integer below 2^24 and the FP32 accumulation of the BF16 MFMAs is exact in any order:
+// Inference form of a pruner: layers 1 and 2 are an integer network on power-of-two grids, so every sum the
+// BF16 MFMAs form is an exact integer below 2^24 and their FP32 accumulation is exact in any order; layer 3 is
+// FP32 in a fixed order:
 //   N1 = rne(W1 2^e1), e1 the largest exponent with max |W1| 2^e1 <= 255; Nb1 = rne(b1 2^e1), |Nb1| <= 2^20
 //   P  = sum of N1 over the active inputs + Nb1                   (first-layer pre-activation, |P| < 2^21)
-//   c  = 0 if P <= 0, else min(255, round-half-up(P 2^-d)); d = e1 - e2 clamped to [-8, 24], e2 the largest
-//        exponent with actRange 2^e2 <= 255 (actRange = calibrated upper range of the first hidden layer)
-//   N2 = rne(W2 2^e3), e3 from max |W2| <= 255; Nb2 = rne(b2 2^(e2 + e3)), |Nb2| <= 2^22
-//   h2 = max(0, sum_u c[u] N2[v][u] + Nb2[v])                     (|.| < 2^23)
-//   N3 = rne(W3 2^e4), e4 from max |W3| <= 32767
-//   logit = float(sum_v h2[v] N3[v])                              (exact below 2^45 in FP64, rounded once)
+//   c  = min(255, max(0, rne(P 2^-d))); d = e1 - e2 clamped to [-8, 15], e2 the largest exponent with
+//        actRange 2^e2 <= 255 (actRange = calibrated upper range of the first hidden layer)
+//   N2 = rne(W2 2^e3), e3 from max |W2| <= 255; Nb2 = rne(b2 2^(e1 - d + e3)), |Nb2| <= 2^22
+//   h2 = max(0, Nb2[v] + sum_u c[u] N2[v][u])                     (exact, |.| < 2^23)
+//   logit = (p0 + p1) + (p2 + p3), p_q = FMA chain over v = 16 t + 4 q + j (t = 0..3, then j = 0..3) of
+//           h2[v] W3[v] from 0 (FP32, W3 unquantized)
 // b3 shifts every logit alike and is not used. Tables:
 //  w1b[u][k]     N1 board part as bfloat16 (exact: |N1| <= 255), k < 240 (slots 240..255 zero) -- MFMA B
 //                operand of the board projection
-//  ptab[p][e][u] Nb1[u] + N1[u][piece] + N1[u][rotation] + N1[u][column] + N1[u][ply p + 1] for placement-list
-//                entry e (exact integers in FP32)
-//  w2b[v][u]     N2 as bfloat16 -- MFMA A operand of layer 2
-//  b2[v], w3[v]  Nb2 and N3 (exact integers in FP32); actShift = d
+//  ptab[p][e][u] (Nb1[u] + N1[u][piece] + N1[u][rotation] + N1[u][column] + N1[u][ply p + 1]) 2^-d for
+//                placement-list entry e (exact in FP32)
+//  projScale     2^-d (the board projection is stored scaled, so P 2^-d = projection + ptab exactly)
+//  w2f[v][q]     MFMA A fragments of layer 2 for output row v and lane quarter q: N2[v][8q .. 8q + 7] then
+//                N2[v][32 + 8q .. 32 + 8q + 7] as bfloat16 (one 32-byte load per lane)
+//  bw[t][q]      Nb2[16t + 4q .. + 3] (the layer-2 accumulator's initial values) then W3[16t + 4q .. + 3]
 struct PrunerDev {
     uint16_t w1b[PR_H][256];
     float ptab[3][162][PR_H];
-    uint16_t w2b[PR_H][PR_H];
-    float b2[PR_H];
-    float w3[PR_H];
-    int32_t actShift;
+    uint16_t w2f[PR_H][4][16];
+    float bw[4][4][8];
+    float projScale;
     int32_t pad[15];
 };
+static_assert(offsetof(PrunerDev, w2f) % 32 == 0 && offsetof(PrunerDev, bw) % 32 == 0, "32-byte vector loads");
 
-// builds the inference tables from the FP32 master weights; exps (optional) gets {e1, e2, e3, e4, d}
+// builds the inference tables from the FP32 master weights; exps (optional) gets {e1, e2, e3, d}
 void build_pruner_tables(float const* w, float actRange, PrunerDev& out, int* exps);
 
 // training (pruner_train.hip)

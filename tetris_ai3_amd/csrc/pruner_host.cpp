@@ -45,14 +45,12 @@ void build_pruner_tables(float const* w, float actRange, PrunerDev& out, int* ex
     int const e1 = grid_exp(max_abs(w + PR_W1, PR_H * PR_IN), 255.0);
     int const e2 = grid_exp(static_cast<double>(actRange), 255.0);
     int const e3 = grid_exp(max_abs(w + PR_W2, PR_H * PR_H), 255.0);
-    int const e4 = grid_exp(max_abs(w + PR_W3, PR_H), 32767.0);
-    int const d = std::clamp(e1 - e2, -8, 24);
+    int const d = std::clamp(e1 - e2, -8, 15);
     if(exps) {
         exps[0] = e1;
         exps[1] = e2;
         exps[2] = e3;
-        exps[3] = e4;
-        exps[4] = d;
+        exps[3] = d;
     }
     std::memset(&out, 0, sizeof(out));
     auto n1 = [&](int u, int k) { return to_grid(w[PR_W1 + u * PR_IN + k], e1, 255.0); };
@@ -65,13 +63,17 @@ void build_pruner_tables(float const* w, float actRange, PrunerDev& out, int* ex
                     int const e = TABLES.lutOff[piece] + i;
                     int const o = TABLES.lut[e].ox & 3, xi = TABLES.lut[e].ox >> 2; // xi = x + 2
                     int32_t const p = nb1 + n1(u, 240 + piece) + n1(u, 247 + o) + n1(u, 251 + xi) + n1(u, 262 + ply - 1);
-                    out.ptab[ply - 1][e][u] = static_cast<float>(p);
+                    out.ptab[ply - 1][e][u] = static_cast<float>(std::ldexp(static_cast<double>(p), -d)); // exact
                 }
-        for(int i = 0; i < PR_H; ++i) out.w2b[u][i] = bf16_bits_exact(to_grid(w[PR_W2 + u * PR_H + i], e3, 255.0));
-        out.b2[u] = static_cast<float>(to_grid(w[PR_B2 + u], e2 + e3, 4194304.0));
-        out.w3[u] = static_cast<float>(to_grid(w[PR_W3 + u], e4, 32767.0));
+        for(int i = 0; i < PR_H; ++i) { // input i sits in quarter (i % 32) / 8, half i / 32
+            uint16_t const v = bf16_bits_exact(to_grid(w[PR_W2 + u * PR_H + i], e3, 255.0));
+            out.w2f[u][(i % 32) / 8][8 * (i / 32) + i % 8] = v;
+        }
+        int const t = u / 16, q = (u % 16) / 4, j = u % 4; // output u = 16 t + 4 q + j
+        out.bw[t][q][j] = static_cast<float>(to_grid(w[PR_B2 + u], e1 - d + e3, 4194304.0));
+        out.bw[t][q][4 + j] = w[PR_W3 + u];
     }
-    out.actShift = d;
+    out.projScale = static_cast<float>(std::ldexp(1.0, -d));
 }
 
 } // namespace ta3h
