@@ -105,6 +105,20 @@ int ta3h_feature_norms(float* out, uint32_t n);
  * No reference counterpart. */
 void ta3h_tables(int8_t* lut_ox, uint8_t* lut_off, uint8_t* tree, uint8_t* tree_count);
 
+/* ---- one CMA-ES generation as a captured HIP graph --------------------------------------------------------
+ * No reference counterpart (the reference launches per island-generation, model_eval.cu:97-153). Captures, on the
+ * evaluator's stream: weights (pinned host staging) -> device, queue reset, the search kernel under the current
+ * model / launch / pruner / trace settings, fitness and placements -> pinned host. Replays differ only in the
+ * weights. Replaces stage + launch + fetch for callers that run the same (num_models, seeds, max_moves) every
+ * generation; results equal ta3h_eval's. ta3h_stage / ta3h_eval and every ta3h_set_* call discard the graph. */
+int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_params, const uint64_t* seeds,
+                       uint32_t games, uint32_t max_moves);
+
+/* Replays the generation graph with new model-major weights (num_models x num_params as prepared); one host
+ * synchronisation. out_fitness[m*games+g], out_placements (optional) as ta3h_fetch. ta3h_last_kernel_ms then
+ * reports the whole replay (copies included). */
+int ta3h_graph_run(ta3h_evaluator* ev, const float* weights, float* out_fitness, uint32_t* out_placements);
+
 /* ---- learned pruner (placement-proposal network) and its training traces -------------------------------
  * No reference counterpart: the reference scores every placement of every kept node (ai/search/beam.hpp:337-372,
  * 407-443). The network: 265 inputs (board slot 24c + r = bit 3 + r of column c, r < 20; one-hots piece 240 + p,

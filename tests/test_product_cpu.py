@@ -79,3 +79,10 @@ def test_pruner_quantization_matches_oracle():
         w = rng.normal(0, 10 ** rng.uniform(-3, 2), T.pruner_num_params()).astype(np.float32)
         act = float(10 ** rng.uniform(-3, 3))
         assert T.pruner_quant_info(w, act) == O.pruner_quant_info(w, act)
+
+
+def test_graph_run_without_prepare_is_a_state_error():
+    ev = T.ModelEvaluator()
+    ev._graph_models, ev._graph_items = 1, 1
+    with pytest.raises(T.HipError, match="no generation graph"):
+        ev.graph_run(np.zeros(97, dtype=np.float32))

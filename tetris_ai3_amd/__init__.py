@@ -56,6 +56,8 @@ def lib():
             "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
             "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
+            "ta3h_graph_prepare": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
+            "ta3h_graph_run": (C.c_int, [vp, f32p, f32p, u32p]),
             "ta3h_pruner_num_params": (C.c_int, []),
             "ta3h_pruner_quant_info": (C.c_int, [f32p, C.c_float, C.POINTER(C.c_int)]),
             "ta3h_set_pruner": (C.c_int, [vp, f32p, C.c_float, C.c_uint32, C.c_uint32]),
@@ -188,6 +190,26 @@ class ModelEvaluator:
         g, w, c = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
         self._check(lib().ta3h_launch_info(self._h, C.byref(g), C.byref(w), C.byref(c)))
         return {"grid_blocks": g.value, "waves_per_game": w.value, "num_cus": c.value}
+
+    # --- one generation as a captured HIP graph ----------------------------------------------------
+    def graph_prepare(self, num_models: int, seeds, max_moves: int):
+        """captures weights-in, search, fitness-out for this (num_models, seeds, max_moves)"""
+        s = np.ascontiguousarray(seeds, dtype=np.uint64).reshape(-1)
+        self._graph_items = int(num_models) * s.size
+        self._graph_models = int(num_models)
+        self._items = self._graph_items
+        self._check(lib().ta3h_graph_prepare(self._h, int(num_models), self.num_params, _ptr(s, C.c_uint64), s.size,
+                                             int(max_moves)))
+
+    def graph_run(self, weights):
+        """one replay with new weights: (fitness, placements)"""
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        if w.size != self._graph_models * self.num_params:
+            raise HipError("graph_run: weights must hold num_models * NUM_PARAMS values")
+        fit = np.zeros(self._graph_items, dtype=np.float32)
+        pl = np.zeros(self._graph_items, dtype=np.uint32)
+        self._check(lib().ta3h_graph_run(self._h, _ptr(w, C.c_float), _ptr(fit, C.c_float), _ptr(pl, C.c_uint32)))
+        return fit, pl
 
     # --- learned pruner ------------------------------------------------------------------------------
     def set_pruner(self, pruner_w, act_range: float, k: int = 8, ply_mask: int = 0b1100):

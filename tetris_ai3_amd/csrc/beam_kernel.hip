@@ -86,11 +86,6 @@ __device__ inline uint32_t wave_excl_scan(uint32_t x) {
     return v - x;
 }
 
-__device__ inline uint32_t wave_or_u32(uint32_t x) {
-#pragma unroll
-    for(int m = 32; m > 0; m >>= 1) x |= shfl_xor_u32(x, m);
-    return x;
-}
 
 __device__ inline uint64_t wave_or_u64(uint64_t x) {
 #pragma unroll
@@ -1215,14 +1210,15 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
 // ---------------------------------------------------------------------------------------------------
 // Waves per SIMD the register allocation is held to: the largest occupancy each network's code fits
 // without spilling registers to memory (V11: <= 102 VGPRs, 16-32-32-1: <= 128). The pruned kernel's LDS
-// caps it at 4 workgroups of 4 waves per CU, so it gets the 128 registers of 4 waves per SIMD.
-template<int KIND, int MODE>
-constexpr int WAVES_PER_SIMD = KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4;
+// caps it at 4 workgroups of 4 waves per CU, so it gets the 128 registers of 4 waves per SIMD; one wave per
+// game needs more registers (3 waves per SIMD).
+template<int KIND, int MODE, int G>
+constexpr int WAVES_PER_SIMD = G == 1 ? 3 : (KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4);
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
 template<int KIND, int G, int MODE = MODE_PLAIN>
-__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND, MODE>, 8)))
+__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND, MODE, G>, 8)))
 void beam_eval_kernel(EvalArgs a) {
     static_assert(MODE == MODE_PLAIN || G * WIDTH * SCR_STRIDE >= 64 * PRS_ROW, "pruner logits share the scratch");
     __shared__ GameLds L;

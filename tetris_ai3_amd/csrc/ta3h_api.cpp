@@ -101,6 +101,17 @@ struct ta3h_evaluator {
     float* dOut = nullptr;
     size_t outCap = 0;
 
+    // one CMA-ES generation as a captured graph: pinned weights -> device, search, fitness -> pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    float* hWeights = nullptr; // pinned, repacked layout
+    size_t hWeightsCap = 0;
+    float* hFitness = nullptr; // pinned
+    size_t hFitnessCap = 0;
+    uint32_t* hPlacements = nullptr;
+    size_t hPlacementsCap = 0;
+    uint32_t graphModels = 0;
+
     int kind = TA3H_MODEL_V11;
     int wavesPerBlock = 2; // waves cooperating on one game
     int persistent = 1;
@@ -161,8 +172,34 @@ struct ta3h_evaluator {
         return TA3H_OK;
     }
 
+    void drop_graph() {
+        if(graphExec) static_cast<void>(hipGraphExecDestroy(graphExec));
+        if(graph) static_cast<void>(hipGraphDestroy(graph));
+        graphExec = nullptr;
+        graph = nullptr;
+    }
+
+    template<class T>
+    int grow_pinned(T*& ptr, size_t& cap, size_t need, char const* what) {
+        if(need <= cap) return TA3H_OK;
+        if(ptr != nullptr) {
+            hipError_t const e = hipHostFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            if(e != hipSuccess) return fail(what, e);
+        }
+        hipError_t const e = hipHostMalloc(reinterpret_cast<void**>(&ptr), need * sizeof(T), hipHostMallocDefault);
+        if(e != hipSuccess) return fail(what, e);
+        cap = need;
+        return TA3H_OK;
+    }
+
     ~ta3h_evaluator() {
         // best effort: a dying evaluator never reports (model_eval.cu:72-83)
+        drop_graph();
+        if(hWeights) static_cast<void>(hipHostFree(hWeights));
+        if(hFitness) static_cast<void>(hipHostFree(hFitness));
+        if(hPlacements) static_cast<void>(hipHostFree(hPlacements));
         if(dWeights) static_cast<void>(hipFree(dWeights));
         if(dSeeds) static_cast<void>(hipFree(dSeeds));
         if(dFitness) static_cast<void>(hipFree(dFitness));
@@ -197,6 +234,7 @@ int ta3h_num_params(int kind) { return params_of(kind); }
 
 int ta3h_set_model(ta3h_evaluator* ev, int kind) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
     if(params_of(kind) < 0) return ev->fail_arg("set_model: unknown model kind " + std::to_string(kind));
     ev->kind = kind;
     ev->staged = false;
@@ -205,6 +243,7 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind) {
 
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
     if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4)
         return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4");
     ev->wavesPerBlock = waves_per_game;
@@ -216,6 +255,7 @@ int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint
 int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uint32_t num_params,
                const uint64_t* seeds, uint32_t games, uint32_t max_moves) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // buffers may be regrown
     ev->staged = false;
     ev->launched = false;
     uint64_t const items = static_cast<uint64_t>(num_models) * games;
@@ -254,14 +294,10 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
     return TA3H_OK;
 }
 
-int ta3h_launch(ta3h_evaluator* ev) {
-    if(!ev) return TA3H_E_ARG;
-    if(!ev->staged) {
-        ev->err = "launch: nothing staged";
-        return TA3H_E_STATE;
-    }
-    ev->launched = true;
-    if(ev->items == 0) return TA3H_OK;
+namespace {
+
+// Enqueues the search of the staged batch on the evaluator's stream (optionally bracketed by the timing events).
+int enqueue_eval(ta3h_evaluator* ev, bool events) {
     ta3h::EvalArgs a{};
     a.weights = ev->dWeights;
     a.seeds = ev->dSeeds;
@@ -293,13 +329,12 @@ int ta3h_launch(ta3h_evaluator* ev) {
     // one workgroup (of wpb cooperating waves) per game in flight
     uint32_t const needed = ev->items;
     uint32_t blocks;
-    hipError_t e0 = hipSuccess;
     if(ev->persistent) {
         blocks = ev->gridBlocks;
         if(blocks == 0) {
             // fill every CU up to the kernel's occupancy limit; workgroups beyond it would only queue
             int perCU = 0;
-            e0 = ta3h::max_active_blocks(ev->kind, wpb, mode, &perCU);
+            hipError_t const e0 = ta3h::max_active_blocks(ev->kind, wpb, mode, &perCU);
             if(e0 != hipSuccess) return ev->fail("occupancy", e0);
             blocks = ev->numCUs * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
         }
@@ -307,13 +342,25 @@ int ta3h_launch(ta3h_evaluator* ev) {
     } else
         blocks = needed;
     ev->lastGrid = blocks;
-    hipError_t e = hipEventRecord(ev->evStart, ev->stream);
-    if(e != hipSuccess) return ev->fail("event record", e);
+    hipError_t e;
+    if(events && (e = hipEventRecord(ev->evStart, ev->stream)) != hipSuccess) return ev->fail("event record", e);
     e = ta3h::launch_beam_eval(ev->kind, a, wpb, blocks, ev->stream);
     if(e != hipSuccess) return ev->fail("kernel launch", e);
-    e = hipEventRecord(ev->evStop, ev->stream);
-    if(e != hipSuccess) return ev->fail("event record", e);
+    if(events && (e = hipEventRecord(ev->evStop, ev->stream)) != hipSuccess) return ev->fail("event record", e);
     return TA3H_OK;
+}
+
+} // namespace
+
+int ta3h_launch(ta3h_evaluator* ev) {
+    if(!ev) return TA3H_E_ARG;
+    if(!ev->staged) {
+        ev->err = "launch: nothing staged";
+        return TA3H_E_STATE;
+    }
+    ev->launched = true;
+    if(ev->items == 0) return TA3H_OK;
+    return enqueue_eval(ev, true);
 }
 
 int ta3h_sync(ta3h_evaluator* ev) {
@@ -363,6 +410,7 @@ int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters) {
 
 int ta3h_set_timeline(ta3h_evaluator* ev, int enabled) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
     ev->timelineOn = enabled != 0;
     ev->staged = false;
     return TA3H_OK;
@@ -443,6 +491,93 @@ int ta3h_feature_norms(float* out, uint32_t n) {
     return r;
 }
 
+int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_params, const uint64_t* seeds,
+                       uint32_t games, uint32_t max_moves) {
+    if(!ev) return TA3H_E_ARG;
+    ev->drop_graph();
+    ev->graphModels = 0;
+    uint64_t const items = static_cast<uint64_t>(num_models) * games;
+    if(items == 0) return ev->fail_arg("graph_prepare: no (model, game) items");
+    if(items > 0xFFFFFFFFull / 2) return ev->fail_arg("graph_prepare: num_models * games too large");
+    int const P = params_of(ev->kind);
+    if(static_cast<int64_t>(num_params) != P)
+        return ev->fail_arg("graph_prepare: weights per model must equal NUM_PARAMS (" + std::to_string(P) + ")");
+    if(seeds == nullptr) return ev->fail_arg("graph_prepare: null seeds");
+    int r = ev->ensure_ready();
+    if(r) return r;
+    uint32_t const stride = stride_of(ev->kind);
+    size_t const wCount = static_cast<size_t>(num_models) * stride;
+    if((r = ev->grow(ev->dWeights, ev->weightsCap, wCount, "malloc weights"))) return r;
+    if((r = ev->grow(ev->dSeeds, ev->seedsCap, games, "malloc seeds"))) return r;
+    if((r = ev->grow(ev->dFitness, ev->fitnessCap, items, "malloc fitness"))) return r;
+    if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
+    if((r = ev->grow(ev->dCounters, ev->countersCap, 2 * items, "malloc counters"))) return r;
+    if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
+    if((r = ev->grow_pinned(ev->hWeights, ev->hWeightsCap, wCount, "pinned weights"))) return r;
+    if((r = ev->grow_pinned(ev->hFitness, ev->hFitnessCap, items, "pinned fitness"))) return r;
+    if((r = ev->grow_pinned(ev->hPlacements, ev->hPlacementsCap, items, "pinned placements"))) return r;
+    std::memset(ev->hWeights, 0, wCount * sizeof(float)); // padding floats stay zero
+    hipError_t e = hipMemcpyAsync(ev->dSeeds, seeds, games * sizeof(uint64_t), hipMemcpyHostToDevice, ev->stream);
+    if(e == hipSuccess) e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("copy seeds", e);
+    ev->items = static_cast<uint32_t>(items);
+    ev->games = games;
+    ev->maxMoves = max_moves;
+    ev->staged = true;
+    ev->launched = false;
+    // capture: weights in, search, fitness and placements out
+    e = hipStreamBeginCapture(ev->stream, hipStreamCaptureModeThreadLocal);
+    if(e != hipSuccess) return ev->fail("begin capture", e);
+    hipError_t ec = hipMemcpyAsync(ev->dWeights, ev->hWeights, wCount * sizeof(float), hipMemcpyHostToDevice, ev->stream);
+    int rc = TA3H_OK;
+    if(ec == hipSuccess) rc = enqueue_eval(ev, false);
+    if(ec == hipSuccess && rc == TA3H_OK)
+        ec = hipMemcpyAsync(ev->hFitness, ev->dFitness, items * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
+    if(ec == hipSuccess && rc == TA3H_OK)
+        ec = hipMemcpyAsync(ev->hPlacements, ev->dPlacements, items * sizeof(uint32_t), hipMemcpyDeviceToHost, ev->stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ev->stream, &g);
+    if(rc != TA3H_OK) {
+        if(g) static_cast<void>(hipGraphDestroy(g));
+        return rc;
+    }
+    if(ec != hipSuccess || e != hipSuccess) {
+        if(g) static_cast<void>(hipGraphDestroy(g));
+        return ev->fail("capture generation", ec != hipSuccess ? ec : e);
+    }
+    ev->graph = g;
+    e = hipGraphInstantiate(&ev->graphExec, g, nullptr, nullptr, 0);
+    if(e != hipSuccess) {
+        ev->drop_graph();
+        return ev->fail("instantiate generation graph", e);
+    }
+    ev->graphModels = num_models;
+    return TA3H_OK;
+}
+
+int ta3h_graph_run(ta3h_evaluator* ev, const float* weights, float* out_fitness, uint32_t* out_placements) {
+    if(!ev) return TA3H_E_ARG;
+    if(ev->graphExec == nullptr) {
+        ev->err = "graph_run: no generation graph prepared";
+        return TA3H_E_STATE;
+    }
+    if(weights == nullptr) return ev->fail_arg("graph_run: null weights");
+    int const P = params_of(ev->kind);
+    uint32_t const stride = stride_of(ev->kind);
+    for(uint32_t m = 0; m < ev->graphModels; ++m)
+        repack(ev->kind, weights + static_cast<size_t>(m) * P, ev->hWeights + static_cast<size_t>(m) * stride);
+    hipError_t e = hipEventRecord(ev->evStart, ev->stream);
+    if(e == hipSuccess) e = hipGraphLaunch(ev->graphExec, ev->stream);
+    if(e == hipSuccess) e = hipEventRecord(ev->evStop, ev->stream);
+    if(e != hipSuccess) return ev->fail("graph launch", e);
+    e = hipStreamSynchronize(ev->stream); // the one host synchronisation of the generation
+    if(e != hipSuccess) return ev->fail("graph run", e);
+    ev->launched = true;
+    if(out_fitness) std::memcpy(out_fitness, ev->hFitness, ev->items * sizeof(float));
+    if(out_placements) std::memcpy(out_placements, ev->hPlacements, ev->items * sizeof(uint32_t));
+    return TA3H_OK;
+}
+
 int ta3h_pruner_num_params(void) { return ta3h::PR_PARAMS; }
 
 int ta3h_pruner_quant_info(const float* w, float act_range, int* info) {
@@ -454,6 +589,7 @@ int ta3h_pruner_quant_info(const float* w, float act_range, int* info) {
 
 int ta3h_set_pruner(ta3h_evaluator* ev, const float* w, float act_range, uint32_t k, uint32_t ply_mask) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
     if(w == nullptr) {
         ev->prunerOn = false;
         return TA3H_OK;
@@ -484,6 +620,7 @@ int ta3h_set_pruner(ta3h_evaluator* ev, const float* w, float act_range, uint32_
 
 int ta3h_set_trace(ta3h_evaluator* ev, uint64_t cap_records, uint32_t every) {
     if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
     if(cap_records == 0 || every == 0) {
         ev->traceEvery = 0;
         return TA3H_OK;
