@@ -30,7 +30,8 @@ enum {
 /* value networks the search scores leaves with */
 enum {
     TA3H_MODEL_V11 = 0, /* reference model_t = ModelV11, 97 params (ta3_ai/incl/ta3/ai/model.hpp:44) */
-    TA3H_MODEL_F16 = 1  /* 16 features -> 32 -> 32 -> 1, 1633 params */
+    TA3H_MODEL_F16 = 1, /* 16 features -> 32 -> 32 -> 1, 1633 params (FP32 fused multiply-adds) */
+    TA3H_MODEL_F32W = 3 /* 32 features -> 128 -> 128 -> 1, 20865 params (BF16 MFMA, FP32 accumulate) */
 };
 
 /* Replaces ModelEvaluator::ModelEvaluator() (model_eval.hpp:29; model_eval.cu:92). Allocates nothing
@@ -49,6 +50,12 @@ int ta3h_num_params(int kind);
 /* Selects the value network (default TA3H_MODEL_V11; the reference binds it at compile time,
  * ta3_ai/incl/ta3/ai/model.hpp:44). */
 int ta3h_set_model(ta3h_evaluator* ev, int kind);
+
+/* Start every game of the following launches with 0-4 garbage rows at the bottom, each full but for one hole,
+ * drawn from xoshiro256** seeded with (game seed ^ 0x6A09E667F3BCC908): n = next() % 5, then for rows 22, 21,
+ * ...: hole = next() % 10. The piece sequence is the seed's as without garbage. No reference counterpart (a
+ * build-defined start-position extension). */
+int ta3h_set_garbage(ta3h_evaluator* ev, int enabled);
 
 /* Launch shape: waves cooperating on one game (1, 2 or 4; a workgroup plays one game at a time),
  * persistent work-queue mode (1) or one workgroup per item (0), and the persistent grid in workgroups
@@ -93,6 +100,12 @@ int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_
 /* Replaces ta3::gpu::device_bits_canary() (ta3_gpu/incl/ta3/gpu/device_canary.hpp:16): runs ctz / popcount /
  * bit_width on the device against known answers; *ok = 1 when all match. */
 int ta3h_device_bits_canary(int* ok);
+
+/* Leaf values of n boards (boards[10*i .. +10], clear histogram clears[i], held-is-I flag held[i]) as the
+ * search scores a candidate board, for one model's weights (host layout): the device counterpart of the
+ * reference's net_ref::evaluate (ta3_gpu/incl/ta3/gpu/detail/value_model.hpp:14-21). */
+int ta3h_model_values(int kind, const float* weights, const uint32_t* boards, const uint32_t* clears,
+                      const uint8_t* held, uint32_t n, float* out);
 
 /* Device values of the normalised features for integers 0..n-1, computed as the search computes them:
  * out[6*i + {0..5}] = {sqrt(i)/100 (i <= 4761, else 0), i/20, i/200, i/5, i/40, i/100} in float32 (no

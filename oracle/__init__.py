@@ -89,6 +89,8 @@ def lib():
             "ta3o_ordered_bits_inv": (C.c_float, [C.c_uint32]),
             "ta3o_features": (None, [C.c_int, C.c_uint32, u32p, C.c_int, f32p]),
             "ta3o_model_value": (C.c_float, [C.c_int, f32p, C.c_uint32, u32p, C.c_int]),
+            "ta3o_f32w_value_fp32": (C.c_float, [f32p, C.c_uint32, u32p, C.c_int]),
+            "ta3o_set_garbage": (None, [C.c_int]),
             "ta3o_search_beam": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, u32p, C.c_int,
                                            ip, ip, ip, f32p, C.POINTER(Counters)]),
             "ta3o_search_exhaustive": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, C.c_int, ip, ip, ip, f32p]),
@@ -262,6 +264,25 @@ def features(kind: int, clears: int, cols, held_is_i: bool) -> np.ndarray:
     c = np.array(cols, dtype=np.uint32)
     lib().ta3o_features(kind, clears, _p(c, C.c_uint32), int(held_is_i), _p(out, C.c_float))
     return out
+
+
+def f32w_value_fp32(w, clears: int, cols, held_is_i: bool) -> float:
+    """the wide network in plain FP32 (no bfloat16 rounding)"""
+    w = _f32(w)
+    c = np.array(cols, dtype=np.uint32)
+    return lib().ta3o_f32w_value_fp32(_p(w, C.c_float), clears, _p(c, C.c_uint32), int(held_is_i))
+
+
+class garbage:
+    """context manager: every game the oracle starts inside it begins with 0-4 garbage rows"""
+
+    def __enter__(self):
+        lib().ta3o_set_garbage(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().ta3o_set_garbage(0)
+        return False
 
 
 def model_value(kind: int, w, clears: int, cols, held_is_i: bool) -> float:

@@ -281,6 +281,24 @@ void engine_reset(Engine& e, uint64_t seed) { // :103-114
     update_game_over(e);
 }
 
+// Build-defined start-position extension: 0-4 garbage rows at the bottom, each full but for one hole, drawn
+// from a second xoshiro256** seeded with seed ^ 0x6A09E667F3BCC908 (so the piece sequence is unchanged):
+// n = next() % 5; then for rows y = 22, 21, ... (n of them): hole column = next() % 10.
+constexpr uint64_t GARBAGE_SALT = 0x6A09E667F3BCC908ull;
+void add_garbage_rows(Engine& e, uint64_t seed) {
+    uint64_t r[4];
+    rng_seed(seed ^ GARBAGE_SALT, r);
+    int const n = static_cast<int>(rng_next(r) % 5);
+    for(int k = 0; k < n; ++k) {
+        int const hole = static_cast<int>(rng_next(r) % WIDTH);
+        int const y = HEIGHT - 1 - k;
+        for(int x = 0; x < WIDTH; ++x)
+            if(x != hole) e.board.c[x] |= 1u << y;
+    }
+    update_game_over(e);
+}
+int g_garbage = 0; // process-wide option of the oracle: start every game with garbage rows
+
 void advance_queue(Engine& e) { // :131-135
     for(int i = 1; i < QUEUE; ++i) e.queue[i - 1] = e.queue[i];
     e.queue[QUEUE - 1] = draw_piece(e);
@@ -552,6 +570,80 @@ void features_f16(uint32_t clears, Board const& b, bool heldIsI, float* out) {
     out[15] = static_cast<float>(m.maxHeight - m.minHeight);
 }
 
+// Build-defined 32-feature set of the wide network: the 16 above, then 16..25 column heights / 20, 26 rows
+// holding a hole / 20, 27 row transitions / 100 (the side walls count as filled), 28 deepest well / 20 (depth =
+// lower neighbour height minus the column's; one neighbour at the walls), 29 filled cells / 200, 30 columns
+// holding a hole / 5, 31 minimum height / 20.
+void features_f32w(uint32_t clears, Board const& b, bool heldIsI, float* out) {
+    features_f16(clears, b, heldIsI, out);
+    Metrics const m = metrics(b);
+    uint32_t holeRows = 0;
+    int holeCols = 0, cells = 0, rowTrans = 0, deepest = 0;
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t const col = b.c[x];
+        out[16 + x] = static_cast<float>(m.heights[x]) / 20.0f;
+        int const top = HEIGHT - m.heights[x]; // first filled row (HEIGHT when empty)
+        uint32_t const below = FIELD & ~((1u << top) - 1u);
+        uint32_t const holes = below & ~col;
+        holeRows |= holes;
+        holeCols += holes != 0 ? 1 : 0;
+        cells += popc32(col);
+        if(x + 1 < WIDTH) rowTrans += popc32((col ^ b.c[x + 1]) & FIELD);
+        int const left = x > 0 ? m.heights[x - 1] : m.heights[x + 1];
+        int const right = x + 1 < WIDTH ? m.heights[x + 1] : m.heights[x - 1];
+        deepest = std::max(deepest, std::min(left, right) - m.heights[x]);
+    }
+    rowTrans += popc32(~b.c[0] & FIELD) + popc32(~b.c[WIDTH - 1] & FIELD);
+    out[26] = static_cast<float>(popc32(holeRows)) / 20.0f;
+    out[27] = static_cast<float>(rowTrans) / 100.0f;
+    out[28] = static_cast<float>(deepest) / 20.0f;
+    out[29] = static_cast<float>(cells) / 200.0f;
+    out[30] = static_cast<float>(holeCols) / 5.0f;
+    out[31] = static_cast<float>(m.minHeight) / 20.0f;
+}
+
+// bfloat16 rounding (nearest, ties to even) of a float32
+inline float bf16_rne(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+// Wide network 32 -> 128 ReLU -> 128 ReLU -> 1, layout as dense(): W1[128][32], b1, W2[128][128], b2, W3[128], b3.
+// Device arithmetic (BF16 MFMA, FP32 accumulate): features, W1 and W2 rounded to bfloat16; each hidden
+// pre-activation = bias + exact sum of the bfloat16 products, rounded once to float32 (the device's FP32
+// accumulation may differ in the last bits); layer 1 output rounded to bfloat16 after ReLU; the output layer in
+// FP32: p_q = FMA chain over o = 16 t + 4 q + j (t = 0..7, then j = 0..3) of h2[o] W3[o], value =
+// ((p0 + p1) + (p2 + p3)) + b3.
+float value_f32w(float const* w, float const* in) {
+    float x[32], h1[128], h2[128];
+    for(int i = 0; i < 32; ++i) x[i] = bf16_rne(in[i]);
+    for(int u = 0; u < 128; ++u) {
+        double v = w[4096 + u];
+        for(int i = 0; i < 32; ++i) v += static_cast<double>(x[i]) * bf16_rne(w[u * 32 + i]);
+        float const f = static_cast<float>(v);
+        h1[u] = bf16_rne(f > 0.0f ? f : 0.0f);
+    }
+    for(int o = 0; o < 128; ++o) {
+        double v = w[4224 + 16384 + o];
+        for(int u = 0; u < 128; ++u) v += static_cast<double>(h1[u]) * bf16_rne(w[4224 + o * 128 + u]);
+        float const f = static_cast<float>(v);
+        h2[o] = f > 0.0f ? f : 0.0f;
+    }
+    float const* w3 = w + 4224 + 16512;
+    float p[4];
+    for(int q = 0; q < 4; ++q) {
+        p[q] = 0.0f;
+        for(int t = 0; t < 8; ++t)
+            for(int j = 0; j < 4; ++j) p[q] = std::fmaf(h2[16 * t + 4 * q + j], w3[16 * t + 4 * q + j], p[q]);
+    }
+    return ((p[0] + p[1]) + (p[2] + p[3])) + w3[128];
+}
+
 // dense layer (ta3_ai/incl/ta3/ai/models/models/utility.hpp:330-347): W[o*In + i], biases after the
 // matrix, v = bias then += in[i] * w in ascending i, ReLU = std::max<float>(0, v)
 void dense(float const* in, int nIn, int nOut, bool relu, float const* w, float* out) {
@@ -595,6 +687,11 @@ struct ModelRef {
                 dense_fma(h1, 32, 32, true, w + 544, h2);
                 dense_fma(h2, 32, 1, false, w + 1600, out);
                 return out[0];
+            }
+            case TA3O_MODEL_F32W: { // 32 -> 128 ReLU -> 128 ReLU -> 1, device (BF16 MFMA) arithmetic
+                float in[32];
+                features_f32w(clears, b, heldIsI, in);
+                return value_f32w(w, in);
             }
             case TA3O_MODEL_HEIGHT: { // beam_test.cpp:20-35
                 uint32_t agg = 0;
@@ -1233,6 +1330,7 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
                 PrunerRef const& pr = PrunerRef{}, std::vector<uint32_t>* traceOut = nullptr, uint32_t traceEvery = 0) {
     Engine g;
     engine_reset(g, seed);
+    if(g_garbage) add_garbage_rows(g, seed);
     Stats stats;
     int rec = 0;
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
@@ -1306,6 +1404,7 @@ int ta3o_num_params(int kind) {
         case TA3O_MODEL_V11: return 97;
         case TA3O_MODEL_F16: return 1633;
         case TA3O_MODEL_HEIGHT: return 0;
+        case TA3O_MODEL_F32W: return 20865;
         default: return -1;
     }
 }
@@ -1314,6 +1413,7 @@ int ta3o_num_features(int kind) {
     switch(kind) {
         case TA3O_MODEL_V11: return 10;
         case TA3O_MODEL_F16: return 16;
+        case TA3O_MODEL_F32W: return 32;
         default: return 0;
     }
 }
@@ -1365,8 +1465,11 @@ void ta3o_piece_shape(int type, int o, uint32_t* cols_nibbles, int* left) {
 void ta3o_engine_reset(ta3o_engine* e, uint64_t seed) {
     Engine g;
     engine_reset(g, seed);
+    if(g_garbage) add_garbage_rows(g, seed);
     to_c(g, e);
 }
+
+void ta3o_set_garbage(int enabled) { g_garbage = enabled != 0; }
 
 int ta3o_engine_place(ta3o_engine* e, int o, int x) {
     Engine g = from_c(e);
@@ -1427,11 +1530,21 @@ void ta3o_features(int kind, uint32_t clears, const uint32_t cols[10], int held_
     Board const b = board_from(cols);
     if(kind == TA3O_MODEL_V11) features_v11(clears, b, held_is_i != 0, out);
     else if(kind == TA3O_MODEL_F16) features_f16(clears, b, held_is_i != 0, out);
+    else if(kind == TA3O_MODEL_F32W) features_f32w(clears, b, held_is_i != 0, out);
 }
 
 float ta3o_model_value(int kind, const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i) {
     ModelRef const m{kind, w};
     return m.value(clears, board_from(cols), held_is_i != 0);
+}
+
+float ta3o_f32w_value_fp32(const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i) {
+    float in[32], h1[128], h2[128], out[1];
+    features_f32w(clears, board_from(cols), held_is_i != 0, in);
+    dense_fma(in, 32, 128, true, w, h1);
+    dense_fma(h1, 128, 128, true, w + 4224, h2);
+    dense_fma(h2, 128, 1, false, w + 4224 + 16512, out);
+    return out[0];
 }
 
 int ta3o_search_beam(const ta3o_engine* e, int kind, const float* w, const uint32_t* widths, int nwidths,

@@ -23,7 +23,7 @@ enum {
     TA3O_MODEL_V11 = 0,    /* reference default model_t: 10 features -> 8 ReLU -> 1, 97 params */
     TA3O_MODEL_F16 = 1,    /* 16 features -> 32 -> 32 -> 1 (ReLU hidden), 1633 params */
     TA3O_MODEL_HEIGHT = 2, /* test double of the reference search tests, 0 params */
-    TA3O_MODEL_F32W = 3    /* 32 features -> 128 -> 128 -> 1 (ReLU hidden), 20865 params */
+    TA3O_MODEL_F32W = 3    /* 32 features -> 128 -> 128 -> 1 (ReLU hidden), 20865 params, BF16 MFMA arithmetic */
 };
 
 /* plain-old-data engine state, field-for-field the reference TetrisEngine (tetris_engine.hpp:81-94) */
@@ -66,6 +66,11 @@ int ta3o_board_fit(const uint32_t cols[10], int type, int o, int x, int* drop);
 /* piece shape table entry (piece_offsets.hpp:186-209) */
 void ta3o_piece_shape(int type, int o, uint32_t* cols_nibbles, int* left);
 
+/* process-wide option: start every game (engine_reset, eval_host, traces) with 0-4 garbage rows, each full
+ * but for one hole, drawn from xoshiro256** seeded with seed ^ 0x6A09E667F3BCC908: n = next() % 5, then per
+ * row y = 22, 21, ...: hole = next() % 10 */
+void ta3o_set_garbage(int enabled);
+
 /* TetrisEngine (tetris_engine.hpp:103-161); place returns cleared rows or -1 for DIED */
 void ta3o_engine_reset(ta3o_engine* e, uint64_t seed);
 int ta3o_engine_place(ta3o_engine* e, int o, int x);
@@ -90,6 +95,8 @@ float ta3o_ordered_bits_inv(uint32_t s);
 int ta3o_num_features(int kind);
 void ta3o_features(int kind, uint32_t clears, const uint32_t cols[10], int held_is_i, float* out);
 float ta3o_model_value(int kind, const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i);
+/* the wide network in plain FP32 (fused multiply-adds in dense() order, no bfloat16 rounding) */
+float ta3o_f32w_value_fp32(const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i);
 
 /* one beamed search move (beam.hpp:483-570) at runtime depth = nwidths + 1.
  * returns 1 and writes the move, or 0 when none. */

@@ -1,0 +1,105 @@
+"""GPU: the wide value network (32 -> 128 -> 128 -> 1 on BF16 MFMA) and garbage-row start positions.
+
+The wide network's MFMA accumulation order is the hardware's, so its values are checked against the oracle's
+device-arithmetic restatement within a stated tolerance and against the plain FP32 network within the bfloat16
+error; games report their agreement rate with the oracle. Everything else (V11 / 16-32-32-1 leaf values on
+arbitrary boards, garbage-row games) stays bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tetris_ai3_amd as T
+from test_oracle_model import random_boards
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def board_set(seed, n):
+    rng = np.random.default_rng(seed)
+    boards = np.array(random_boards(n, seed), dtype=np.uint32)
+    clears = (rng.integers(0, 5, len(boards)) | rng.integers(0, 3, len(boards)) << 8
+              | rng.integers(0, 4, len(boards)) << 24).astype(np.uint32)
+    held = rng.integers(0, 2, len(boards)).astype(np.uint8)
+    return boards, clears, held
+
+
+@pytest.mark.parametrize("kind", [O.MODEL_V11, O.MODEL_F16])
+def test_leaf_values_bit_exact(kind):
+    say(f"case: leaf values kind={kind}")
+    boards, clears, held = board_set(3, 200)
+    w = np.random.default_rng(4).normal(0, 0.5, O.num_params(kind)).astype(np.float32)
+    got = T.model_values(kind, w, boards, clears, held)
+    want = np.array([O.model_value(kind, w, int(c), b, bool(h)) for b, c, h in zip(boards, clears, held)],
+                    dtype=np.float32)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_wide_leaf_values_within_tolerance():
+    """device vs the oracle's device-arithmetic restatement (exact sums): within 1e-4 of the value spread;
+    vs the plain FP32 network: within 5% of the spread (bfloat16 operands)"""
+    say("case: wide leaf values")
+    boards, clears, held = board_set(5, 300)
+    w = np.random.default_rng(6).normal(0, 0.2, 20865).astype(np.float32)
+    got = T.model_values(T.MODEL_F32W, w, boards, clears, held).astype(np.float64)
+    emu = np.array([O.model_value(O.MODEL_F32W, w, int(c), b, bool(h)) for b, c, h in zip(boards, clears, held)])
+    fp32 = np.array([O.f32w_value_fp32(w, int(c), b, bool(h)) for b, c, h in zip(boards, clears, held)])
+    spread = fp32.std()
+    say(f"max |device - emulation| = {np.abs(got - emu).max():.3g}, max |device - fp32| = "
+        f"{np.abs(got - fp32).max():.3g}, spread {spread:.3g}")
+    assert np.abs(got - emu).max() <= 1e-4 * spread
+    assert np.abs(got - fp32).max() <= 0.05 * spread
+
+
+def test_wide_games_agree_with_oracle():
+    """whole games: fitness agreement rate with the oracle's restatement (bit-identical games)"""
+    say("case: wide games 4 models x 6 seeds x 60")
+    nm = 4
+    w = np.random.default_rng(7).normal(0, 0.3, nm * 20865).astype(np.float32)
+    seeds = O.expand_seeds(8, 6)
+    ev = T.ModelEvaluator(model=T.MODEL_F32W)
+    got = ev.eval(w, nm, seeds, 60)
+    ev.close()
+    want, _ = O.eval_host_mt(O.MODEL_F32W, w, nm, seeds, 60)
+    rate = float((got == want).mean())
+    say(f"identical games: {rate:.3f}")
+    assert rate >= 0.75
+
+
+def test_wide_pruned_k34_is_the_wide_full_beam():
+    say("case: wide pruned k=34")
+    nm = 2
+    w = np.random.default_rng(9).normal(0, 0.3, nm * 20865).astype(np.float32)
+    seeds = O.expand_seeds(10, 4)
+    ev = T.ModelEvaluator(model=T.MODEL_F32W)
+    full = ev.eval(w, nm, seeds, 50)
+    ev.set_pruner(T.pruner_init(11), 4.0, k=34, ply_mask=0b1110)
+    pruned = ev.eval(w, nm, seeds, 50)
+    ev.close()
+    assert pruned.tobytes() == full.tobytes()
+
+
+@pytest.mark.parametrize("kind", [O.MODEL_V11, O.MODEL_F16])
+def test_garbage_games_bit_exact(kind):
+    say(f"case: garbage games kind={kind}")
+    nm = 2
+    w = np.random.default_rng(12).normal(0, 0.5, nm * O.num_params(kind)).astype(np.float32)
+    seeds = O.expand_seeds(13, 8)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_garbage(True)
+    ev.stage(w, nm, seeds, 80)
+    ev.launch()
+    fit, pl = ev.fetch()
+    ev.close()
+    with O.garbage():
+        want, want_pl = O.eval_host_mt(kind, w, nm, seeds, 80)
+    assert fit.tobytes() == want.tobytes() and (pl == want_pl).all()
+    plain, _ = O.eval_host_mt(kind, w, nm, seeds, 80)
+    assert fit.tobytes() != plain.tobytes()

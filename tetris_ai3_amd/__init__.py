@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libta3h.so")
 
 MODEL_V11 = 0
 MODEL_F16 = 1
+MODEL_F32W = 3
 
 _lib = None
 
@@ -56,6 +57,8 @@ def lib():
             "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
             "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
+            "ta3h_set_garbage": (C.c_int, [vp, C.c_int]),
+            "ta3h_model_values": (C.c_int, [C.c_int, f32p, u32p, u32p, C.POINTER(C.c_uint8), C.c_uint32, f32p]),
             "ta3h_graph_prepare": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
             "ta3h_graph_run": (C.c_int, [vp, f32p, f32p, u32p]),
             "ta3h_pruner_num_params": (C.c_int, []),
@@ -117,6 +120,10 @@ class ModelEvaluator:
     @property
     def num_params(self) -> int:
         return num_params(self.model)
+
+    def set_garbage(self, enabled: bool = True):
+        """start every game of the following launches with 0-4 seeded garbage rows"""
+        self._check(lib().ta3h_set_garbage(self._h, int(enabled)))
 
     def set_launch(self, waves_per_game: int = 2, persistent: bool = True, grid_blocks: int = 0):
         """waves cooperating on one game (1, 2 or 4), persistent work queue or one workgroup per item,
@@ -323,7 +330,7 @@ def pruner_init(seed: int = 0) -> np.ndarray:
 
 
 # launch shapes measured fastest on MI355X per network (full CMA-ES generation, see autotune())
-DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4}  # V11 at 5 waves/SIMD, F16 at 4
+DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4, MODEL_F32W: 4}  # V11 at 5 waves/SIMD, F16 at 4
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
@@ -372,6 +379,21 @@ def expand_seeds(seed: int, games: int) -> np.ndarray:
 def simulate(evaluator: ModelEvaluator, model_weights, num_models: int, games: int, seed: int, max_moves: int):
     """Batched simulate (model_eval.hpp:84-103): num_models models over `games` shared games seeded from `seed`."""
     return evaluator.eval(model_weights, num_models, expand_seeds(seed, games), max_moves)
+
+
+def model_values(kind: int, weights, boards, clears, held) -> np.ndarray:
+    """leaf values of boards [n, 10] as the search scores candidate boards (one model, host layout)"""
+    w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+    if w.size != num_params(kind):
+        raise HipError(f"model_values: {num_params(kind)} weights expected")
+    b = np.ascontiguousarray(boards, dtype=np.uint32).reshape(-1, 10)
+    c = np.ascontiguousarray(clears, dtype=np.uint32).reshape(-1)
+    h = np.ascontiguousarray(held, dtype=np.uint8).reshape(-1)
+    out = np.zeros(b.shape[0], dtype=np.float32)
+    if lib().ta3h_model_values(kind, _ptr(w, C.c_float), _ptr(b, C.c_uint32), _ptr(c, C.c_uint32), _ptr(h, C.c_uint8),
+                               b.shape[0], _ptr(out, C.c_float)) != 0:
+        raise HipError("model_values failed")
+    return out
 
 
 def device_bits_canary() -> bool:

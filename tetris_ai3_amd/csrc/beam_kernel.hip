@@ -260,6 +260,29 @@ __device__ void engine_reset(EngineState& e, uint64_t seed) { // tetris_engine.h
     e.placements = 0;
 }
 
+// Start-position extension (build-defined): 0-4 garbage rows at the bottom, each full but for one hole, from a
+// second xoshiro256** seeded with seed ^ 0x6A09E667F3BCC908 (the piece sequence is unchanged):
+// n = next() % 5, then for rows y = 22, 21, ... (n of them): hole column = next() % 10.
+__device__ void add_garbage_rows(EngineState& e, uint64_t seed) {
+    uint64_t r[4];
+    seed ^= 0x6A09E667F3BCC908ULL;
+    for(int i = 0; i < 4; ++i) {
+        seed += 0x9e3779b97f4a7c15ULL;
+        uint64_t z = seed;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        r[i] = z ^ (z >> 31);
+    }
+    uint32_t const n = static_cast<uint32_t>(rng_next(r) % 5u);
+    for(uint32_t k = 0; k < n; ++k) {
+        uint32_t const hole = static_cast<uint32_t>(rng_next(r) % WIDTH);
+        uint32_t const row = 1u << (HEIGHT - 1 - k);
+        for(uint32_t x = 0; x < WIDTH; ++x)
+            if(x != hole) e.cols[x] |= row;
+    }
+    update_game_over(e);
+}
+
 __device__ inline int clear_full_rows(uint32_t* cols) { // board2.hpp:333-342 with :218-234
     uint32_t full = FIELD;
     for(int x = 0; x < WIDTH; ++x) full &= cols[x];
@@ -348,6 +371,10 @@ __device__ float final_score(EngineState const& e) { // score_v1.hpp:20-34; mode
 // ---------------------------------------------------------------------------------------------------
 // candidate evaluation: apply (search.hpp:294-302) + features + value network
 // ---------------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Weights are read through the constant address space: the per-model base address is wave-uniform, so
 // every weight becomes a scalar load into an SGPR that VALU instructions take as an operand for free.
 using wptr = float const __attribute__((address_space(4)))*;
@@ -430,16 +457,30 @@ constexpr uint32_t PLANE0 = 0x2AAAAAu, PLANE1 = 0x4CCCCCu, PLANE2 = 0x70F0F0u, P
 // which keeps the register footprint small.
 constexpr int SCR_STRIDE = 64;
 
+// Features of the 64 candidates of a chunk for the wide network, bfloat16, one 64-byte row per lane; the 16-byte
+// chunk q of row r sits at position q ^ (r & 3), which spreads the 16 rows a fragment read touches over the banks.
+__device__ inline uint32_t feat_index(uint32_t row, uint32_t q) { return row * 32 + 8 * (q ^ (row & 3u)); }
+
+__device__ inline uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
+
+// Feature pass over the candidate board (one streaming pass over the columns, v_board_metrics.hpp:130-225; a
+// column's well flag is settled once its right neighbour is known). Fills f[0..9] (V11) or f[0..15] (16-feature
+// set); the 32-feature set also writes its features 16..31 (bfloat16) straight into the lane's row of featTile.
 template<int KIND>
-__device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool heldIsI, wptr w) {
-    // one streaming pass over the columns (v_board_metrics.hpp:130-225); a column's well flag is settled
-    // once its right neighbour is known
+__device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool heldIsI, float* f,
+                                     uint16_t* featTile) {
+    constexpr bool WIDE = KIND != MODEL_V11;
+    constexpr bool F32 = KIND == MODEL_F32W;
     int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
+    int holeCols = 0, cells = 0, rowTr = 0, deepest = 0;
+    uint32_t holeRows = 0, cPrev = 0;
     int hPrev2 = 0, hPrev = 0;
+    uint32_t const lane = lane_id();
 #pragma unroll 2
     for(int x = 0; x < WIDTH; ++x) {
         uint32_t const c = col[x * SCR_STRIDE];
-        int const hx = HEIGHT - ctz_nz(c | FLOOR);
+        int const top = ctz_nz(c | FLOOR);
+        int const hx = HEIGHT - top;
         int const n = popc(c);
         holes += hx - n;
         maxH = hx > maxH ? hx : maxH;
@@ -449,22 +490,36 @@ __device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool he
         if(x > 0) {
             int const d = hx - hPrev;
             sv += d * d;
-            if(KIND == MODEL_F16) bump += d < 0 ? -d : d;
+            if(WIDE) bump += d < 0 ? -d : d;
             int const lim = hPrev + 3; // wells_t (v_board_metrics.hpp:204-225) for column x-1
             bool const leftOk = x == 1 ? true : hPrev2 >= lim;
             wells += (leftOk && hx >= lim) ? 1 : 0;
         }
-        if(KIND == MODEL_F16) {
+        if(WIDE) {
             minH = hx < minH ? hx : minH;
             agg += hx;
             uint32_t const yc = c | FLOOR;
             ytr += popc((yc ^ (yc >> 1)) & FIELD);
         }
+        if(F32) {
+            uint32_t const k = 16 + static_cast<uint32_t>(x);
+            featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(div20(hx));
+            uint32_t const hole = FIELD & ~((1u << top) - 1u) & ~c; // empty cells under the column's top
+            holeRows |= hole;
+            holeCols += hole != 0 ? 1 : 0;
+            cells += n;
+            rowTr += popc((x > 0 ? (cPrev ^ c) : ~c) & FIELD); // the left wall counts as filled
+            if(x > 0) { // well depth of column x-1: its lower neighbour (only the right one at the wall)
+                int const lo = x == 1 ? hx : (hPrev2 < hx ? hPrev2 : hx);
+                int const dp = lo - hPrev;
+                deepest = dp > deepest ? dp : deepest;
+            }
+            cPrev = c;
+        }
         hPrev2 = hPrev;
         hPrev = hx;
     }
     wells += hPrev2 >= hPrev + 3 ? 1 : 0; // column 9 against its single neighbour
-    float f[16];
 #pragma unroll
     for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
     f[4] = c_svNorm[sv];
@@ -473,7 +528,7 @@ __device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool he
     f[7] = static_cast<float>(maxH);
     f[8] = heldIsI ? 1.0f : 0.0f;
     f[9] = div5(wells);
-    if constexpr(KIND == MODEL_F16) {
+    if constexpr(WIDE) {
         uint32_t const c0 = clears & 0xFFu, c1 = (clears >> 8) & 0xFFu, c2 = (clears >> 16) & 0xFFu,
                        c3 = clears >> 24;
         int const lines = static_cast<int>(c0 + 2 * c1 + 3 * c2 + 4 * c3);
@@ -485,10 +540,88 @@ __device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool he
         f[13] = __fmul_rn(static_cast<float>(lines), 0.1f);
         f[14] = __fmul_rn(static_cast<float>(cscore), 0.1f);
         f[15] = static_cast<float>(maxH - minH);
-        return value_f16(w, f);
-    } else {
-        return value_v11(w, f);
     }
+    if constexpr(F32) {
+        int const d9 = hPrev2 - hPrev; // column 9 against column 8
+        deepest = d9 > deepest ? d9 : deepest;
+        rowTr += popc(~cPrev & FIELD); // the right wall
+        uint16_t* const c3 = &featTile[feat_index(lane, 3)]; // features 24..31
+        c3[2] = bf16_bits(div20(popc(holeRows)));
+        c3[3] = bf16_bits(div100(rowTr));
+        c3[4] = bf16_bits(div20(deepest));
+        c3[5] = bf16_bits(div200(cells));
+        c3[6] = bf16_bits(div5(holeCols));
+        c3[7] = bf16_bits(div20(minH));
+    }
+}
+
+template<int KIND>
+__device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool heldIsI, wptr w) {
+    float f[16];
+    leaf_features<KIND>(col, clears, heldIsI, f, nullptr);
+    if constexpr(KIND == MODEL_F16) return value_f16(w, f);
+    else return value_v11(w, f);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// wide network 32 -> 128 -> 128 -> 1 on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate)
+// ---------------------------------------------------------------------------------------------------
+// (weight layout: f32w in kernels.hpp)
+
+// features 0..15 of the lane's candidate into chunks 0 and 1 of its row
+__device__ inline void store_features16(uint16_t* featTile, float const (&f)[16]) {
+    uint32_t const lane = lane_id();
+#pragma unroll
+    for(uint32_t q = 0; q < 2; ++q) {
+        bf16x8 v;
+#pragma unroll
+        for(int j = 0; j < 8; ++j) v[j] = static_cast<__bf16>(f[8 * q + j]);
+        *reinterpret_cast<bf16x8*>(&featTile[feat_index(lane, q)]) = v;
+    }
+}
+
+// Value of every lane's candidate (the 64 rows of featWave; rows of illegal lanes hold stale values whose
+// results are discarded: MFMA columns are independent). Per 16-candidate column tile: layer 1 = 8 MFMAs,
+// ReLU, bfloat16; layers 2 and 3 = 32 MFMAs, ReLU, then the FP32 output FMA chain of lane quarter q over
+// o = 16 t + 4 q + j; value = ((p0 + p1) + (p2 + p3)) + b3.
+__device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave) {
+    uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
+    bf16x8 const* w1f = reinterpret_cast<bf16x8 const*>(wb + f32w::W1F);
+    bf16x8 const* w2f = reinterpret_cast<bf16x8 const*>(wb + f32w::W2F);
+    f32x4 const* b1 = reinterpret_cast<f32x4 const*>(wb + f32w::B1);
+    f32x4 const* b2 = reinterpret_cast<f32x4 const*>(wb + f32w::B2);
+    f32x4 const* w3 = reinterpret_cast<f32x4 const*>(wb + f32w::W3);
+    float const b3 = *reinterpret_cast<float const*>(wb + f32w::B3);
+    float mine = 0.0f;
+#pragma unroll 1
+    for(uint32_t ct = 0; ct < 4; ++ct) {
+        bf16x8 const x = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * ct + r16, q)]);
+        bf16x8 h[4]; // layer-2 B operands: K block kb = layer-1 tiles 2 kb, 2 kb + 1
+#pragma unroll
+        for(uint32_t t = 0; t < 8; ++t) {
+            bf16x8 const* const w1l = opaque_ptr(w1f); // one tile's fragment in flight at a time
+            f32x4 acc = b1[4 * t + q]; // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1l[(t * 16 + r16) * 4 + q], x, acc, 0, 0, 0);
+#pragma unroll
+            for(int j = 0; j < 4; ++j) h[t >> 1][4 * (t & 1u) + j] = static_cast<__bf16>(__builtin_fmaxf(acc[j], 0.0f));
+        }
+        float part = 0.0f;
+#pragma unroll 1
+        for(uint32_t t = 0; t < 8; ++t) {
+            f32x4 acc = b2[4 * t + q];
+#pragma unroll
+            for(uint32_t kb = 0; kb < 4; ++kb)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2f[((t * 4 + kb) * 16 + r16) * 4 + q], h[kb], acc, 0, 0, 0);
+            f32x4 const w = w3[4 * t + q];
+#pragma unroll
+            for(int j = 0; j < 4; ++j) part = __fmaf_rn(__builtin_fmaxf(acc[j], 0.0f), w[j], part);
+        }
+        part += __shfl_xor(part, 16); // (p0 + p1), (p2 + p3)
+        part += __shfl_xor(part, 32); // (p0 + p1) + (p2 + p3)
+        float const v = part + b3;
+        mine = q == ct ? v : mine; // lane 16 ct + r16 owns candidate 16 ct + r16
+    }
+    return mine;
 }
 
 // fit (board2.hpp:306-320) over the piece's occupied columns only: a column without a piece nibble can
@@ -599,9 +732,15 @@ __device__ inline void group_sync() {
 //    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
 //  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
 //    least one legal placement (expand_leaves, beam.hpp:407-443).
+// per-game data of the wide network: its weights and the wave's feature tile
+struct WideArgs {
+    uint8_t const* wb;
+    uint16_t* feat;
+};
+
 template<int KIND, int STAGE, int K, int G, bool TRACE = false>
 __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
-                                     uint64_t* mask, uint32_t& nLegal) {
+                                     uint64_t* mask, uint32_t& nLegal, WideArgs const& wide) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
     uint64_t ms = 0;
@@ -623,8 +762,20 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             if(legal) {
                 int const cleared = place_into(parent, e.cols, e.col0, drop, col);
                 uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
-                score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl));
+                if constexpr(KIND == MODEL_F32W) {
+                    float ft[16];
+                    leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
+                    store_features16(wide.feat, ft);
+                } else {
+                    score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl));
+                }
             }
+        }
+        if constexpr(KIND == MODEL_F32W) { // the whole wave runs the network over the chunk's feature tile
+            wave_sync();
+            float const v = value_f32w(wide.wb, wide.feat);
+            if(legal) score = ordered_bits(v);
+            wave_sync(); // the tile is rewritten by the next chunk
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
         uint32_t const slot = s * 34 + i;
@@ -701,9 +852,6 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
 // ---------------------------------------------------------------------------------------------------
 // pruned stages: placement-proposal network on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate)
 // ---------------------------------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // 8 board bits -> 8 bfloat16 values 0.0 / 1.0
 __device__ inline bf16x8 bits_bf16(uint32_t byte) {
@@ -1039,6 +1187,12 @@ __device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, ui
     return uniform(L.stageTotal);
 }
 
+// feature tiles of the wide network, one per wave
+template<int G>
+struct alignas(16) FeatLds {
+    uint16_t v[G][64 * 32];
+};
+
 // Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
 // logits share the lanes' candidate-board scratch, which is idle until the value network runs).
 struct alignas(16) PruneLds {
@@ -1053,8 +1207,8 @@ enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
 // ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
 // pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
 template<int KIND, int G, int MODE>
-__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, uint32_t wv, uint32_t* col,
-                          bool traceMove, uint64_t& nApply, uint32_t& nLegal) {
+__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, WideArgs const& wide, uint32_t wv,
+                          uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
@@ -1075,7 +1229,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
     nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, L.candSeg, total, w, wv, col, nullptr, nLegal);
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
@@ -1106,8 +1260,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
             }
         }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal)
-                         : run_stage<KIND, STAGE_MID, K2, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
+                         : run_stage<KIND, STAGE_MID, K2, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
@@ -1160,12 +1314,12 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
         if(traceMove) {
             if(wv == 0) L.segBest[lane] = 0;
             group_sync<G>();
-            leaf = run_stage<KIND, STAGE_LEAF, 1, G, true>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G, true>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         } else {
-            leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         }
     } else {
-        leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal);
+        leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
     }
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(MODE == MODE_TRACE) {
@@ -1213,7 +1367,11 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, ui
 // caps it at 4 workgroups of 4 waves per CU, so it gets the 128 registers of 4 waves per SIMD; one wave per
 // game needs more registers (3 waves per SIMD).
 template<int KIND, int MODE, int G>
-constexpr int WAVES_PER_SIMD = G == 1 ? 3 : (KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4);
+constexpr int WAVES_PER_SIMD =
+    G == 1 ? 3
+           : (KIND == MODEL_F32W ? (G == 4 && MODE != MODE_PRUNED ? 4 : 3)
+                                 : (KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4));
+// (the wide network's feature tiles cap its LDS occupancy at 4 workgroups of 4 waves per CU, 3 when pruned)
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
@@ -1225,6 +1383,7 @@ void beam_eval_kernel(EvalArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
+    __shared__ std::conditional_t<KIND == MODEL_F32W, FeatLds<G>, uint32_t> featStore; // wide network only
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
@@ -1239,9 +1398,12 @@ void beam_eval_kernel(EvalArgs a) {
         // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
+        WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr};
+        if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
         if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
             engine_reset(L.eng, a.seeds[g]);
+            if(a.garbage) add_garbage_rows(L.eng, a.seeds[g]);
         }
         group_sync<G>();
         uint64_t nApply = 0;
@@ -1252,7 +1414,7 @@ void beam_eval_kernel(EvalArgs a) {
         for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
             if(uniform(L.eng.gameOver) != 0) break;
             bool const traceMove = MODE == MODE_TRACE && a.traceEvery != 0 && mv % a.traceEvery == tracePhase;
-            if(!play_move<KIND, G, MODE>(a, L, pruneLds, w, wv, col, traceMove, nApply, nLegal)) break;
+            if(!play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col, traceMove, nApply, nLegal)) break;
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
         group_sync<G>();
@@ -1270,6 +1432,36 @@ void beam_eval_kernel(EvalArgs a) {
         group_sync<G>();
         if(!a.persistent) break;
     }
+}
+
+// Leaf values of given boards, one board per lane, exactly as the search scores a candidate board (features of
+// the board, then the network). Weights in device layout.
+template<int KIND>
+__global__ __launch_bounds__(64) void model_value_kernel(float const* __restrict__ weights,
+                                                         uint32_t const* __restrict__ boards,
+                                                         uint32_t const* __restrict__ clears,
+                                                         uint8_t const* __restrict__ held, uint32_t n,
+                                                         float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint32_t scratch[WIDTH * SCR_STRIDE];
+    __shared__ std::conditional_t<KIND == MODEL_F32W, FeatLds<1>, uint32_t> featStore;
+    uint32_t const lane = lane_id();
+    uint32_t const b = blockIdx.x * 64 + lane;
+    uint32_t const src = b < n ? b : 0u; // the last block's spare lanes recompute board 0
+    uint32_t* const col = scratch + lane;
+    for(int x = 0; x < WIDTH; ++x) col[x * SCR_STRIDE] = boards[10ull * src + x];
+    bool const heldI = held[src] != 0;
+    float v;
+    if constexpr(KIND == MODEL_F32W) {
+        uint16_t* const feat = reinterpret_cast<FeatLds<1>*>(&featStore)->v[0];
+        float f[16];
+        leaf_features<KIND>(col, clears[src], heldI, f, feat);
+        store_features16(feat, f);
+        wave_sync();
+        v = value_f32w(reinterpret_cast<uint8_t const*>(weights), feat);
+    } else {
+        v = leaf_value<KIND>(col, clears[src], heldI, (wptr)weights);
+    }
+    if(b < n) out[b] = v;
 }
 
 __global__ void reset_counter_kernel(uint32_t* q) {
@@ -1381,6 +1573,7 @@ hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t block
     switch(kind) {
         case MODEL_V11: return launch_kind<MODEL_V11>(a, wpb, blocks, stream);
         case MODEL_F16: return launch_kind<MODEL_F16>(a, wpb, blocks, stream);
+        case MODEL_F32W: return launch_kind<MODEL_F32W>(a, wpb, blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1399,6 +1592,19 @@ hipError_t occupancy_kind(int wpb, int* n, int mode) {
 }
 } // namespace
 
+hipError_t launch_model_values(int kind, float const* weights, uint32_t const* boards, uint32_t const* clears,
+                               uint8_t const* held, uint32_t n, float* out, hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    dim3 const grid((n + 63) / 64);
+    switch(kind) {
+        case MODEL_V11: hipLaunchKernelGGL(model_value_kernel<MODEL_V11>, grid, dim3(64), 0, s, weights, boards, clears, held, n, out); break;
+        case MODEL_F16: hipLaunchKernelGGL(model_value_kernel<MODEL_F16>, grid, dim3(64), 0, s, weights, boards, clears, held, n, out); break;
+        case MODEL_F32W: hipLaunchKernelGGL(model_value_kernel<MODEL_F32W>, grid, dim3(64), 0, s, weights, boards, clears, held, n, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t pruner_infer_samples(PrunerDev const* pr, uint32_t const* samples, uint32_t n, float* out, hipStream_t s) {
     if(n == 0) return hipSuccess;
     hipLaunchKernelGGL(pruner_infer_kernel, dim3((n + 15) / 16), dim3(64), 0, s, pr, samples, n, out);
@@ -1409,6 +1615,7 @@ hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksP
     switch(kind) {
         case MODEL_V11: return occupancy_kind<MODEL_V11>(wavesPerBlock, blocksPerCU, mode);
         case MODEL_F16: return occupancy_kind<MODEL_F16>(wavesPerBlock, blocksPerCU, mode);
+        case MODEL_F32W: return occupancy_kind<MODEL_F32W>(wavesPerBlock, blocksPerCU, mode);
         default: return hipErrorInvalidValue;
     }
 }

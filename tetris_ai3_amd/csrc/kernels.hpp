@@ -11,7 +11,17 @@ namespace ta3h {
 enum ModelKind : int {
     MODEL_V11 = 0, // reference default value network: 10 features -> 8 ReLU -> 1 (97 params)
     MODEL_F16 = 1, // 16 features -> 32 ReLU -> 32 ReLU -> 1 (1633 params)
+    MODEL_F32W = 3, // 32 features -> 128 ReLU -> 128 ReLU -> 1 on BF16 MFMA (20865 params)
 };
+
+// Device weight layout of the wide network, per model (bytes): w1f [8 t][16 r][4 q] x 8 bf16 = MFMA A fragment
+// (rows 16 t + r, features 8 q .. + 7) of layer 1; w2f [8 t][4 kb][16 r][4 q] x 8 bf16 = A fragment of layer 2
+// for output rows 16 t + r, K block kb, whose element e is hidden unit 32 kb + 4 q + e (e < 4) or
+// 32 kb + 16 + 4 q + (e - 4) -- the units lane quarter q holds in layer 1's output tiles 2 kb and 2 kb + 1, so
+// layer 1's outputs are layer 2's B operands in place; then b1[128], b2[128], W3[128], b3 as float.
+namespace f32w {
+inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 41984, B3 = 42496, BYTES = 42512;
+}
 
 struct EvalArgs {
     float const* weights;     // device layout per model, weightStride floats apart
@@ -26,6 +36,7 @@ struct EvalArgs {
     uint32_t maxMoves;
     uint32_t weightStride;
     uint32_t persistent;
+    uint32_t garbage; // start every game with 0-4 garbage rows (seeded from the game seed)
     // pruned beam: at plies in pruneMask (bit 1, 2 = middle levels, bit 3 = leaves) only the pruner's top pruneK
     // legal placements of every list are scored by the value network (null pruner = full beam)
     PrunerDev const* pruner;
@@ -42,6 +53,9 @@ struct EvalArgs {
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
 // uploads the constant tables computed on the host; call once per device before the first launch
 hipError_t upload_constants();
+// leaf values of n boards (10 words each) as the search scores candidate boards; weights in device layout
+hipError_t launch_model_values(int kind, float const* weights, uint32_t const* boards, uint32_t const* clears,
+                               uint8_t const* held, uint32_t n, float* out, hipStream_t s);
 // mode: 0 full beam, 1 full beam with traces, 2 pruned beam
 hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksPerCU);
 hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream);

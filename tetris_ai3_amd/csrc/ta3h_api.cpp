@@ -26,16 +26,59 @@ int params_of(int kind) {
     switch(kind) {
         case TA3H_MODEL_V11: return 97;
         case TA3H_MODEL_F16: return 1633;
+        case TA3H_MODEL_F32W: return 20865;
         default: return -1;
     }
 }
 
+namespace wide = ta3h::f32w; // device layout of the wide network (bytes)
+
 // floats per model in device memory (a multiple of 16 so every model starts 64-byte aligned)
-uint32_t stride_of(int kind) { return (static_cast<uint32_t>(params_of(kind)) + 15u) & ~15u; }
+uint32_t stride_of(int kind) {
+    uint32_t const n = kind == TA3H_MODEL_F32W ? wide::BYTES / 4 : static_cast<uint32_t>(params_of(kind));
+    return (n + 15u) & ~15u;
+}
+
+uint16_t bf16_bits(float v) { // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// wide network, host layout W1[128][32], b1, W2[128][128], b2, W3[128], b3 -> MFMA fragment layout:
+// w1f[t][r][q][e] = W1[16 t + r][8 q + e]; w2f[t][kb][r][q][e] = W2[16 t + r][unit(kb, q, e)] with
+// unit = 32 kb + 4 q + e (e < 4) or 32 kb + 16 + 4 q + e - 4 (the units lane quarter q holds in layer 1's
+// output tiles 2 kb, 2 kb + 1); biases and W3 as float.
+void repack_f32w(float const* src, uint8_t* dst) {
+    uint16_t* w1f = reinterpret_cast<uint16_t*>(dst + wide::W1F);
+    for(int t = 0; t < 8; ++t)
+        for(int r = 0; r < 16; ++r)
+            for(int q = 0; q < 4; ++q)
+                for(int e = 0; e < 8; ++e) w1f[((t * 16 + r) * 4 + q) * 8 + e] = bf16_bits(src[(16 * t + r) * 32 + 8 * q + e]);
+    float const* w2 = src + 4224;
+    uint16_t* w2f = reinterpret_cast<uint16_t*>(dst + wide::W2F);
+    for(int t = 0; t < 8; ++t)
+        for(int kb = 0; kb < 4; ++kb)
+            for(int r = 0; r < 16; ++r)
+                for(int q = 0; q < 4; ++q)
+                    for(int e = 0; e < 8; ++e) {
+                        int const unit = e < 4 ? 32 * kb + 4 * q + e : 32 * kb + 16 + 4 * q + (e - 4);
+                        w2f[(((t * 4 + kb) * 16 + r) * 4 + q) * 8 + e] = bf16_bits(w2[(16 * t + r) * 128 + unit]);
+                    }
+    std::memcpy(dst + wide::B1, src + 4096, 128 * sizeof(float));
+    std::memcpy(dst + wide::B2, src + 4224 + 16384, 128 * sizeof(float));
+    std::memcpy(dst + wide::W3, src + 4224 + 16512, 129 * sizeof(float)); // W3 and b3
+}
 
 // host layout (reference dense(), utility.hpp:330-347) -> device layout of the selected network
 void repack(int kind, float const* src, float* dst) {
     int const P = params_of(kind);
+    if(kind == TA3H_MODEL_F32W) {
+        std::memset(dst, 0, stride_of(kind) * sizeof(float));
+        repack_f32w(src, reinterpret_cast<uint8_t*>(dst));
+        return;
+    }
     if(kind == TA3H_MODEL_F16) {
         // W1[32][16] (512), b1 (32) unchanged; W2[32 out][32 in] -> W2T[in][out]; b2, W3, b3 unchanged
         std::memcpy(dst, src, 544 * sizeof(float));
@@ -113,6 +156,7 @@ struct ta3h_evaluator {
     uint32_t graphModels = 0;
 
     int kind = TA3H_MODEL_V11;
+    int garbage = 0;
     int wavesPerBlock = 2; // waves cooperating on one game
     int persistent = 1;
     uint32_t gridBlocks = 0; // 0: occupancy-derived
@@ -241,6 +285,13 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind) {
     return TA3H_OK;
 }
 
+int ta3h_set_garbage(ta3h_evaluator* ev, int enabled) {
+    if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
+    ev->garbage = enabled ? 1 : 0;
+    return TA3H_OK;
+}
+
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
     ev->drop_graph(); // captured settings and buffers change
@@ -311,6 +362,7 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.maxMoves = ev->maxMoves;
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
+    a.garbage = static_cast<uint32_t>(ev->garbage);
     int mode = 0, wpb = ev->wavesPerBlock;
     if(ev->prunerOn) {
         a.pruner = ev->dPruner;
@@ -463,6 +515,33 @@ int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_
         *waves_per_game = static_cast<uint32_t>(ev->prunerOn || ev->traceEvery != 0 ? 4 : ev->wavesPerBlock);
     if(num_cus) *num_cus = ev->numCUs;
     return TA3H_OK;
+}
+
+int ta3h_model_values(int kind, const float* weights, const uint32_t* boards, const uint32_t* clears,
+                      const uint8_t* held, uint32_t n, float* out) {
+    if(params_of(kind) < 0 || !weights || !boards || !clears || !held || !out) return TA3H_E_ARG;
+    if(n == 0) return TA3H_OK;
+    std::vector<float> packed(stride_of(kind));
+    repack(kind, weights, packed.data());
+    size_t const bytes = packed.size() * 4 + n * (40 + 4 + 1 + 4) + 64;
+    uint8_t* d = nullptr;
+    if(hipMalloc(&d, bytes) != hipSuccess) return TA3H_E_HIP;
+    float* dw = reinterpret_cast<float*>(d);
+    uint32_t* db = reinterpret_cast<uint32_t*>(d + packed.size() * 4);
+    uint32_t* dc = db + 10ull * n;
+    float* dout = reinterpret_cast<float*>(dc + n);
+    uint8_t* dh = reinterpret_cast<uint8_t*>(dout + n);
+    int r = TA3H_OK;
+    if(hipMemcpy(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(db, boards, 40ull * n, hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(dc, clears, 4ull * n, hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(dh, held, n, hipMemcpyHostToDevice) != hipSuccess
+       || ta3h::upload_constants() != hipSuccess
+       || ta3h::launch_model_values(kind, dw, db, dc, dh, n, dout, nullptr) != hipSuccess
+       || hipMemcpy(out, dout, 4ull * n, hipMemcpyDeviceToHost) != hipSuccess)
+        r = TA3H_E_HIP;
+    static_cast<void>(hipFree(d));
+    return r;
 }
 
 int ta3h_device_bits_canary(int* ok) {
