@@ -132,6 +132,62 @@ int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_par
  * reports the whole replay (copies included). */
 int ta3h_graph_run(ta3h_evaluator* ev, const float* weights, float* out_fitness, uint32_t* out_placements);
 
+/* ---- CMA-ES island loop (the evaluator's caller) and .champ checkpoints -----------------------------------
+ * Mirrors ta3_trainer's Trainer2 / TetrisProblem2 (ta3_trainer/src/trainer2.cpp:144-171, 317-366;
+ * tetris_problem2.cpp:29-72): `islands` x `population` models per generation, evaluated on the iteration's
+ * common seeds (G games, simulate()'s expansion of one seed drawn per iteration from the trainer seed), model
+ * fitness = mean of its games (maximised), decision vectors in double narrowed to float weights. Optimiser:
+ * separable CMA-ES per island (the reference's sep-CMA-ES option, src/pagmo/sep_cmaes.cpp), island means
+ * ~ N(0, init_std^2), step size sigma0, lambda = population, mu = lambda / 2; ring migration after each
+ * iteration. All islands' models of a generation run as one batch: one graph replay on the evaluator. */
+typedef struct ta3h_trainer ta3h_trainer;
+
+typedef struct {
+    int kind;                 /* TA3H_MODEL_* */
+    uint32_t islands;         /* 4 */
+    uint32_t population;      /* lambda per island, e.g. 50 */
+    uint32_t games;           /* games per model and generation, e.g. 50 */
+    uint32_t max_moves;       /* game cap, e.g. 2000 */
+    uint32_t generations;     /* generations per iteration, e.g. 10 */
+    double sigma0;            /* initial step size, e.g. 0.3 */
+    double init_std;          /* std of the initial island means, e.g. 0.5 */
+    uint64_t seed;            /* trainer seed: islands, sampling and per-iteration game seeds */
+} ta3h_train_config;
+
+typedef struct {
+    uint64_t iteration;        /* iterations completed */
+    double best_fitness;       /* best model fitness of the iteration */
+    double last_mean_fitness;  /* mean model fitness of its last generation */
+    double champion_fitness;   /* best model fitness so far (the champion's) */
+    double device_ms;          /* sum of the generations' graph replay times (0 with a callback) */
+    uint64_t iteration_seed;   /* seed whose expansion gave the iteration's games */
+} ta3h_iteration_report;
+
+/* Evaluation hook for running the same loop on another evaluator (e.g. a CPU restatement): fill
+ * fitness[m * games + g] for num_models x num_params weights on `games` seeds; return 0 on success. */
+typedef int (*ta3h_eval_callback)(void* ctx, const float* weights, uint32_t num_models, uint32_t num_params,
+                                  const uint64_t* seeds, uint32_t games, uint32_t max_moves, float* fitness);
+
+int ta3h_trainer_create(const ta3h_train_config* cfg, ta3h_trainer** out);
+void ta3h_trainer_destroy(ta3h_trainer* t);
+const char* ta3h_trainer_last_error(const ta3h_trainer* t);
+
+/* One iteration (cfg.generations generations) evaluated on ev (its model kind is set to cfg.kind; its
+ * pruner / garbage / launch settings apply) or, when cb is not NULL, through cb. */
+int ta3h_trainer_run_iteration(ta3h_trainer* t, ta3h_evaluator* ev, ta3h_eval_callback cb, void* ctx,
+                               ta3h_iteration_report* report);
+
+/* Best model so far (num_params doubles; out may be NULL) and its fitness. */
+int ta3h_trainer_champion(const ta3h_trainer* t, double* out, uint64_t cap, double* fitness);
+
+/* Current search mean (num_params doubles) and step size of one island. */
+int ta3h_trainer_island_state(const ta3h_trainer* t, uint32_t island, double* mean, double* sigma);
+
+/* .champ files (trainer2.cpp:251-265): u64 count, then count little-endian doubles. Read with w = NULL to
+ * learn the count. */
+int ta3h_champ_write(const char* path, const double* w, uint64_t count);
+int ta3h_champ_read(const char* path, double* w, uint64_t cap, uint64_t* count);
+
 /* ---- learned pruner (placement-proposal network) and its training traces -------------------------------
  * No reference counterpart: the reference scores every placement of every kept node (ai/search/beam.hpp:337-372,
  * 407-443). The network: 265 inputs (board slot 24c + r = bit 3 + r of column c, r < 20; one-hots piece 240 + p,

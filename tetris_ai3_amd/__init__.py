@@ -24,6 +24,21 @@ MODEL_F32W = 3
 _lib = None
 
 
+class TrainConfig(C.Structure):
+    _fields_ = [("kind", C.c_int), ("islands", C.c_uint32), ("population", C.c_uint32), ("games", C.c_uint32),
+                ("max_moves", C.c_uint32), ("generations", C.c_uint32), ("sigma0", C.c_double),
+                ("init_std", C.c_double), ("seed", C.c_uint64)]
+
+
+class IterationReport(C.Structure):
+    _fields_ = [("iteration", C.c_uint64), ("best_fitness", C.c_double), ("last_mean_fitness", C.c_double),
+                ("champion_fitness", C.c_double), ("device_ms", C.c_double), ("iteration_seed", C.c_uint64)]
+
+
+EVAL_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32,
+                            C.POINTER(C.c_uint64), C.c_uint32, C.c_uint32, C.POINTER(C.c_float))
+
+
 class HipError(RuntimeError):
     """A failure reported by the evaluator library ("hip: <what>: <message>" or an argument error)."""
 
@@ -61,6 +76,14 @@ def lib():
             "ta3h_model_values": (C.c_int, [C.c_int, f32p, u32p, u32p, C.POINTER(C.c_uint8), C.c_uint32, f32p]),
             "ta3h_graph_prepare": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
             "ta3h_graph_run": (C.c_int, [vp, f32p, f32p, u32p]),
+            "ta3h_trainer_create": (C.c_int, [C.POINTER(TrainConfig), C.POINTER(vp)]),
+            "ta3h_trainer_destroy": (None, [vp]),
+            "ta3h_trainer_last_error": (C.c_char_p, [vp]),
+            "ta3h_trainer_run_iteration": (C.c_int, [vp, vp, EVAL_CALLBACK, vp, C.POINTER(IterationReport)]),
+            "ta3h_trainer_champion": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double)]),
+            "ta3h_trainer_island_state": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "ta3h_champ_write": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_uint64]),
+            "ta3h_champ_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint64)]),
             "ta3h_pruner_num_params": (C.c_int, []),
             "ta3h_pruner_quant_info": (C.c_int, [f32p, C.c_float, C.POINTER(C.c_int)]),
             "ta3h_set_pruner": (C.c_int, [vp, f32p, C.c_float, C.c_uint32, C.c_uint32]),
@@ -289,6 +312,82 @@ class ModelEvaluator:
         self._check(lib().ta3h_pruner_logits(self._h, _ptr(w, C.c_float), float(act_range), _ptr(smp, C.c_uint32),
                                              smp.shape[0], _ptr(out, C.c_float)))
         return out
+
+
+class Trainer:
+    """CMA-ES island loop around the evaluator (ta3_trainer's Trainer2 shape): one batched generation of every
+    island per graph replay. evaluate(weights[M, P], seeds, max_moves) -> fitness[M * G] replaces the device
+    evaluator when given (e.g. the CPU restatement, to run the identical loop on the host)."""
+
+    def __init__(self, kind=MODEL_V11, islands=4, population=50, games=50, max_moves=2000, generations=10,
+                 sigma0=0.3, init_std=0.5, seed=0):
+        self.cfg = TrainConfig(kind, islands, population, games, max_moves, generations, sigma0, init_std, seed)
+        self._h = C.c_void_p()
+        if lib().ta3h_trainer_create(C.byref(self.cfg), C.byref(self._h)) != 0:
+            raise HipError("trainer: invalid configuration")
+        self.num_params = num_params(kind)
+
+    def close(self):
+        if self._h:
+            lib().ta3h_trainer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_iteration(self, evaluator: "ModelEvaluator | None" = None, evaluate=None) -> dict:
+        rep = IterationReport()
+        cb = EVAL_CALLBACK()
+        if evaluate is not None:
+            def _cb(ctx, w, m, p, seeds, games, max_moves, out):
+                try:
+                    wa = np.ctypeslib.as_array(w, shape=(m * p,)).copy()
+                    sa = np.ctypeslib.as_array(seeds, shape=(games,)).copy()
+                    fit = np.asarray(evaluate(wa, m, sa, max_moves), dtype=np.float32).reshape(-1)
+                    np.ctypeslib.as_array(out, shape=(m * games,))[:] = fit
+                    return 0
+                except Exception:
+                    return 1
+            cb = EVAL_CALLBACK(_cb)
+        h = evaluator._h if evaluator is not None else None
+        rc = lib().ta3h_trainer_run_iteration(self._h, h, cb, None, C.byref(rep))
+        if rc != 0:
+            raise HipError(lib().ta3h_trainer_last_error(self._h).decode())
+        return {f: getattr(rep, f) for f, _ in IterationReport._fields_}
+
+    def champion(self):
+        out = np.zeros(self.num_params, dtype=np.float64)
+        fit = C.c_double(0)
+        if lib().ta3h_trainer_champion(self._h, _ptr(out, C.c_double), out.size, C.byref(fit)) != 0:
+            raise HipError("trainer: no champion yet")
+        return out, fit.value
+
+    def island_state(self, island: int):
+        mean = np.zeros(self.num_params, dtype=np.float64)
+        sigma = C.c_double(0)
+        if lib().ta3h_trainer_island_state(self._h, island, _ptr(mean, C.c_double), C.byref(sigma)) != 0:
+            raise HipError("trainer: no such island")
+        return mean, sigma.value
+
+
+def champ_write(path: str, w) -> None:
+    """.champ: u64 count + count little-endian doubles (trainer2.cpp:251-265)"""
+    a = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+    if lib().ta3h_champ_write(os.fsencode(path), _ptr(a, C.c_double), a.size) != 0:
+        raise HipError(f"champ_write: cannot write {path}")
+
+
+def champ_read(path: str) -> np.ndarray:
+    n = C.c_uint64(0)
+    if lib().ta3h_champ_read(os.fsencode(path), None, 0, C.byref(n)) != 0:
+        raise HipError(f"champ_read: cannot read {path}")
+    out = np.zeros(n.value, dtype=np.float64)
+    if lib().ta3h_champ_read(os.fsencode(path), _ptr(out, C.c_double), out.size, C.byref(n)) != 0:
+        raise HipError(f"champ_read: malformed {path}")
+    return out
 
 
 def pruner_num_params() -> int:

@@ -1,0 +1,73 @@
+"""Training run: CMA-ES islands around the GPU evaluator, one graph replay per generation.
+
+    python -m tetris_ai3_amd.train --kind v11 --iterations 20 --generations 10 --out run/v11
+
+Defaults are the reference trainer's setup (4 islands x 50 population x 50 games, 10 generations per
+iteration, README.md:9) with sigma 0.3 and island means ~ N(0, 0.5^2). Prints one JSON line per iteration
+(best / champion fitness, generation latency, placements per second) and writes the champion as
+<out>_<iteration>.champ (u64 count + doubles, the reference's .champ format) and <out>.champ.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import MODEL_F16, MODEL_F32W, MODEL_V11, ModelEvaluator, Trainer, champ_write
+
+KINDS = {"v11": MODEL_V11, "f16": MODEL_F16, "f32w": MODEL_F32W}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--kind", choices=sorted(KINDS), default="v11")
+    ap.add_argument("--islands", type=int, default=4)
+    ap.add_argument("--population", type=int, default=50)
+    ap.add_argument("--games", type=int, default=50)
+    ap.add_argument("--max-moves", type=int, default=2000)
+    ap.add_argument("--generations", type=int, default=10)
+    ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--sigma0", type=float, default=0.3)
+    ap.add_argument("--init-std", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=0x7E7215)
+    ap.add_argument("--garbage", action="store_true", help="start games with 0-4 garbage rows")
+    ap.add_argument("--out", default="", help="champion path prefix (.champ files); empty = none")
+    args = ap.parse_args(argv)
+
+    kind = KINDS[args.kind]
+    trainer = Trainer(kind=kind, islands=args.islands, population=args.population, games=args.games,
+                      max_moves=args.max_moves, generations=args.generations, sigma0=args.sigma0,
+                      init_std=args.init_std, seed=args.seed)
+    ev = ModelEvaluator(model=kind)
+    if args.garbage:
+        ev.set_garbage(True)
+    if args.out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    ev._items = args.islands * args.population * args.games  # the trainer's generation batch
+    for it in range(args.iterations):
+        t0 = time.perf_counter()
+        rep = trainer.run_iteration(evaluator=ev)
+        wall = time.perf_counter() - t0
+        _, placements = ev.fetch()  # the last generation's per-item placement counts
+        w, fit = trainer.champion()
+        line = {"iteration": rep["iteration"], "best_fitness": rep["best_fitness"],
+                "champion_fitness": fit, "last_mean_fitness": rep["last_mean_fitness"],
+                "generation_ms": rep["device_ms"] / args.generations, "wall_s": wall,
+                "last_generation_placements": int(placements.sum()),
+                "last_generation_game_p50": float(np.percentile(placements, 50)),
+                "last_generation_game_p99": float(np.percentile(placements, 99))}
+        print(json.dumps(line), flush=True)
+        if args.out:
+            champ_write(f"{args.out}_{rep['iteration']:03d}.champ", w)
+            champ_write(f"{args.out}.champ", w)
+    ev.close()
+    trainer.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
