@@ -187,11 +187,19 @@ struct alignas(16) GameLds {
     uint64_t partBest[4];                     // per-wave best leaf keys
     uint64_t partMask[4];                     // per-wave kept-node / segment legality masks
     uint32_t partLegal[4];                    // per-wave legal-candidate counts of the item
-    uint32_t survivors[68];                   // traces: slots kept by a middle level (2,176 bits)
-    uint64_t segBest[64];                     // traces: best leaf key of every leaf list
-    uint32_t keptCount;                       // pruned stages: candidates kept for the value network
     EngineState eng;
 };
+
+// Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
+// logits share the lanes' candidate-board scratch, which is idle until the value network runs).
+struct alignas(16) PruneLds {
+    float bp[32][PR_H];
+    uint16_t kept[32 * 2 * 34];
+    uint64_t segBest[64];   // traces: best leaf key of every leaf list
+    uint32_t survivors[68]; // traces: slots kept by a middle level (2,176 bits)
+    uint32_t keptCount;     // pruned stages: candidates kept for the value network
+};
+
 
 __device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:112-124
     uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
@@ -732,10 +740,12 @@ __device__ inline void group_sync() {
 //    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
 //  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
 //    least one legal placement (expand_leaves, beam.hpp:407-443).
-// per-game data of the wide network: its weights and the wave's feature tile
+// per-game data of the wide network (its weights and the wave's feature tile) and, in trace launches, the
+// leaf lists' best keys
 struct WideArgs {
     uint8_t const* wb;
     uint16_t* feat;
+    uint64_t* segBest;
 };
 
 template<int KIND, int STAGE, int K, int G, bool TRACE = false>
@@ -783,7 +793,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
             acc = key > acc ? key : acc;
             if constexpr(TRACE) {
-                if(legal) atomicMax(reinterpret_cast<unsigned long long*>(&L.segBest[s]), static_cast<unsigned long long>(key));
+                if(legal) atomicMax(reinterpret_cast<unsigned long long*>(&wide.segBest[s]), static_cast<unsigned long long>(key));
             }
             ms |= legal ? (1ull << s) : 0ull;
         } else {
@@ -976,9 +986,10 @@ constexpr uint32_t PRS_ROW = 36;
 // Legality of a stage's candidates, one per lane: prs[36 s + i] = -inf for illegal placement i of list s; the
 // legal ones are listed (any order, as list << 6 | placement index) in `legalList`. Returns their count.
 template<int G>
-__device__ uint32_t pruner_legal(GameLds& L, uint32_t total, float* prs, uint16_t* legalList, uint32_t wv) {
+__device__ uint32_t pruner_legal(GameLds& L, uint32_t& count, uint32_t total, float* prs, uint16_t* legalList,
+                                 uint32_t wv) {
     uint32_t const lane = lane_id();
-    if(wv == 0 && lane == 0) L.keptCount = 0;
+    if(wv == 0 && lane == 0) count = 0;
     group_sync<G>();
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         uint32_t const f = base + lane;
@@ -995,14 +1006,14 @@ __device__ uint32_t pruner_legal(GameLds& L, uint32_t total, float* prs, uint16_
         uint64_t const bal = __ballot(legal);
         uint32_t const cnt = static_cast<uint32_t>(__popcll(bal));
         uint32_t b = 0;
-        if(lane == 0 && cnt != 0) b = atomicAdd(&L.keptCount, cnt);
+        if(lane == 0 && cnt != 0) b = atomicAdd(&count, cnt);
         b = uniform(b);
         if(legal)
             legalList[b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)))] =
                 static_cast<uint16_t>(L.candSeg[f]);
     }
     group_sync<G>();
-    return uniform(L.keptCount);
+    return uniform(count);
 }
 
 // first placement-list entry of each piece (lutOff), one byte per piece
@@ -1051,9 +1062,10 @@ __device__ void pruner_scores(GameLds const& L, PrunerDev const* __restrict__ pr
 // returns how many were kept and lists them in `kept` (list << 6 | placement index, any order). A lane reads its
 // whole list's logits with nine 16-byte loads (lanes of one list read the same addresses).
 template<int G>
-__device__ uint32_t pruner_select(GameLds& L, uint32_t total, float const* prs, uint16_t* kept, uint32_t K, uint32_t wv) {
+__device__ uint32_t pruner_select(GameLds& L, uint32_t& count, uint32_t total, float const* prs, uint16_t* kept,
+                                  uint32_t K, uint32_t wv) {
     uint32_t const lane = lane_id();
-    if(wv == 0 && lane == 0) L.keptCount = 0;
+    if(wv == 0 && lane == 0) count = 0;
     group_sync<G>();
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         uint32_t const f = base + lane;
@@ -1082,12 +1094,12 @@ __device__ uint32_t pruner_select(GameLds& L, uint32_t total, float const* prs, 
         uint64_t const bal = __ballot(keep);
         uint32_t const cnt = static_cast<uint32_t>(__popcll(bal));
         uint32_t b = 0;
-        if(lane == 0 && cnt != 0) b = atomicAdd(&L.keptCount, cnt);
+        if(lane == 0 && cnt != 0) b = atomicAdd(&count, cnt);
         b = uniform(b);
         if(keep) kept[b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)))] = static_cast<uint16_t>(si);
     }
     group_sync<G>();
-    return uniform(L.keptCount);
+    return uniform(count);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -1114,7 +1126,8 @@ __device__ inline void emit_record(EvalArgs const& a, bool want, uint32_t const*
 // one record per legal candidate of the stage; label = slot kept by the level (ply 1, 2) or the list's best
 // leaf (ply 3)
 template<int G>
-__device__ void emit_stage(EvalArgs const& a, GameLds const& L, uint32_t total, uint32_t ply, uint32_t wv) {
+__device__ void emit_stage(EvalArgs const& a, GameLds const& L, PruneLds const* P, uint32_t total, uint32_t ply,
+                           uint32_t wv) {
     uint32_t const lane = lane_id();
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         uint32_t const f = base + lane;
@@ -1130,8 +1143,8 @@ __device__ void emit_stage(EvalArgs const& a, GameLds const& L, uint32_t total, 
             int drop = 0;
             legal = fit_span(&L.boards[pb * WIDTH], e.cols, e.col0, drop);
             uint32_t const slot = s * 34 + i;
-            bool const label = ply < 3 ? ((L.survivors[slot >> 5] >> (slot & 31u)) & 1u) != 0
-                                       : (SPAN - static_cast<uint32_t>(L.segBest[s])) == slot;
+            bool const label = ply < 3 ? ((P->survivors[slot >> 5] >> (slot & 31u)) & 1u) != 0
+                                       : (SPAN - static_cast<uint32_t>(P->segBest[s])) == slot;
             meta = piece | static_cast<uint32_t>(e.ox & 3) << 3 | static_cast<uint32_t>(e.ox >> 2) << 5 | ply << 9
                  | (label ? 1u << 11 : 0u);
         }
@@ -1193,12 +1206,6 @@ struct alignas(16) FeatLds {
     uint16_t v[G][64 * 32];
 };
 
-// Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
-// logits share the lanes' candidate-board scratch, which is idle until the value network runs).
-struct alignas(16) PruneLds {
-    float bp[32][PR_H];
-    uint16_t kept[32 * 2 * 34];
-};
 
 enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
 
@@ -1250,12 +1257,12 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
         uint32_t evalTotal = total;
         if constexpr(MODE == MODE_PRUNED) {
             if((a.pruneMask >> level) & 1u) {
-                uint32_t const nl = pruner_legal<G>(L, total, prs, P->kept, wv);
+                uint32_t const nl = pruner_legal<G>(L, P->keptCount, total, prs, P->kept, wv);
                 pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
                 group_sync<G>();
                 pruner_scores<G>(L, a.pruner, P->kept, nl, cur, static_cast<uint32_t>(level), P->bp, prs, wv);
                 group_sync<G>();
-                evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
+                evalTotal = pruner_select<G>(L, P->keptCount, total, prs, P->kept, a.pruneK, wv);
                 seg = P->kept;
             }
         }
@@ -1275,15 +1282,15 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
         if constexpr(MODE == MODE_TRACE) {
             if(traceMove) {
                 if(wv == 0) {
-                    for(uint32_t i = lane; i < 68; i += 64) L.survivors[i] = 0;
+                    for(uint32_t i = lane; i < 68; i += 64) P->survivors[i] = 0;
                     wave_sync();
                     if(lane < next) {
                         uint32_t const slot = 0xFFFFFFFFu - static_cast<uint32_t>(top);
-                        atomicOr(&L.survivors[slot >> 5], 1u << (slot & 31u));
+                        atomicOr(&P->survivors[slot >> 5], 1u << (slot & 31u));
                     }
                 }
                 group_sync<G>();
-                emit_stage<G>(a, L, total, static_cast<uint32_t>(level), wv);
+                emit_stage<G>(a, L, P, total, static_cast<uint32_t>(level), wv);
             }
         }
         if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false, col);
@@ -1299,12 +1306,12 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     uint32_t evalTotal = total;
     if constexpr(MODE == MODE_PRUNED) {
         if((a.pruneMask >> 3) & 1u) {
-            uint32_t const nl = pruner_legal<G>(L, total, prs, P->kept, wv);
+            uint32_t const nl = pruner_legal<G>(L, P->keptCount, total, prs, P->kept, wv);
             pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
             group_sync<G>();
             pruner_scores<G>(L, a.pruner, P->kept, nl, cur, 3u, P->bp, prs, wv);
             group_sync<G>();
-            evalTotal = pruner_select<G>(L, total, prs, P->kept, a.pruneK, wv);
+            evalTotal = pruner_select<G>(L, P->keptCount, total, prs, P->kept, a.pruneK, wv);
             seg = P->kept;
         }
     }
@@ -1312,7 +1319,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     uint64_t leaf;
     if constexpr(MODE == MODE_TRACE) {
         if(traceMove) {
-            if(wv == 0) L.segBest[lane] = 0;
+            if(wv == 0) P->segBest[lane] = 0;
             group_sync<G>();
             leaf = run_stage<KIND, STAGE_LEAF, 1, G, true>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         } else {
@@ -1323,7 +1330,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     }
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(MODE == MODE_TRACE) {
-        if(traceMove) emit_stage<G>(a, L, total, 3u, wv);
+        if(traceMove) emit_stage<G>(a, L, P, total, 3u, wv);
     }
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
@@ -1398,7 +1405,8 @@ void beam_eval_kernel(EvalArgs a) {
         // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
-        WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr};
+        WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr,
+                      MODE == MODE_TRACE ? pruneLds->segBest : nullptr};
         if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
         if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();

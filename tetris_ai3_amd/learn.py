@@ -61,24 +61,77 @@ def train_pruner(ev: ModelEvaluator, recs, epochs: int = 3, batch: int = 512, lr
     return w, losses, time.perf_counter() - t0
 
 
-def list_recall(ev: ModelEvaluator, w, act_range: float, recs, k: int) -> dict:
-    """Per ply: the fraction of positive records that rank in the top k of their list by pruner logit
-    (a list = the records of one parent board and piece at one ply, in record order)."""
-    logit = ev.pruner_logits(w, act_range, recs)
-    meta = recs[:, 10]
-    ply = (meta >> PLY_SHIFT) & 3
-    lab = (meta >> LABEL_SHIFT) & 1
-    key = np.concatenate([recs[:, :10], (meta & 7)[:, None], ply[:, None]], axis=1)
-    _, grp = np.unique(key, axis=0, return_inverse=True)
-    grp = grp.reshape(-1)
-    # rank within the group by logit descending, record order ascending
-    order = np.lexsort((np.arange(len(logit)), -logit, grp))
-    rank = np.empty(len(order), dtype=np.int64)
-    starts = np.r_[0, np.flatnonzero(np.diff(grp[order])) + 1]
-    sizes = np.diff(np.r_[starts, len(order)])
-    rank[order] = np.arange(len(order)) - np.repeat(starts, sizes)
-    out = {}
-    for p in (1, 2, 3):
-        m = (ply == p) & (lab == 1)
-        out[p] = float((rank[m] < k).mean()) if m.any() else float("nan")
-    return out
+def _main(argv=None) -> int:
+    """Pruner study on a trained model (cfg3 shape): train on the model's own full-beam traces from one seed
+    set, then judge pruned vs full beam on `--games` fresh games. Prints one JSON line per setting."""
+    import argparse
+    import json
+
+    from . import MODEL_F16, MODEL_F32W, MODEL_V11, champ_read, expand_seeds
+
+    kinds = {"v11": MODEL_V11, "f16": MODEL_F16, "f32w": MODEL_F32W}
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--champ", required=True, help=".champ weights of the model (or models: count = n x params)")
+    ap.add_argument("--kind", choices=sorted(kinds), default="f16")
+    ap.add_argument("--games", type=int, default=10000)
+    ap.add_argument("--max-moves", type=int, default=2000)
+    ap.add_argument("--trace-games", type=int, default=200)
+    ap.add_argument("--every", type=int, default=8)
+    ap.add_argument("--records", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--settings", default="8:12,8:14,4:12,4:14",
+                    help="comma list of k:ply_mask (mask bits 1, 2 = middle levels, 3 = leaves)")
+    ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--save", default="", help="write the trained pruner (npy: weights then act_range)")
+    args = ap.parse_args(argv)
+
+    kind = kinds[args.kind]
+    w = champ_read(args.champ).astype(np.float32)
+    ev = ModelEvaluator(model=kind)
+    nm = w.size // ev.num_params
+    trace_seeds = expand_seeds(args.seed, args.trace_games)
+    recs, produced, _ = collect_traces(ev, w, nm, trace_seeds, args.max_moves, args.every, args.records * 2)
+    rng = np.random.default_rng(args.seed)
+    if recs.shape[0] > args.records:
+        recs = recs[rng.choice(recs.shape[0], args.records, replace=False)]
+    pw, losses, secs = train_pruner(ev, recs, epochs=args.epochs, seed=1)
+    act = calibrate_act_range(ev, pw, recs)
+    print(json.dumps({"traces": int(produced), "trained_on": int(recs.shape[0]), "train_s": secs,
+                      "epoch_loss": [float(x) for x in losses], "act_range": act,
+                      "label_stats": label_stats(recs)}), flush=True)
+    if args.save:
+        np.save(args.save, np.r_[pw, np.float32(act)])
+    seeds = expand_seeds(args.seed + 1, args.games)
+
+    def run():
+        ev.stage(w, nm, seeds, args.max_moves)
+        ev.launch()
+        ev.sync()
+        ms = ev.last_kernel_ms()
+        fit, pl = ev.fetch()
+        cnt = ev.fetch_counters()
+        return fit, pl, cnt, ms
+
+    ev.set_pruner(None, 0.0)
+    full, full_pl, full_cnt, full_ms = run()
+    base = {"beam": "full", "mean_score": float(full.mean()), "placements": int(full_pl.sum()), "ms": full_ms,
+            "placements_per_s": float(full_pl.sum()) / full_ms * 1e3,
+            "value_evals_per_placement": float(full_cnt[:, 1].sum()) / float(full_pl.sum())}
+    print(json.dumps(base), flush=True)
+    for item in args.settings.split(","):
+        k, mask = (int(v) for v in item.split(":"))
+        ev.set_pruner(pw, act, k=k, ply_mask=mask)
+        fit, pl, cnt, ms = run()
+        rate = float(pl.sum()) / ms * 1e3
+        print(json.dumps({"beam": "pruned", "k": k, "ply_mask": mask, "mean_score": float(fit.mean()),
+                          "mean_score_loss_pct": 100.0 * (1.0 - float(fit.mean()) / float(full.mean())),
+                          "same_games": float((fit == full).mean()), "placements": int(pl.sum()), "ms": ms,
+                          "placements_per_s": rate, "speedup": rate / base["placements_per_s"],
+                          "value_evals_per_placement": float(cnt[:, 1].sum()) / float(pl.sum())}), flush=True)
+    ev.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys as _sys
+    _sys.exit(_main())
