@@ -146,6 +146,9 @@ def main():
     ap.add_argument("--model", default="f16", choices=["f16", "v11"])
     ap.add_argument("--workload", default="generation", choices=["generation", "wide"])
     ap.add_argument("--no-pruned", action="store_true", help="skip the pruned-beam leg")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="copy 1 GiB device-to-device first: a kernel of known traffic to scale FETCH_SIZE / "
+                         "WRITE_SIZE by in the same profiled run")
     ap.add_argument("--waves-per-game", type=int, default=0, help="0: the network's tuned default")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -155,6 +158,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.pmc_calibration:
+        import torch
+        src = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     import tetris_ai3_amd as T
     wide = args.workload == "wide"
     if wide:
