@@ -45,7 +45,7 @@ C_MLP = {0: 184, 1: 1697}
 VALU_PEAK_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz (non-packed 32-bit VALU ops)
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 F32W_FLOPS = 2 * 20865      # per leaf evaluation of the wide network (on MFMA)
-PRUNE_K, PRUNE_MASK = 8, 0b1100
+PRUNE_K, PRUNE_MASK, PRUNE_EPOCHS = 8, 0b1100, 8
 
 
 def workload(model_kind: int):
@@ -85,7 +85,7 @@ def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
     recs, produced, _ = learn.collect_traces(ev, w, M, seeds, CAP, every=512, cap=3_000_000)
     rng = np.random.default_rng(1)
     train = recs if recs.shape[0] <= 2_000_000 else recs[rng.choice(recs.shape[0], 2_000_000, replace=False)]
-    pw, losses, _ = learn.train_pruner(ev, train, epochs=3, batch=512, lr=1e-3, seed=1)
+    pw, losses, _ = learn.train_pruner(ev, train, epochs=PRUNE_EPOCHS, batch=512, lr=1e-3, seed=1)
     act = learn.calibrate_act_range(ev, pw, train)
     setup_s = time.perf_counter() - t0
     ev.set_pruner(pw, act, k=PRUNE_K, ply_mask=PRUNE_MASK)
@@ -109,7 +109,8 @@ def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
             "mean_score_loss_pct": 100.0 * loss, "meets_1pct": bool(loss <= 0.01),
             "value_evals_per_placement": float(cnt[:, 1].sum()) / max(float(pl.sum()), 1.0),
             "trained_on": f"{train.shape[0]} of {produced} full-beam trace records of this generation "
-                          f"(1 move in 512 per game), 3 epochs", "training_loss": [float(x) for x in losses],
+                          f"(1 move in 512 per game), {PRUNE_EPOCHS} epochs",
+            "training_loss": [float(x) for x in losses],
             "setup_s": setup_s}
 
 

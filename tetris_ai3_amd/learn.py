@@ -78,7 +78,7 @@ def _main(argv=None) -> int:
     ap.add_argument("--trace-games", type=int, default=200)
     ap.add_argument("--every", type=int, default=8)
     ap.add_argument("--records", type=int, default=2_000_000)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--settings", default="8:12,8:14,4:12,4:14",
                     help="comma list of k:ply_mask (mask bits 1, 2 = middle levels, 3 = leaves)")
     ap.add_argument("--seed", type=int, default=0x5EED)
