@@ -1369,15 +1369,16 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 // ---------------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------------
-// Waves per SIMD the register allocation is held to: the largest occupancy each network's code fits
-// without spilling registers to memory (V11: <= 102 VGPRs, 16-32-32-1: <= 128). The pruned kernel's LDS
-// caps it at 4 workgroups of 4 waves per CU, so it gets the 128 registers of 4 waves per SIMD; one wave per
-// game needs more registers (3 waves per SIMD).
+// Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (<= 102 VGPRs, no spills);
+// 16-32-32-1 at 5 with 4 waves per game (96 VGPRs; a few loop-invariant values spill to scratch and are
+// reloaded once per stage, 10% faster than 4 waves without spills) and at 4 with 2 waves per game. The trace
+// and pruned kernels' LDS caps them at 4 workgroups of 4 waves per CU, so they get the 128 registers of 4
+// waves per SIMD; one wave per game needs more registers (3 waves per SIMD).
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
            : (KIND == MODEL_F32W ? (G == 4 && MODE != MODE_PRUNED ? 4 : 3)
-                                 : (KIND == MODEL_V11 && MODE != MODE_PRUNED ? 5 : 4));
+                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
 // (the wide network's feature tiles cap its LDS occupancy at 4 workgroups of 4 waves per CU, 3 when pruned)
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
