@@ -1,0 +1,100 @@
+"""GPU: the two-phase schedule (every game to a move budget, then only the parked survivors) changes nothing
+but the order of work: fitness, placements, work counters and trace records equal the one-phase launch's,
+which equal the oracle's."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tetris_ai3_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(ev, w, nm, seeds, cap):
+    ev.stage(w, nm, seeds, cap)
+    ev.launch()
+    fit, pl = ev.fetch()
+    return fit, pl, ev.fetch_counters()
+
+
+@pytest.mark.parametrize("kind,wpg", [(O.MODEL_V11, 2), (O.MODEL_F16, 4), (O.MODEL_F32W, 4)])
+def test_two_phases_equal_one_phase(kind, wpg):
+    say(f"case: two phases kind={kind} wpg={wpg}")
+    nm = 3
+    w = np.random.default_rng(51).normal(0, 0.5 if kind != O.MODEL_F32W else 0.3,
+                                         nm * O.num_params(kind)).astype(np.float32)
+    seeds = O.expand_seeds(52, 6)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_launch(wpg, True, 0)
+    ev.set_phases(0)
+    one = run(ev, w, nm, seeds, 120)
+    for budget in (1, 7, 40, 119, 120, 500):
+        ev.set_phases(budget)
+        two = run(ev, w, nm, seeds, 120)
+        assert one[0].tobytes() == two[0].tobytes(), budget
+        assert (one[1] == two[1]).all() and (one[2] == two[2]).all(), budget
+    ev.close()
+    if kind != O.MODEL_F32W:
+        host, host_pl = O.eval_host_mt(kind, w, nm, seeds, 120)
+        assert one[0].tobytes() == host.tobytes() and (one[1] == host_pl).all()
+
+
+def test_two_phase_traces_and_graph():
+    say("case: two-phase traces and graph replay")
+    kind, nm = O.MODEL_V11, 2
+    w = np.random.default_rng(53).normal(0, 0.5, nm * 97).astype(np.float32)
+    seeds = O.expand_seeds(54, 4)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_phases(0)
+    cap = 4_000_000
+    ev.set_trace(cap, every=3)
+    ev.stage(w, nm, seeds, 50)
+    ev.launch()
+    ev.fetch()
+    recs1, produced1 = ev.fetch_trace()
+    assert 0 < produced1 <= cap  # every record kept, so the set cannot depend on the order of work
+    ev.set_phases(10)
+    ev.set_trace(cap, every=3)
+    ev.stage(w, nm, seeds, 50)
+    ev.launch()
+    ev.fetch()
+    recs2, produced2 = ev.fetch_trace()
+    assert produced2 == produced1
+    srt = lambda r: r[np.lexsort(r.T[::-1])]
+    assert recs1.shape == recs2.shape and (srt(recs1) == srt(recs2)).all()
+    ev.set_trace(0)
+    ev.graph_prepare(nm, seeds, 50)
+    fit_g, _ = ev.graph_run(w)
+    fit_e = ev.eval(w, nm, seeds, 50)
+    host, _ = O.eval_host_mt(kind, w, nm, seeds, 50)
+    assert fit_g.tobytes() == host.tobytes() and fit_e.tobytes() == host.tobytes()
+    ev.close()
+
+
+def test_two_phase_pruned_and_autotune():
+    say("case: two-phase pruned beam and autotune")
+    kind, nm = O.MODEL_F16, 3
+    w = np.random.default_rng(55).normal(0, 0.5, nm * O.num_params(kind)).astype(np.float32)
+    seeds = O.expand_seeds(56, 5)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_pruner(T.pruner_init(3), 4.0, k=6, ply_mask=0b1110)
+    ev.set_phases(0)
+    one = run(ev, w, nm, seeds, 90)
+    for budget in (5, 30):
+        ev.set_phases(budget)
+        two = run(ev, w, nm, seeds, 90)
+        assert one[0].tobytes() == two[0].tobytes() and (one[1] == two[1]).all() and (one[2] == two[2]).all()
+    ev.set_pruner(None, 0.0)
+    timings = T.autotune(ev, w, nm, seeds, 90, candidates=((2, 0), (4, 0)), phases=(0, 20))
+    assert set(timings) == {(2, 0, 0), (2, 0, 20), (4, 0, 0), (4, 0, 20)}
+    host, host_pl = O.eval_host_mt(kind, w, nm, seeds, 90)
+    fit, pl = run(ev, w, nm, seeds, 90)[:2]
+    assert fit.tobytes() == host.tobytes() and (pl == host_pl).all()
+    ev.close()

@@ -73,6 +73,7 @@ def lib():
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
             "ta3h_feature_norms": (C.c_int, [f32p, C.c_uint32]),
             "ta3h_set_garbage": (C.c_int, [vp, C.c_int]),
+            "ta3h_set_phases": (C.c_int, [vp, C.c_uint32]),
             "ta3h_model_values": (C.c_int, [C.c_int, f32p, u32p, u32p, C.POINTER(C.c_uint8), C.c_uint32, f32p]),
             "ta3h_graph_prepare": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
             "ta3h_graph_run": (C.c_int, [vp, f32p, f32p, u32p]),
@@ -143,6 +144,11 @@ class ModelEvaluator:
     @property
     def num_params(self) -> int:
         return num_params(self.model)
+
+    def set_phases(self, first_phase_moves: int):
+        """two-phase schedule: every game to first_phase_moves, then only the survivors, the games of the models
+        with the most survivors first (0 = one phase; default 128). Results never change."""
+        self._check(lib().ta3h_set_phases(self._h, int(first_phase_moves)))
 
     def set_garbage(self, enabled: bool = True):
         """start every game of the following launches with 0-4 seeded garbage rows"""
@@ -433,19 +439,21 @@ DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4, MODEL_F32W: 4}  # V11 at 5
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
-             candidates=((1, 0), (2, 0), (4, 0))):
-    """Times one launch per launch shape (waves per game, grid) on the given batch and keeps the fastest
-    on `evaluator`. Shapes only change speed, never results; returns {shape: kernel_ms}."""
+             candidates=((1, 0), (2, 0), (4, 0)), phases=(128,)):
+    """Times one launch per launch shape (waves per game, grid) and first-phase budget on the given batch and
+    keeps the fastest on `evaluator`. Neither changes results; returns {(wpg, grid, phase): kernel_ms}."""
     timings = {}
-    evaluator.stage(weights, num_models, seeds, max_moves)
     for wpg, grid in candidates:
-        evaluator.set_launch(wpg, True, grid)
-        evaluator.stage(weights, num_models, seeds, max_moves)
-        evaluator.launch()
-        evaluator.sync()
-        timings[(wpg, grid)] = evaluator.last_kernel_ms()
+        for phase in phases:
+            evaluator.set_launch(wpg, True, grid)
+            evaluator.set_phases(phase)
+            evaluator.stage(weights, num_models, seeds, max_moves)
+            evaluator.launch()
+            evaluator.sync()
+            timings[(wpg, grid, phase)] = evaluator.last_kernel_ms()
     best = min(timings, key=timings.get)
     evaluator.set_launch(best[0], True, best[1])
+    evaluator.set_phases(best[2])
     evaluator.stage(weights, num_models, seeds, max_moves)
     return timings
 

@@ -170,6 +170,10 @@ struct EngineState {
     uint32_t placements;
 };
 
+constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
+static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 4 <= PARK_WORDS && ENGINE_WORDS <= 64,
+              "a parked game fits its record and one wave copies it");
+
 struct alignas(16) GameLds {
     uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
     uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:91-113)
@@ -1377,7 +1381,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
-           : (KIND == MODEL_F32W ? (G == 4 && MODE != MODE_PRUNED ? 4 : 3)
+           : (KIND == MODEL_F32W ? (G == 4 && MODE == MODE_PLAIN ? 4 : 3)
                                  : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
 // (the wide network's feature tiles cap its LDS occupancy at 4 workgroups of 4 waves per CU, 3 when pruned)
 
@@ -1396,38 +1400,66 @@ void beam_eval_kernel(EvalArgs a) {
     uint32_t const lane = lane_id();
     uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
     uint32_t item = a.persistent ? 0u : blockIdx.x;
+    // second phase of a two-phase schedule: the work items are the parked games
+    uint32_t const nItems = a.resumePhase ? uniform(*a.resumeCount) : a.items;
     for(;;) {
         if(a.persistent) {
             if(threadIdx.x == 0) L.item = atomicAdd(a.queue, 1u);
             group_sync<G>();
             item = uniform(L.item);
         }
-        if(item >= a.items) break;
+        if(item >= nItems) break;
+        if(a.resumePhase) item = uniform(a.resumeOrder[item]);
         // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
         WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr,
                       MODE == MODE_TRACE ? pruneLds->segBest : nullptr};
         if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
-        if(threadIdx.x == 0) {
+        uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
+        uint64_t nApply = 0;
+        uint32_t nLegal = 0, mv = 0;
+        if(a.resumePhase) { // a parked game: its engine state, counters and move index
+            if(wv == 0 && lane < ENGINE_WORDS) reinterpret_cast<uint32_t*>(&L.eng)[lane] = park[lane];
+            nApply = uniform(static_cast<uint64_t>(park[ENGINE_WORDS]) | static_cast<uint64_t>(park[ENGINE_WORDS + 1]) << 32);
+            nLegal = wv == 0 ? uniform(park[ENGINE_WORDS + 2]) : 0u;
+            mv = uniform(park[ENGINE_WORDS + 3]);
+        } else if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
             engine_reset(L.eng, a.seeds[g]);
             if(a.garbage) add_garbage_rows(L.eng, a.seeds[g]);
         }
         group_sync<G>();
-        uint64_t nApply = 0;
-        uint32_t nLegal = 0;
         // traced moves: index congruent to the game seed modulo traceEvery (spreads them over the games)
         uint32_t const tracePhase =
             MODE == MODE_TRACE && a.traceEvery != 0 ? uniform(static_cast<uint32_t>(a.seeds[g] % a.traceEvery)) : 0u;
-        for(uint32_t mv = 0; mv < a.maxMoves; ++mv) {
-            if(uniform(L.eng.gameOver) != 0) break;
-            bool const traceMove = MODE == MODE_TRACE && a.traceEvery != 0 && mv % a.traceEvery == tracePhase;
-            if(!play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col, traceMove, nApply, nLegal)) break;
+        uint32_t const limit =
+            a.phaseMoves != 0 && a.persistent && !a.resumePhase && a.phaseMoves < a.maxMoves ? a.phaseMoves : a.maxMoves;
+        bool ended = false;
+        for(; mv < limit; ++mv) {
+            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col,
+                                                                         MODE == MODE_TRACE && a.traceEvery != 0
+                                                                             && mv % a.traceEvery == tracePhase,
+                                                                         nApply, nLegal)) {
+                ended = true;
+                break;
+            }
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
         group_sync<G>();
-        if(threadIdx.x == 0) {
+        if(!ended && limit < a.maxMoves) { // park the game for the second phase
+            uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
+            if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
+            if(threadIdx.x == 0) {
+                uint32_t legal = 0;
+                for(int k = 0; k < G; ++k) legal += L.partLegal[k];
+                out[ENGINE_WORDS] = static_cast<uint32_t>(nApply);
+                out[ENGINE_WORDS + 1] = static_cast<uint32_t>(nApply >> 32);
+                out[ENGINE_WORDS + 2] = legal;
+                out[ENGINE_WORDS + 3] = mv;
+                a.resumeList[atomicAdd(a.resumeCount, 1u)] = item;
+            }
+        } else if(threadIdx.x == 0) {
             a.fitness[item] = final_score(L.eng);
             if(a.placements) a.placements[item] = L.eng.placements;
             if(a.counters) {
@@ -1475,6 +1507,42 @@ __global__ __launch_bounds__(64) void model_value_kernel(float const* __restrict
 
 __global__ void reset_counter_kernel(uint32_t* q) {
     if(threadIdx.x == 0) q[0] = 0;
+}
+
+// Between the two phases (one workgroup): counts the parked games of every model, then writes the parked list
+// in descending order of that count (a counting sort on min(count, ORDER_BINS - 1); the order inside a bin is
+// whatever the atomics give, which changes no result, only the order of work).
+constexpr uint32_t ORDER_BINS = 1024;
+__global__ void __launch_bounds__(ORDER_BINS) order_resume_kernel(uint32_t const* list, uint32_t const* count,
+                                                                   uint32_t* order, uint32_t* modelParked,
+                                                                   uint32_t models, uint32_t games) {
+    __shared__ uint32_t bins[ORDER_BINS];
+    uint32_t const t = threadIdx.x, n = *count;
+    for(uint32_t m = t; m < models; m += ORDER_BINS) modelParked[m] = 0;
+    bins[t] = 0;
+    __syncthreads();
+    for(uint32_t i = t; i < n; i += ORDER_BINS) atomicAdd(&modelParked[list[i] / games], 1u);
+    __threadfence_block();
+    __syncthreads();
+    auto bin_of = [&](uint32_t item) { // most parked games -> bin 0
+        uint32_t const c = __hip_atomic_load(&modelParked[item / games], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return ORDER_BINS - 1 - (c < ORDER_BINS - 1 ? c : ORDER_BINS - 1);
+    };
+    for(uint32_t i = t; i < n; i += ORDER_BINS) atomicAdd(&bins[bin_of(list[i])], 1u);
+    __syncthreads();
+    if(t == 0) { // exclusive prefix sum over the bins (1024 adds)
+        uint32_t run = 0;
+        for(uint32_t b = 0; b < ORDER_BINS; ++b) {
+            uint32_t const c = bins[b];
+            bins[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for(uint32_t i = t; i < n; i += ORDER_BINS) {
+        uint32_t const item = list[i];
+        order[atomicAdd(&bins[bin_of(item)], 1u)] = item;
+    }
 }
 
 // Device self-check of the bit intrinsics the board code relies on (device_canary.hpp:16); writes 1 on
@@ -1568,6 +1636,8 @@ __global__ __launch_bounds__(64) void pruner_infer_kernel(PrunerDev const* __res
 }
 } // namespace
 
+hipError_t launch_model(int kind, EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream);
+
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
     if(a.persistent) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
@@ -1579,6 +1649,26 @@ hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t block
         hipError_t const e = hipGetLastError();
         if(e != hipSuccess) return e;
     }
+    if(a.phaseMoves != 0 && a.persistent) { // two phases: all items to phaseMoves, then the parked games
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.resumeCount);
+        hipError_t e = hipGetLastError();
+        if(e != hipSuccess) return e;
+        EvalArgs b = a;
+        b.resumePhase = 0;
+        e = launch_model(kind, b, wpb, blocks, stream);
+        if(e != hipSuccess) return e;
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
+        if((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(order_resume_kernel, dim3(1), dim3(ORDER_BINS), 0, stream, a.resumeList, a.resumeCount,
+                           a.resumeOrder, a.modelParked, a.items / a.games, a.games);
+        if((e = hipGetLastError()) != hipSuccess) return e;
+        b.resumePhase = 1;
+        return launch_model(kind, b, wpb, blocks, stream);
+    }
+    return launch_model(kind, a, wpb, blocks, stream);
+}
+
+hipError_t launch_model(int kind, EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
     switch(kind) {
         case MODEL_V11: return launch_kind<MODEL_V11>(a, wpb, blocks, stream);
         case MODEL_F16: return launch_kind<MODEL_F16>(a, wpb, blocks, stream);

@@ -23,6 +23,8 @@ namespace f32w {
 inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 41984, B3 = 42496, BYTES = 42512;
 }
 
+inline constexpr uint32_t PARK_WORDS = 64; // parked game: engine state + counters + move index (256 B)
+
 struct EvalArgs {
     float const* weights;     // device layout per model, weightStride floats apart
     uint64_t const* seeds;    // one per game
@@ -37,6 +39,19 @@ struct EvalArgs {
     uint32_t weightStride;
     uint32_t persistent;
     uint32_t garbage; // start every game with 0-4 garbage rows (seeded from the game seed)
+    // Two-phase schedule (persistent mode): the first launch plays every item for at most phaseMoves moves and
+    // parks the unfinished ones (engine state, counters, move index) in parked[item], listing them in
+    // resumeList[0 .. *resumeCount); the second launch (resumePhase = 1) plays only the listed items on to
+    // maxMoves, so the long games all start together, in resumeOrder: the games of the models with the most
+    // parked games first (a model's games have similar lengths, and the longest work should start first).
+    // phaseMoves = 0: one phase.
+    uint32_t phaseMoves;
+    uint32_t resumePhase;
+    uint32_t* parked;      // [items][PARK_WORDS]
+    uint32_t* resumeList;  // [items], in parking order
+    uint32_t* resumeCount; // one counter
+    uint32_t* resumeOrder; // [items], resumeList sorted by the model's parked-game count, descending
+    uint32_t* modelParked; // [items / games] parked games per model
     // pruned beam: at plies in pruneMask (bit 1, 2 = middle levels, bit 3 = leaves) only the pruner's top pruneK
     // legal placements of every list are scored by the value network (null pruner = full beam)
     PrunerDev const* pruner;

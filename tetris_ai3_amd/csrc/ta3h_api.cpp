@@ -113,6 +113,12 @@ struct ta3h_evaluator {
     size_t timelineCap = 0;
     bool timelineOn = false;
     uint32_t* dQueue = nullptr;
+    // two-phase schedule
+    uint32_t phaseMoves = 128; // first-phase move budget of the two-phase schedule; 0 = one phase
+    uint32_t* dParked = nullptr;
+    size_t parkedCap = 0;
+    uint32_t* dResume = nullptr; // parked list [items], its counter, resume order [items], per-model counts
+    size_t resumeCap = 0;
 
     // pruned beam
     ta3h::PrunerDev* dPruner = nullptr;
@@ -251,7 +257,7 @@ struct ta3h_evaluator {
         if(dCounters) static_cast<void>(hipFree(dCounters));
         if(dTimeline) static_cast<void>(hipFree(dTimeline));
         if(dQueue) static_cast<void>(hipFree(dQueue));
-        for(void* p : {static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
+        for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
                        static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
                        static_cast<void*>(dOrder), static_cast<void*>(dAct), static_cast<void*>(dDelta),
                        static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut)})
@@ -282,6 +288,13 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind) {
     if(params_of(kind) < 0) return ev->fail_arg("set_model: unknown model kind " + std::to_string(kind));
     ev->kind = kind;
     ev->staged = false;
+    return TA3H_OK;
+}
+
+int ta3h_set_phases(ta3h_evaluator* ev, uint32_t first_phase_moves) {
+    if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
+    ev->phaseMoves = first_phase_moves;
     return TA3H_OK;
 }
 
@@ -363,6 +376,19 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
     a.garbage = static_cast<uint32_t>(ev->garbage);
+    if(ev->phaseMoves != 0 && ev->persistent && ev->phaseMoves < ev->maxMoves) {
+        int r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked");
+        if(r) return r;
+        // list [items], counter, order [items], per-model counts [items / games]
+        size_t const words = 2 * static_cast<size_t>(ev->items) + 1 + ev->items / ev->games;
+        if((r = ev->grow(ev->dResume, ev->resumeCap, words, "malloc resume list"))) return r;
+        a.phaseMoves = ev->phaseMoves;
+        a.parked = ev->dParked;
+        a.resumeList = ev->dResume;
+        a.resumeCount = ev->dResume + ev->items;
+        a.resumeOrder = ev->dResume + ev->items + 1;
+        a.modelParked = ev->dResume + 2 * static_cast<size_t>(ev->items) + 1;
+    }
     int mode = 0, wpb = ev->wavesPerBlock;
     if(ev->prunerOn) {
         a.pruner = ev->dPruner;
