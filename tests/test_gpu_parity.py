@@ -78,6 +78,38 @@ def test_single_model_matches_oracle(kind):
     e.close()
 
 
+@pytest.mark.parametrize("kind", [T.MODEL_V11, T.MODEL_F16])
+def test_zero_skipping_matches_oracle(kind):
+    """The value networks skip the steps whose input is zero in every active lane of a wave (exact for finite
+    weights); a model with a non-finite weight takes the plain path. One batch against the oracle: a random
+    model; one with -0.0 biases (the signed-zero corner the skipping must not change); one with half of its
+    first-layer units dead (negative bias, zero weights: most units skip); one whose output bias is +inf
+    (plain path; every leaf value is +inf, so ties fall to the lowest slot)."""
+    P = T.num_params(kind)
+    b1, n1, b3 = (80, 8, 96) if kind == T.MODEL_V11 else (512, 32, 1632)
+    fan_in = 10 if kind == T.MODEL_V11 else 16
+    rng = np.random.default_rng(71)
+    ws = rng.normal(0, 0.5, (4, P)).astype(np.float32)
+    ws[1, b1:b1 + n1] = -0.0
+    if kind == T.MODEL_F16:
+        ws[1, 1568:1600] = -0.0
+    ws[1, b3] = -0.0
+    dead = np.arange(0, n1, 2)
+    for o in dead:
+        ws[2, o * fan_in:(o + 1) * fan_in] = 0.0
+        ws[2, b1 + o] = -1.0
+    ws[3, b3] = np.inf
+    seeds = O.expand_seeds(72, 4)
+    say(f"case: zero skipping kind={kind}")
+    e = T.ModelEvaluator(model=kind)
+    e.stage(ws.reshape(-1), 4, seeds, 150)
+    e.launch()
+    got, pl = e.fetch()
+    e.close()
+    want, want_pl = O.eval_host_mt(kind, ws.reshape(-1), 4, seeds, 150)
+    assert got.tobytes() == want.tobytes() and (pl == want_pl).all(), (got, want)
+
+
 def test_multi_model_batch_matches_single_and_host(ev):
     """parity_test.cpp:225-255"""
     rng = np.random.default_rng(100)

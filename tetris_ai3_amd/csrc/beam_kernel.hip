@@ -411,7 +411,8 @@ __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::
 __device__ inline float mad_rn(float acc, float a, float b) { return __fadd_rn(acc, __fmul_rn(a, b)); }
 __device__ inline float fma_rn(float acc, float a, float b) { return __fmaf_rn(a, b, acc); }
 
-// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347)
+// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347). Its network is too small
+// for the zero skipping of the 16-feature network to pay (measured 4% slower per generation).
 __device__ inline float value_v11(wptr w, float const (&f)[16]) {
     float out = w[96];
 #pragma unroll
@@ -424,23 +425,54 @@ __device__ inline float value_v11(wptr w, float const (&f)[16]) {
     return out;
 }
 
+// features of the 16-feature set that are zero across a whole chunk of candidates most of the time: the
+// multi-line clear counts (1-3); with them the single-clear count, the held-I flag and the two line-clear
+// totals (0, 8, 13, 14)
+constexpr uint32_t RARE_FEATURES = 0xEu, F16_OFTEN_ZERO = 0x610Fu;
+
 // 16 -> 32 ReLU -> 32 ReLU -> 1 on the device layout: W1[32][16], b1[32], W2 transposed [32 in][32 out],
 // b2[32], W3[32], b3. Each step is one fused multiply-add (the network's defined arithmetic); layer 2
 // accumulates in ascending input order, the order of a row-wise dense().
 constexpr int F16_W1 = 0, F16_B1 = 512, F16_W2T = 544, F16_B2 = 1568, F16_W3 = 1600, F16_B3 = 1632;
 
-__device__ inline float value_f16(wptr w, float const (&f)[16]) {
-    float acc[32];
-#pragma unroll
-    for(int j = 0; j < 32; ++j) acc[j] = w[F16_B2 + j];
+// Hidden layers with exact skipping (SPARSE, finite weights only): a step fma(acc, x, w) whose x is zero in every
+// active lane adds a zero product, which can change only the sign of a zero accumulator, and every layer-1 and
+// layer-2 accumulator reaches the output only through relu, which maps both zeros to +0. SKIP lists the features
+// known to be zero in every active lane; with SPARSE a first-layer unit that is zero in every active lane skips
+// its 32 second-layer steps (a wave-uniform branch).
+template<uint32_t SKIP, bool SPARSE>
+__device__ inline void f16_hidden(wptr w, float const (&f)[16], float (&acc)[32]) {
 #pragma unroll 2
     for(int o = 0; o < 32; ++o) {
         float v = w[F16_B1 + o];
 #pragma unroll
-        for(int i = 0; i < 16; ++i) v = fma_rn(v, f[i], w[F16_W1 + o * 16 + i]);
+        for(int i = 0; i < 16; ++i)
+            if(((SKIP >> i) & 1u) == 0) v = fma_rn(v, f[i], w[F16_W1 + o * 16 + i]);
         float const h = relu(v);
+        if(!SPARSE || __ballot(h != 0.0f) != 0) {
 #pragma unroll
-        for(int j = 0; j < 32; ++j) acc[j] = fma_rn(acc[j], h, w[F16_W2T + o * 32 + j]);
+            for(int j = 0; j < 32; ++j) acc[j] = fma_rn(acc[j], h, w[F16_W2T + o * 32 + j]);
+        }
+    }
+}
+
+// sparse: the model's weights are all finite (checked per item), which makes the skipping exact
+__device__ inline float value_f16(wptr w, float const (&f)[16], bool sparse) {
+    float acc[32];
+#pragma unroll
+    for(int j = 0; j < 32; ++j) acc[j] = w[F16_B2 + j];
+    if(!sparse) {
+        f16_hidden<0u, false>(w, f, acc);
+    } else {
+        bool rare = false, often = false;
+#pragma unroll
+        for(int i = 0; i < 16; ++i) {
+            if((RARE_FEATURES >> i) & 1u) rare |= f[i] != 0.0f;
+            if((F16_OFTEN_ZERO >> i) & 1u) often |= f[i] != 0.0f;
+        }
+        if(__ballot(often) == 0) f16_hidden<F16_OFTEN_ZERO, true>(w, f, acc);
+        else if(__ballot(rare) == 0) f16_hidden<RARE_FEATURES, true>(w, f, acc);
+        else f16_hidden<0u, true>(w, f, acc);
     }
     float out = w[F16_B3];
 #pragma unroll
@@ -568,10 +600,10 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
 }
 
 template<int KIND>
-__device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool heldIsI, wptr w) {
+__device__ inline float leaf_value(uint32_t const* col, uint32_t clears, bool heldIsI, wptr w, bool sparse) {
     float f[16];
     leaf_features<KIND>(col, clears, heldIsI, f, nullptr);
-    if constexpr(KIND == MODEL_F16) return value_f16(w, f);
+    if constexpr(KIND == MODEL_F16) return value_f16(w, f, sparse);
     else return value_v11(w, f);
 }
 
@@ -750,6 +782,7 @@ struct WideArgs {
     uint8_t const* wb;
     uint16_t* feat;
     uint64_t* segBest;
+    bool finite; // every weight of the game's model is finite (enables the exact zero skipping of value_f16)
 };
 
 template<int KIND, int STAGE, int K, int G, bool TRACE = false>
@@ -781,7 +814,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
                     leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
                     store_features16(wide.feat, ft);
                 } else {
-                    score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl));
+                    score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl, wide.finite));
                 }
             }
         }
@@ -1414,8 +1447,16 @@ void beam_eval_kernel(EvalArgs a) {
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
         WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr,
-                      MODE == MODE_TRACE ? pruneLds->segBest : nullptr};
+                      MODE == MODE_TRACE ? pruneLds->segBest : nullptr, false};
         if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
+        if constexpr(KIND == MODEL_F16) { // each wave checks that the model's weights are all finite
+            uint32_t const* const wbits =
+                reinterpret_cast<uint32_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride);
+            bool inf = false;
+            for(uint32_t k = lane; k <= static_cast<uint32_t>(F16_B3); k += 64)
+                inf |= (wbits[k] & 0x7F800000u) == 0x7F800000u;
+            wide.finite = __ballot(inf) == 0;
+        }
         uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
         uint64_t nApply = 0;
         uint32_t nLegal = 0, mv = 0;
@@ -1500,7 +1541,7 @@ __global__ __launch_bounds__(64) void model_value_kernel(float const* __restrict
         wave_sync();
         v = value_f32w(reinterpret_cast<uint8_t const*>(weights), feat);
     } else {
-        v = leaf_value<KIND>(col, clears[src], heldI, (wptr)weights);
+        v = leaf_value<KIND>(col, clears[src], heldI, (wptr)weights, false);
     }
     if(b < n) out[b] = v;
 }
