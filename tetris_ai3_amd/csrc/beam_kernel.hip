@@ -625,10 +625,15 @@ __device__ inline void store_features16(uint16_t* featTile, float const (&f)[16]
 }
 
 // Value of every lane's candidate (the 64 rows of featWave; rows of illegal lanes hold stale values whose
-// results are discarded: MFMA columns are independent). Per 16-candidate column tile: layer 1 = 8 MFMAs,
-// ReLU, bfloat16; layers 2 and 3 = 32 MFMAs, ReLU, then the FP32 output FMA chain of lane quarter q over
-// o = 16 t + 4 q + j; value = ((p0 + p1) + (p2 + p3)) + b3.
+// results are discarded: MFMA columns are independent). The 64 candidates are four 16-candidate column tiles,
+// taken CB at a time: each weight fragment is loaded once for the CB tiles (the model's 42 KB of fragments
+// exceed the vector L1, so every reload is an L2 read; four tiles per block run the wide generation 1.8x
+// faster than one). Per column tile: layer 1 = 8 MFMAs, ReLU, bfloat16; layers 2 and 3 = 32 MFMAs, ReLU, then
+// the FP32 output FMA chain of lane quarter q over o = 16 t + 4 q + j; value = ((p0 + p1) + (p2 + p3)) + b3.
+// Every tile's operations and their order do not depend on CB.
+template<int CB>
 __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave) {
+    static_assert(CB == 1 || CB == 2 || CB == 4);
     uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
     bf16x8 const* w1f = reinterpret_cast<bf16x8 const*>(wb + f32w::W1F);
     bf16x8 const* w2f = reinterpret_cast<bf16x8 const*>(wb + f32w::W2F);
@@ -638,32 +643,55 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
     float const b3 = *reinterpret_cast<float const*>(wb + f32w::B3);
     float mine = 0.0f;
 #pragma unroll 1
-    for(uint32_t ct = 0; ct < 4; ++ct) {
-        bf16x8 const x = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * ct + r16, q)]);
-        bf16x8 h[4]; // layer-2 B operands: K block kb = layer-1 tiles 2 kb, 2 kb + 1
+    for(uint32_t cb = 0; cb < 4; cb += CB) {
+        bf16x8 h[CB][4]; // layer-2 B operands of each tile: K block kb = layer-1 tiles 2 kb, 2 kb + 1
+        {
+            bf16x8 x[CB];
 #pragma unroll
-        for(uint32_t t = 0; t < 8; ++t) {
-            bf16x8 const* const w1l = opaque_ptr(w1f); // one tile's fragment in flight at a time
-            f32x4 acc = b1[4 * t + q]; // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1l[(t * 16 + r16) * 4 + q], x, acc, 0, 0, 0);
+            for(int c = 0; c < CB; ++c) x[c] = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * (cb + c) + r16, q)]);
 #pragma unroll
-            for(int j = 0; j < 4; ++j) h[t >> 1][4 * (t & 1u) + j] = static_cast<__bf16>(__builtin_fmaxf(acc[j], 0.0f));
+            for(uint32_t t = 0; t < 8; ++t) {
+                bf16x8 const* const w1l = opaque_ptr(w1f); // one tile's fragment in flight at a time
+                bf16x8 const a = w1l[(t * 16 + r16) * 4 + q];
+                f32x4 const bias = b1[4 * t + q]; // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
+#pragma unroll
+                for(int c = 0; c < CB; ++c) {
+                    f32x4 const acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c], bias, 0, 0, 0);
+#pragma unroll
+                    for(int j = 0; j < 4; ++j)
+                        h[c][t >> 1][4 * (t & 1u) + j] = static_cast<__bf16>(__builtin_fmaxf(acc[j], 0.0f));
+                }
+            }
         }
-        float part = 0.0f;
+        float part[CB];
+#pragma unroll
+        for(int c = 0; c < CB; ++c) part[c] = 0.0f;
 #pragma unroll 1
         for(uint32_t t = 0; t < 8; ++t) {
-            f32x4 acc = b2[4 * t + q];
+            f32x4 acc[CB];
+            f32x4 const bias = b2[4 * t + q];
 #pragma unroll
-            for(uint32_t kb = 0; kb < 4; ++kb)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2f[((t * 4 + kb) * 16 + r16) * 4 + q], h[kb], acc, 0, 0, 0);
+            for(int c = 0; c < CB; ++c) acc[c] = bias;
+#pragma unroll
+            for(uint32_t kb = 0; kb < 4; ++kb) {
+                bf16x8 const a = w2f[((t * 4 + kb) * 16 + r16) * 4 + q];
+#pragma unroll
+                for(int c = 0; c < CB; ++c) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[c][kb], acc[c], 0, 0, 0);
+            }
             f32x4 const w = w3[4 * t + q];
 #pragma unroll
-            for(int j = 0; j < 4; ++j) part = __fmaf_rn(__builtin_fmaxf(acc[j], 0.0f), w[j], part);
+            for(int c = 0; c < CB; ++c)
+#pragma unroll
+                for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(__builtin_fmaxf(acc[c][j], 0.0f), w[j], part[c]);
         }
-        part += __shfl_xor(part, 16); // (p0 + p1), (p2 + p3)
-        part += __shfl_xor(part, 32); // (p0 + p1) + (p2 + p3)
-        float const v = part + b3;
-        mine = q == ct ? v : mine; // lane 16 ct + r16 owns candidate 16 ct + r16
+#pragma unroll
+        for(int c = 0; c < CB; ++c) {
+            float p = part[c];
+            p += __shfl_xor(p, 16); // (p0 + p1), (p2 + p3)
+            p += __shfl_xor(p, 32); // (p0 + p1) + (p2 + p3)
+            float const v = p + b3;
+            mine = q == cb + static_cast<uint32_t>(c) ? v : mine; // lane 16 ct + r16 owns candidate 16 ct + r16
+        }
     }
     return mine;
 }
@@ -785,7 +813,7 @@ struct WideArgs {
     bool finite; // every weight of the game's model is finite (enables the exact zero skipping of value_f16)
 };
 
-template<int KIND, int STAGE, int K, int G, bool TRACE = false>
+template<int KIND, int STAGE, int K, int G, bool TRACE = false, int WIDE_CB = 4>
 __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
                                      uint64_t* mask, uint32_t& nLegal, WideArgs const& wide) {
     uint32_t const lane = lane_id();
@@ -820,7 +848,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         }
         if constexpr(KIND == MODEL_F32W) { // the whole wave runs the network over the chunk's feature tile
             wave_sync();
-            float const v = value_f32w(wide.wb, wide.feat);
+            float const v = value_f32w<WIDE_CB>(wide.wb, wide.feat);
             if(legal) score = ordered_bits(v);
             wave_sync(); // the tile is rewritten by the next chunk
         }
@@ -1254,6 +1282,7 @@ template<int KIND, int G, int MODE>
 __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, WideArgs const& wide, uint32_t wv,
                           uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal) {
     uint32_t const lane = lane_id();
+    constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1273,7 +1302,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
     nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
@@ -1304,8 +1333,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
             }
         }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
-                         : run_stage<KIND, STAGE_MID, K2, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
+                         : run_stage<KIND, STAGE_MID, K2, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
@@ -1358,12 +1387,12 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
         if(traceMove) {
             if(wv == 0) P->segBest[lane] = 0;
             group_sync<G>();
-            leaf = run_stage<KIND, STAGE_LEAF, 1, G, true>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G, true, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         } else {
-            leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         }
     } else {
-        leaf = run_stage<KIND, STAGE_LEAF, 1, G>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
     }
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(MODE == MODE_TRACE) {
@@ -1414,9 +1443,9 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
-           : (KIND == MODEL_F32W ? (G == 4 && MODE == MODE_PLAIN ? 4 : 3)
+           : (KIND == MODEL_F32W ? 3
                                  : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
-// (the wide network's feature tiles cap its LDS occupancy at 4 workgroups of 4 waves per CU, 3 when pruned)
+// (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two)
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
@@ -1539,7 +1568,7 @@ __global__ __launch_bounds__(64) void model_value_kernel(float const* __restrict
         leaf_features<KIND>(col, clears[src], heldI, f, feat);
         store_features16(feat, f);
         wave_sync();
-        v = value_f32w(reinterpret_cast<uint8_t const*>(weights), feat);
+        v = value_f32w<4>(reinterpret_cast<uint8_t const*>(weights), feat);
     } else {
         v = leaf_value<KIND>(col, clears[src], heldI, (wptr)weights, false);
     }
