@@ -200,7 +200,7 @@ def main():
     placements = int(pl.sum())
     ms_per_step = elapsed * 1000.0 / args.steps
     value = placements * args.steps / elapsed
-    applies, legal = int(cnt[:, 0].sum()), int(cnt[:, 1].sum())
+    applies, legal, lanes = int(cnt[:, 0].sum()), int(cnt[:, 1].sum()), int(cnt[:, 2].sum())
     kms = float(np.mean(kernel_ms))
     lengths = np.sort(pl)
     if wide:  # the wide network runs on MFMA: price its flops against the BF16 peak
@@ -248,6 +248,7 @@ def main():
             "game_length_max": int(lengths[-1]),
             "mean_score": float(fit.mean()),
             "candidates_per_placement": applies / max(placements, 1),
+            "lane_utilisation": legal / max(lanes, 1),
             "launch": info,
             "kernel_ms": kms,
         },

@@ -89,8 +89,9 @@ int ta3h_launch(ta3h_evaluator* ev);
 int ta3h_sync(ta3h_evaluator* ev);
 int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements);
 
-/* Per-item work counters of the last launch: out[2*item] = candidate placements fitted, out[2*item+1] =
- * candidates that fit and were scored (the algorithmic work of the search; no reference counterpart). */
+/* Per-item work counters of the last launch (no reference counterpart): out[3*item] = candidate placements
+ * fitted, out[3*item+1] = candidates that fit and were scored (the algorithmic work of the search),
+ * out[3*item+2] = lanes the scoring chunks occupied (64 per chunk; legal / lanes = lane utilisation). */
 int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 
 /* Optional per-item trace (no reference counterpart): when enabled before ta3h_stage, each launch records

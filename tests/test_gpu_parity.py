@@ -110,6 +110,30 @@ def test_zero_skipping_matches_oracle(kind):
     assert got.tobytes() == want.tobytes() and (pl == want_pl).all(), (got, want)
 
 
+def test_work_counters():
+    """Per-item counters: candidates fitted and legal equal the oracle's; candidate lanes issued are whole
+    64-lane chunks covering every fitted candidate of a plain beam, with less than one chunk of padding per
+    stage (four stages per move)"""
+    kind, nm = T.MODEL_F16, 3
+    w = np.random.default_rng(81).normal(0, 0.5, nm * T.num_params(kind)).astype(np.float32)
+    seeds = O.expand_seeds(82, 4)
+    say("case: work counters")
+    e = T.ModelEvaluator(model=kind)
+    e.stage(w, nm, seeds, 100)
+    e.launch()
+    _, pl = e.fetch()
+    cnt = e.fetch_counters().astype(np.int64)
+    e.close()
+    P = T.num_params(kind)
+    for m in range(nm):
+        c = O.Counters()
+        O.eval_host(kind, w[m * P:(m + 1) * P], seeds, 100, c)
+        assert int(cnt[m * 4:(m + 1) * 4, 0].sum()) == c.applies and int(cnt[m * 4:(m + 1) * 4, 1].sum()) == c.legal
+    applies, legal, lanes = cnt[:, 0], cnt[:, 1], cnt[:, 2]
+    assert (lanes % 64 == 0).all() and (legal <= applies).all() and (applies <= lanes).all()
+    assert (lanes - applies < 4 * 64 * (pl.astype(np.int64) + 1)).all()
+
+
 def test_multi_model_batch_matches_single_and_host(ev):
     """parity_test.cpp:225-255"""
     rng = np.random.default_rng(100)

@@ -201,10 +201,10 @@ class ModelEvaluator:
         return fit, pl
 
     def fetch_counters(self):
-        """per-item (candidates fitted, candidates legal) of the last launch"""
-        out = np.zeros(2 * self._items, dtype=np.uint64)
+        """per-item (candidates fitted, candidates legal, candidate lanes issued) of the last launch"""
+        out = np.zeros(3 * self._items, dtype=np.uint64)
         self._check(lib().ta3h_fetch_counters(self._h, _ptr(out, C.c_uint64)))
-        return out.reshape(-1, 2)
+        return out.reshape(-1, 3)
 
     def set_timeline(self, enabled: bool = True):
         self._check(lib().ta3h_set_timeline(self._h, int(enabled)))

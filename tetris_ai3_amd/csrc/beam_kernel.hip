@@ -171,7 +171,7 @@ struct EngineState {
 };
 
 constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
-static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 4 <= PARK_WORDS && ENGINE_WORDS <= 64,
+static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 6 <= PARK_WORDS && ENGINE_WORDS <= 64,
               "a parked game fits its record and one wave copies it");
 
 struct alignas(16) GameLds {
@@ -1280,7 +1280,7 @@ enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
 // pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
 template<int KIND, int G, int MODE>
 __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, WideArgs const& wide, uint32_t wv,
-                          uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal) {
+                          uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // gather the committed state (beam.hpp:503-505)
@@ -1303,6 +1303,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     nApply += total;
     uint64_t mask = 0;
     uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
+    nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
@@ -1335,6 +1336,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
         mask = 0;
         top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
                          : run_stage<KIND, STAGE_MID, K2, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
@@ -1394,6 +1396,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     } else {
         leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
     }
+    nSlots += (evalTotal + 63u) & ~63u;
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(MODE == MODE_TRACE) {
         if(traceMove) emit_stage<G>(a, L, P, total, 3u, wv);
@@ -1487,13 +1490,14 @@ void beam_eval_kernel(EvalArgs a) {
             wide.finite = __ballot(inf) == 0;
         }
         uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
-        uint64_t nApply = 0;
+        uint64_t nApply = 0, nSlots = 0;
         uint32_t nLegal = 0, mv = 0;
         if(a.resumePhase) { // a parked game: its engine state, counters and move index
             if(wv == 0 && lane < ENGINE_WORDS) reinterpret_cast<uint32_t*>(&L.eng)[lane] = park[lane];
             nApply = uniform(static_cast<uint64_t>(park[ENGINE_WORDS]) | static_cast<uint64_t>(park[ENGINE_WORDS + 1]) << 32);
             nLegal = wv == 0 ? uniform(park[ENGINE_WORDS + 2]) : 0u;
             mv = uniform(park[ENGINE_WORDS + 3]);
+            nSlots = uniform(static_cast<uint64_t>(park[ENGINE_WORDS + 4]) | static_cast<uint64_t>(park[ENGINE_WORDS + 5]) << 32);
         } else if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
             engine_reset(L.eng, a.seeds[g]);
@@ -1510,7 +1514,7 @@ void beam_eval_kernel(EvalArgs a) {
             if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col,
                                                                          MODE == MODE_TRACE && a.traceEvery != 0
                                                                              && mv % a.traceEvery == tracePhase,
-                                                                         nApply, nLegal)) {
+                                                                         nApply, nLegal, nSlots)) {
                 ended = true;
                 break;
             }
@@ -1527,6 +1531,8 @@ void beam_eval_kernel(EvalArgs a) {
                 out[ENGINE_WORDS + 1] = static_cast<uint32_t>(nApply >> 32);
                 out[ENGINE_WORDS + 2] = legal;
                 out[ENGINE_WORDS + 3] = mv;
+                out[ENGINE_WORDS + 4] = static_cast<uint32_t>(nSlots);
+                out[ENGINE_WORDS + 5] = static_cast<uint32_t>(nSlots >> 32);
                 a.resumeList[atomicAdd(a.resumeCount, 1u)] = item;
             }
         } else if(threadIdx.x == 0) {
@@ -1535,8 +1541,9 @@ void beam_eval_kernel(EvalArgs a) {
             if(a.counters) {
                 uint64_t legal = 0;
                 for(int k = 0; k < G; ++k) legal += L.partLegal[k];
-                a.counters[2 * static_cast<uint64_t>(item)] = nApply;
-                a.counters[2 * static_cast<uint64_t>(item) + 1] = legal;
+                a.counters[3 * static_cast<uint64_t>(item)] = nApply;
+                a.counters[3 * static_cast<uint64_t>(item) + 1] = legal;
+                a.counters[3 * static_cast<uint64_t>(item) + 2] = nSlots;
             }
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item) + 1] = wall_clock64();
         }

@@ -30,7 +30,7 @@ struct EvalArgs {
     uint64_t const* seeds;    // one per game
     float* fitness;           // [items], model-major (item = m * games + g)
     uint32_t* placements;     // [items] engine.place calls per item (may be null)
-    uint64_t* counters;       // [items][2]: candidates fitted, candidates legal (may be null)
+    uint64_t* counters;       // [items][3]: candidates fitted, candidates legal, lanes issued (may be null)
     uint64_t* timeline;       // [items][2]: device wall clock at item start / end (may be null)
     uint32_t* queue;          // work-queue counter (persistent mode)
     uint32_t items;

@@ -343,7 +343,7 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
     if((r = ev->grow(ev->dSeeds, ev->seedsCap, games, "malloc seeds"))) return r;
     if((r = ev->grow(ev->dFitness, ev->fitnessCap, items, "malloc fitness"))) return r;
     if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
-    if((r = ev->grow(ev->dCounters, ev->countersCap, 2 * items, "malloc counters"))) return r;
+    if((r = ev->grow(ev->dCounters, ev->countersCap, 3 * items, "malloc counters"))) return r;
     if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
     ev->packed.resize(wCount);
     for(uint32_t m = 0; m < num_models; ++m)
@@ -478,7 +478,7 @@ int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters) {
         return TA3H_E_STATE;
     }
     if(ev->items == 0) return TA3H_OK;
-    hipError_t e = hipMemcpyAsync(out_counters, ev->dCounters, 2 * static_cast<size_t>(ev->items) * sizeof(uint64_t),
+    hipError_t e = hipMemcpyAsync(out_counters, ev->dCounters, 3 * static_cast<size_t>(ev->items) * sizeof(uint64_t),
                                   hipMemcpyDeviceToHost, ev->stream);
     if(e != hipSuccess) return ev->fail("copy counters", e);
     e = hipStreamSynchronize(ev->stream);
@@ -616,7 +616,7 @@ int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_par
     if((r = ev->grow(ev->dSeeds, ev->seedsCap, games, "malloc seeds"))) return r;
     if((r = ev->grow(ev->dFitness, ev->fitnessCap, items, "malloc fitness"))) return r;
     if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
-    if((r = ev->grow(ev->dCounters, ev->countersCap, 2 * items, "malloc counters"))) return r;
+    if((r = ev->grow(ev->dCounters, ev->countersCap, 3 * items, "malloc counters"))) return r;
     if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
     if((r = ev->grow_pinned(ev->hWeights, ev->hWeightsCap, wCount, "pinned weights"))) return r;
     if((r = ev->grow_pinned(ev->hFitness, ev->hFitnessCap, items, "pinned fitness"))) return r;
