@@ -23,7 +23,9 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "kernels.hpp"
 #include "tables.hpp"
@@ -1796,7 +1798,17 @@ hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksP
     }
 }
 
+// Once per process and device (later calls return at once), so that an evaluator created while another one's
+// kernels run on the same device never rewrites a table those kernels read.
 hipError_t upload_constants() {
+    static std::mutex mu;
+    static std::vector<int> done; // devices whose table is in place
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if(e != hipSuccess) return e;
+    std::lock_guard<std::mutex> const lock(mu);
+    for(int d : done)
+        if(d == dev) return hipSuccess;
     // fsqrt (v_board_metrics.hpp:23-37) then the /100 normaliser (:229), both IEEE on the host
     float lut[SV_MAX + 1];
     for(int sv = 0; sv <= SV_MAX; ++sv) {
@@ -1804,7 +1816,9 @@ hipError_t upload_constants() {
         float const r = x <= 0.0f ? 0.0f : std::sqrt(x);
         lut[sv] = r / 100.0f;
     }
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_svNorm), lut, sizeof(lut), 0, hipMemcpyHostToDevice);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_svNorm), lut, sizeof(lut), 0, hipMemcpyHostToDevice);
+    if(e == hipSuccess) done.push_back(dev);
+    return e;
 }
 
 hipError_t launch_bits_canary(uint32_t* ok, hipStream_t stream) {

@@ -66,7 +66,8 @@ struct EvalArgs {
 };
 
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
-// uploads the constant tables computed on the host; call once per device before the first launch
+// uploads the constant tables computed on the host before the first launch on the current device (once per
+// process and device; later calls return at once)
 hipError_t upload_constants();
 // leaf values of n boards (10 words each) as the search scores candidate boards; weights in device layout
 hipError_t launch_model_values(int kind, float const* weights, uint32_t const* boards, uint32_t const* clears,

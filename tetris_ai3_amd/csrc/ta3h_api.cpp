@@ -201,7 +201,7 @@ struct ta3h_evaluator {
         if(e != hipSuccess) return fail("create event", e);
         e = hipMalloc(&dQueue, 64);
         if(e != hipSuccess) return fail("malloc queue", e);
-        e = ta3h::upload_constants(); // per process and device; idempotent
+        e = ta3h::upload_constants(); // once per process and device
         if(e != hipSuccess) return fail("upload constants", e);
         ready = true;
         return TA3H_OK;
