@@ -619,6 +619,11 @@ inline float bf16_rne(float v) {
 // accumulation may differ in the last bits); layer 1 output rounded to bfloat16 after ReLU; the output layer in
 // FP32: p_q = FMA chain over o = 16 t + 4 q + j (t = 0..7, then j = 0..3) of h2[o] W3[o], value =
 // ((p0 + p1) + (p2 + p3)) + b3.
+// The device's relu of the wide network: the signed maximum of the value's bits and 0, i.e. +0 for a value with
+// the sign bit set and the value itself otherwise. Equal to (0 < v) ? v : +0 for every v that is not NaN (a NaN
+// arises only from non-finite or enormous weights; the bfloat16 rounding of a NaN is not restated).
+inline float relu_bits(float v) { return std::signbit(v) ? 0.0f : v; }
+
 float value_f32w(float const* w, float const* in) {
     float x[32], h1[128], h2[128];
     for(int i = 0; i < 32; ++i) x[i] = bf16_rne(in[i]);
@@ -626,13 +631,13 @@ float value_f32w(float const* w, float const* in) {
         double v = w[4096 + u];
         for(int i = 0; i < 32; ++i) v += static_cast<double>(x[i]) * bf16_rne(w[u * 32 + i]);
         float const f = static_cast<float>(v);
-        h1[u] = bf16_rne(f > 0.0f ? f : 0.0f);
+        h1[u] = relu_bits(bf16_rne(f)); // the device rounds to bfloat16, then takes relu
     }
     for(int o = 0; o < 128; ++o) {
         double v = w[4224 + 16384 + o];
         for(int u = 0; u < 128; ++u) v += static_cast<double>(h1[u]) * bf16_rne(w[4224 + o * 128 + u]);
         float const f = static_cast<float>(v);
-        h2[o] = f > 0.0f ? f : 0.0f;
+        h2[o] = relu_bits(f);
     }
     float const* w3 = w + 4224 + 16512;
     float p[4];

@@ -386,6 +386,8 @@ __device__ float final_score(EngineState const& e) { // score_v1.hpp:20-34; mode
 // candidate evaluation: apply (search.hpp:294-302) + features + value network
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -407,6 +409,15 @@ __device__ inline T* opaque_ptr(T* p) {
     uint64_t v = reinterpret_cast<uint64_t>(p);
     asm volatile("" : "+s"(v));
     return reinterpret_cast<T*>(v);
+}
+
+// the same for a pointer into device global memory, kept in the global address space so that its loads are
+// global loads (counted by vmcnt only) rather than flat loads, whose waits also drain the wave's LDS accesses
+template<typename T>
+__device__ inline T __attribute__((address_space(1)))* opaque_global(T* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+s"(v));
+    return reinterpret_cast<T __attribute__((address_space(1)))*>(v);
 }
 
 __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::max<float>(0, v)
@@ -633,6 +644,22 @@ __device__ inline void store_features16(uint16_t* featTile, float const (&f)[16]
 // faster than one). Per column tile: layer 1 = 8 MFMAs, ReLU, bfloat16; layers 2 and 3 = 32 MFMAs, ReLU, then
 // the FP32 output FMA chain of lane quarter q over o = 16 t + 4 q + j; value = ((p0 + p1) + (p2 + p3)) + b3.
 // Every tile's operations and their order do not depend on CB.
+// relu of two values already rounded to bfloat16: the signed 16-bit maximum of their bits and 0 (one v_pk_max_i16)
+// equals bfloat16(relu(x)) for every x that is not NaN (negative values and -0 have the sign bit set). A NaN
+// (only from non-finite or enormous weights) with the sign bit clear stays NaN.
+__device__ inline bf16x2 relu_bf16x2(float a, float b) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    bf16x2 const v = __builtin_convertvector(f32x2{a, b}, bf16x2);
+    i16x2 const m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), i16x2{0, 0});
+    return __builtin_bit_cast(bf16x2, m);
+}
+
+// relu of an FP32 value as the signed maximum of its bits and 0: (0 < x) ? x : +0 for every x that is not NaN
+__device__ inline float relu_bits(float x) {
+    int const b = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
+
 template<int CB>
 __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave) {
     static_assert(CB == 1 || CB == 2 || CB == 4);
@@ -646,22 +673,21 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
     float mine = 0.0f;
 #pragma unroll 1
     for(uint32_t cb = 0; cb < 4; cb += CB) {
-        bf16x8 h[CB][4]; // layer-2 B operands of each tile: K block kb = layer-1 tiles 2 kb, 2 kb + 1
+        u32x4 h[CB][4]; // layer-2 B operands of each tile (bfloat16 pairs): K block kb = layer-1 tiles 2 kb, 2 kb + 1
         {
             bf16x8 x[CB];
 #pragma unroll
             for(int c = 0; c < CB; ++c) x[c] = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * (cb + c) + r16, q)]);
 #pragma unroll
             for(uint32_t t = 0; t < 8; ++t) {
-                bf16x8 const* const w1l = opaque_ptr(w1f); // one tile's fragment in flight at a time
+                auto const w1l = opaque_global(w1f); // one tile's fragment in flight at a time
                 bf16x8 const a = w1l[(t * 16 + r16) * 4 + q];
                 f32x4 const bias = b1[4 * t + q]; // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
 #pragma unroll
                 for(int c = 0; c < CB; ++c) {
                     f32x4 const acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c], bias, 0, 0, 0);
-#pragma unroll
-                    for(int j = 0; j < 4; ++j)
-                        h[c][t >> 1][4 * (t & 1u) + j] = static_cast<__bf16>(__builtin_fmaxf(acc[j], 0.0f));
+                    h[c][t >> 1][2 * (t & 1u)] = __builtin_bit_cast(uint32_t, relu_bf16x2(acc[0], acc[1]));
+                    h[c][t >> 1][2 * (t & 1u) + 1] = __builtin_bit_cast(uint32_t, relu_bf16x2(acc[2], acc[3]));
                 }
             }
         }
@@ -678,13 +704,14 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
             for(uint32_t kb = 0; kb < 4; ++kb) {
                 bf16x8 const a = w2f[((t * 4 + kb) * 16 + r16) * 4 + q];
 #pragma unroll
-                for(int c = 0; c < CB; ++c) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[c][kb], acc[c], 0, 0, 0);
+                for(int c = 0; c < CB; ++c)
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][kb]), acc[c], 0, 0, 0);
             }
             f32x4 const w = w3[4 * t + q];
 #pragma unroll
             for(int c = 0; c < CB; ++c)
 #pragma unroll
-                for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(__builtin_fmaxf(acc[c][j], 0.0f), w[j], part[c]);
+                for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(relu_bits(acc[c][j]), w[j], part[c]);
         }
 #pragma unroll
         for(int c = 0; c < CB; ++c) {
