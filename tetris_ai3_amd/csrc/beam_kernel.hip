@@ -424,8 +424,9 @@ __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::
 __device__ inline float mad_rn(float acc, float a, float b) { return __fadd_rn(acc, __fmul_rn(a, b)); }
 __device__ inline float fma_rn(float acc, float a, float b) { return __fmaf_rn(a, b, acc); }
 
-// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347). Its network is too small
-// for the zero skipping of the 16-feature network to pay (measured 4% slower per generation).
+// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347). The zero skipping of the
+// 16-feature network does not pay here: with three variants the V11 generation ran 4% slower, with one ballot
+// and two copies (skipping the multi-line clear counts) 5% slower.
 __device__ inline float value_v11(wptr w, float const (&f)[16]) {
     float out = w[96];
 #pragma unroll
@@ -1043,7 +1044,7 @@ __device__ inline void pruner_layers23(PrunerDev const* __restrict__ pr, bf16x8 
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, h1a[c], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, h1b[c], acc, 0, 0, 0);
 #pragma unroll
-            for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(__builtin_fmaxf(acc[j], 0.0f), w3[j], part[c]);
+            for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(relu_bits(acc[j]), w3[j], part[c]); // acc: integers < 2^24
         }
         a0 = na0;
         a1 = na1;
