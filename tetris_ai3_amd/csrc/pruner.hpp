@@ -10,6 +10,7 @@ namespace ta3h {
 inline constexpr int PR_IN = 265, PR_H = 64;
 inline constexpr int PR_W1 = 0, PR_B1 = PR_W1 + PR_H * PR_IN, PR_W2 = PR_B1 + PR_H, PR_B2 = PR_W2 + PR_H * PR_H,
                      PR_W3 = PR_B2 + PR_H, PR_B3 = PR_W3 + PR_H, PR_PARAMS = PR_B3 + 1;
+inline constexpr int PR_SPLIT = 32; // sample ranges of a training batch's gradient sum (partial sums, PR_SPLIT x PR_PARAMS)
 
 // Inference form of a pruner: layers 1 and 2 are an integer network on power-of-two grids, so every sum the
 // BF16 MFMAs form is an exact integer below 2^24 and their FP32 accumulation is exact in any order; layer 3 is
@@ -47,7 +48,8 @@ void build_pruner_tables(float const* w, float actRange, PrunerDev& out, int* ex
 // training (pruner_train.hip)
 hipError_t pruner_fwd_bwd(float const* w, uint32_t const* samples, uint32_t const* order, uint32_t n, float invBatch,
                           float posWeight, float* act, float* delta, float* dscore, float* loss, hipStream_t s);
-hipError_t pruner_grad_adam(float* w, float* m, float* v, float* gradOut, uint32_t const* samples,
+// partial: PR_SPLIT x PR_PARAMS floats of scratch
+hipError_t pruner_grad_adam(float* w, float* m, float* v, float* gradOut, float* partial, uint32_t const* samples,
                             uint32_t const* order, uint32_t n, float const* act, float const* delta,
                             float const* dscore, int adam, float lr, float beta1, float beta2, float eps, float corr1,
                             float corr2, hipStream_t s);

@@ -145,6 +145,8 @@ struct ta3h_evaluator {
     size_t deltaCap = 0;
     float* dScore = nullptr;
     size_t scoreCap = 0;
+    float* dPartial = nullptr; // gradient partial sums, PR_SPLIT x PR_PARAMS
+    size_t partialCap = 0;
     float* dLoss = nullptr;
     size_t lossCap = 0;
     float* dOut = nullptr;
@@ -260,7 +262,8 @@ struct ta3h_evaluator {
         for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
                        static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
                        static_cast<void*>(dOrder), static_cast<void*>(dAct), static_cast<void*>(dDelta),
-                       static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut)})
+                       static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut),
+                       static_cast<void*>(dPartial)})
             if(p) static_cast<void>(hipFree(p));
         if(evStart) static_cast<void>(hipEventDestroy(evStart));
         if(evStop) static_cast<void>(hipEventDestroy(evStop));
@@ -799,6 +802,9 @@ int grow_batch(ta3h_evaluator* ev, size_t b) {
     if((r = ev->grow(ev->dAct, ev->actCap, b * 4 * ta3h::PR_H, "malloc activations"))) return r;
     if((r = ev->grow(ev->dDelta, ev->deltaCap, b * 2 * ta3h::PR_H, "malloc deltas"))) return r;
     if((r = ev->grow(ev->dScore, ev->scoreCap, b, "malloc dscore"))) return r;
+    if((r = ev->grow(ev->dPartial, ev->partialCap, static_cast<size_t>(ta3h::PR_SPLIT) * ta3h::PR_PARAMS,
+                     "malloc gradient partials")))
+        return r;
     return TA3H_OK;
 }
 
@@ -816,8 +822,8 @@ int ta3h_pruner_grad(ta3h_evaluator* ev, const float* w, const uint32_t* samples
     hipError_t e = ta3h::pruner_fwd_bwd(ev->dPw, dSmp, nullptr, n, 1.0f / static_cast<float>(n), pos_weight, ev->dAct,
                                         ev->dDelta, ev->dScore, ev->dLoss, ev->stream);
     if(e != hipSuccess) return ev->fail("pruner forward/backward", e);
-    e = ta3h::pruner_grad_adam(ev->dPw, nullptr, nullptr, ev->dOut, dSmp, nullptr, n, ev->dAct, ev->dDelta, ev->dScore,
-                               0, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, ev->stream);
+    e = ta3h::pruner_grad_adam(ev->dPw, nullptr, nullptr, ev->dOut, ev->dPartial, dSmp, nullptr, n, ev->dAct,
+                               ev->dDelta, ev->dScore, 0, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, ev->stream);
     if(e != hipSuccess) return ev->fail("pruner gradient", e);
     e = hipMemcpyAsync(grad, ev->dOut, ta3h::PR_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ev->stream);
     if(e != hipSuccess) return ev->fail("copy gradient", e);
@@ -874,8 +880,8 @@ int ta3h_pruner_train(ta3h_evaluator* ev, float* w, const uint32_t* samples, uin
             if(e != hipSuccess) return ev->fail("pruner forward/backward", e);
             float const c1 = 1.0f - static_cast<float>(std::pow(0.9, static_cast<double>(step)));
             float const c2 = 1.0f - static_cast<float>(std::pow(0.999, static_cast<double>(step)));
-            e = ta3h::pruner_grad_adam(ev->dPw, m, v, nullptr, dSmp, ev->dOrder + b0, nb, ev->dAct, ev->dDelta,
-                                       ev->dScore, 1, lr, beta1, beta2, eps, c1, c2, ev->stream);
+            e = ta3h::pruner_grad_adam(ev->dPw, m, v, nullptr, ev->dPartial, dSmp, ev->dOrder + b0, nb, ev->dAct,
+                                       ev->dDelta, ev->dScore, 1, lr, beta1, beta2, eps, c1, c2, ev->stream);
             if(e != hipSuccess) return ev->fail("pruner adam", e);
         }
         // the next epoch's order upload must not overtake this epoch's kernels
