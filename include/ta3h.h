@@ -34,6 +34,9 @@ enum {
     TA3H_MODEL_F32W = 3 /* 32 features -> 128 -> 128 -> 1, 20865 params (BF16 MFMA, FP32 accumulate) */
 };
 
+/* ta3h_set_phases: first-phase budget chosen from max_moves (the default) */
+#define TA3H_PHASES_AUTO 0xFFFFFFFFu
+
 /* Replaces ModelEvaluator::ModelEvaluator() (model_eval.hpp:29; model_eval.cu:92). Allocates nothing
  * on the device until the first evaluation (grow-only buffers, model_eval.cu:54-70). */
 int ta3h_create(ta3h_evaluator** out);
@@ -55,8 +58,9 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind);
  * the first phase plays every game for at most first_phase_moves moves and parks the unfinished ones on the
  * device (engine state, counters, move index); the second phase plays only those on to max_moves, so the long
  * games all start together instead of trickling out of the work queue; the second phase takes the games of the
- * models with the most parked games first. 0 = one phase; the default is 128 (a full F16 generation of
- * 200 models x 50 games runs about 8% faster than in one phase, V11 about 4%). */
+ * models with the most parked games first. 0 = one phase. TA3H_PHASES_AUTO (the default) = max_moves / 16
+ * clamped to [128, 512]: 128 at a 2,000-piece cap (a full F16 generation of 200 models x 50 games runs about 8%
+ * faster than in one phase, V11 about 4%), 312 at 5,000 (2-3% faster than 128 on the wide workload). */
 int ta3h_set_phases(ta3h_evaluator* ev, uint32_t first_phase_moves);
 
 /* Start every game of the following launches with 0-4 garbage rows at the bottom, each full but for one hole,

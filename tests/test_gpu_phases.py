@@ -1,6 +1,8 @@
 """GPU: the two-phase schedule (every game to a move budget, then only the parked survivors) changes nothing
 but the order of work: fitness, placements, work counters and trace records equal the one-phase launch's,
 which equal the oracle's."""
+import json
+import os
 import sys
 
 import numpy as np
@@ -35,7 +37,7 @@ def test_two_phases_equal_one_phase(kind, wpg):
     ev.set_launch(wpg, True, 0)
     ev.set_phases(0)
     one = run(ev, w, nm, seeds, 120)
-    for budget in (1, 7, 40, 119, 120, 500):
+    for budget in (1, 7, 40, 119, 120, 500, None):
         ev.set_phases(budget)
         two = run(ev, w, nm, seeds, 120)
         assert one[0].tobytes() == two[0].tobytes(), budget
@@ -98,3 +100,23 @@ def test_two_phase_pruned_and_autotune():
     fit, pl = run(ev, w, nm, seeds, 90)[:2]
     assert fit.tobytes() == host.tobytes() and (pl == host_pl).all()
     ev.close()
+
+
+def test_auto_budget_parks_long_games():
+    """the default first-phase budget (max_moves / 16 clamped to [128, 512]: 131 at a 2,100-piece cap) with games
+    that outlive it: the reference's V11 champion, whose games run to the cap"""
+    say("case: auto budget, V11 champion, 4 games x 2100 moves")
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "v11_champion.json")) as f:
+        w = np.array(json.load(f)["weights_f64"], dtype=np.float64).astype(np.float32)
+    seeds = O.expand_seeds(57, 4)
+    ev = T.ModelEvaluator(model=O.MODEL_V11)
+    ev.set_phases(0)
+    one = run(ev, w, 1, seeds, 2100)
+    ev.set_phases(None)
+    auto = run(ev, w, 1, seeds, 2100)
+    ev.close()
+    assert (one[1] > 131).all()  # every game outlived the first phase
+    assert one[0].tobytes() == auto[0].tobytes() and (one[1] == auto[1]).all() and (one[2] == auto[2]).all()
+    host, host_pl = O.eval_host_mt(O.MODEL_V11, w, 1, seeds, 2100)
+    assert auto[0].tobytes() == host.tobytes() and (auto[1] == host_pl).all()

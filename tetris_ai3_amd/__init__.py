@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libta3h.so")
 MODEL_V11 = 0
 MODEL_F16 = 1
 MODEL_F32W = 3
+PHASES_AUTO = 0xFFFFFFFF  # set_phases(None): first-phase budget from max_moves (the default)
 
 _lib = None
 
@@ -145,10 +146,11 @@ class ModelEvaluator:
     def num_params(self) -> int:
         return num_params(self.model)
 
-    def set_phases(self, first_phase_moves: int):
+    def set_phases(self, first_phase_moves: int | None):
         """two-phase schedule: every game to first_phase_moves, then only the survivors, the games of the models
-        with the most survivors first (0 = one phase; default 128). Results never change."""
-        self._check(lib().ta3h_set_phases(self._h, int(first_phase_moves)))
+        with the most survivors first (0 = one phase; None = max_moves / 16 clamped to [128, 512], the default).
+        Results never change."""
+        self._check(lib().ta3h_set_phases(self._h, PHASES_AUTO if first_phase_moves is None else int(first_phase_moves)))
 
     def set_garbage(self, enabled: bool = True):
         """start every game of the following launches with 0-4 seeded garbage rows"""
@@ -439,7 +441,7 @@ DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4, MODEL_F32W: 4}  # V11 at 5
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
-             candidates=((1, 0), (2, 0), (4, 0)), phases=(128,)):
+             candidates=((1, 0), (2, 0), (4, 0)), phases=(None,)):
     """Times one launch per launch shape (waves per game, grid) and first-phase budget on the given batch and
     keeps the fastest on `evaluator`. Neither changes results; returns {(wpg, grid, phase): kernel_ms}."""
     timings = {}

@@ -114,7 +114,7 @@ struct ta3h_evaluator {
     bool timelineOn = false;
     uint32_t* dQueue = nullptr;
     // two-phase schedule
-    uint32_t phaseMoves = 128; // first-phase move budget of the two-phase schedule; 0 = one phase
+    uint32_t phaseMoves = TA3H_PHASES_AUTO; // first-phase move budget of the two-phase schedule; 0 = one phase
     uint32_t* dParked = nullptr;
     size_t parkedCap = 0;
     uint32_t* dResume = nullptr; // parked list [items], its counter, resume order [items], per-model counts
@@ -379,13 +379,16 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
     a.garbage = static_cast<uint32_t>(ev->garbage);
-    if(ev->phaseMoves != 0 && ev->persistent && ev->phaseMoves < ev->maxMoves) {
+    uint32_t const phase = ev->phaseMoves != TA3H_PHASES_AUTO
+                               ? ev->phaseMoves
+                               : (ev->maxMoves / 16 < 128 ? 128u : (ev->maxMoves / 16 > 512 ? 512u : ev->maxMoves / 16));
+    if(phase != 0 && ev->persistent && phase < ev->maxMoves) {
         int r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked");
         if(r) return r;
         // list [items], counter, order [items], per-model counts [items / games]
         size_t const words = 2 * static_cast<size_t>(ev->items) + 1 + ev->items / ev->games;
         if((r = ev->grow(ev->dResume, ev->resumeCap, words, "malloc resume list"))) return r;
-        a.phaseMoves = ev->phaseMoves;
+        a.phaseMoves = phase;
         a.parked = ev->dParked;
         a.resumeList = ev->dResume;
         a.resumeCount = ev->dResume + ev->items;
