@@ -196,6 +196,14 @@ int ta3h_trainer_champion(const ta3h_trainer* t, double* out, uint64_t cap, doub
 /* Current search mean (num_params doubles) and step size of one island. */
 int ta3h_trainer_island_state(const ta3h_trainer* t, uint32_t island, double* mean, double* sigma);
 
+/* Checkpoint of the whole trainer between iterations (replaces Trainer2::saveState's Boost/zstd archive of the
+ * optimiser state, trainer2.cpp:213-249, with this library's own versioned binary format): config, iteration
+ * count, seed generator, champion and every island's CMA-ES state. A trainer loaded from it (a new handle,
+ * destroyed with ta3h_trainer_destroy) continues exactly as the saved one would have. Load fails with
+ * TA3H_E_ARG on a missing, truncated, oversized or foreign file. */
+int ta3h_trainer_save(const ta3h_trainer* t, const char* path);
+int ta3h_trainer_load(const char* path, ta3h_trainer** out);
+
 /* .champ files (trainer2.cpp:251-265): u64 count, then count little-endian doubles. Read with w = NULL to
  * learn the count. */
 int ta3h_champ_write(const char* path, const double* w, uint64_t count);

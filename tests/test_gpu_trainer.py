@@ -38,3 +38,19 @@ def test_device_training_equals_cpu_training(kind):
         mc, sc = cpu.island_state(i)
         assert sg == sc and (mg == mc).all()
     ev.close()
+
+
+def test_training_command_resumes_from_its_checkpoint(tmp_path, capsys):
+    """python -m tetris_ai3_amd.train: one iteration, then --resume from the .ckpt it wrote for one more, gives
+    the champion of an uninterrupted two-iteration run"""
+    from tetris_ai3_amd import train
+    say("case: training command, checkpoint and resume")
+    common = ["--kind", "v11", "--islands", "2", "--population", "6", "--games", "3", "--max-moves", "50",
+              "--generations", "2", "--seed", "33"]
+    whole, part = str(tmp_path / "whole"), str(tmp_path / "part")
+    assert train.main(common + ["--iterations", "2", "--out", whole]) == 0
+    assert train.main(common + ["--iterations", "1", "--out", part]) == 0
+    assert train.main(["--resume", part + ".ckpt", "--iterations", "1", "--out", part]) == 0
+    capsys.readouterr()
+    assert open(whole + ".champ", "rb").read() == open(part + ".champ", "rb").read()
+    assert open(whole + ".ckpt", "rb").read() == open(part + ".ckpt", "rb").read()

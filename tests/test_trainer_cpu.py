@@ -98,3 +98,44 @@ def test_reference_champion_file_reads_as_the_golden_weights(tmp_path):
     out = str(tmp_path / "copy.champ")
     T.champ_write(out, w)
     assert open(out, "rb").read() == open(REF_CHAMP, "rb").read()
+
+
+def test_checkpoint_resumes_identically(tmp_path):
+    """a trainer saved between iterations and loaded again continues exactly as the uninterrupted one (the
+    reference checks that a serialized optimiser resumes identically, dd_cmaes_test.cpp:82-104)"""
+    ev = oracle_eval(O.MODEL_V11)
+    t = small_trainer(11)
+    for _ in range(2):
+        t.run_iteration(evaluate=ev)
+    path = str(tmp_path / "trainer.ckpt")
+    t.save(path)
+    cont = [t.run_iteration(evaluate=ev) for _ in range(2)]
+    w_cont, f_cont = t.champion()
+    u = T.Trainer.load(path)
+    assert u.num_params == 97
+    res = [u.run_iteration(evaluate=ev) for _ in range(2)]
+    w_res, f_res = u.champion()
+    assert res == cont and f_res == f_cont and (w_res == w_cont).all()
+    for i in range(2):
+        (ma, sa), (mb, sb) = t.island_state(i), u.island_state(i)
+        assert sa == sb and (ma == mb).all()
+    t.close()
+    u.close()
+
+
+def test_checkpoint_rejects_damaged_files(tmp_path):
+    t = small_trainer(12)
+    t.run_iteration(evaluate=oracle_eval(O.MODEL_V11))
+    path = str(tmp_path / "trainer.ckpt")
+    t.save(path)
+    raw = open(path, "rb").read()
+    for name, data in (("truncated", raw[:-5]), ("trailing", raw + b"\0"), ("magic", b"X" + raw[1:]),
+                       ("empty", b"")):
+        bad = str(tmp_path / f"{name}.ckpt")
+        with open(bad, "wb") as f:
+            f.write(data)
+        with pytest.raises(T.HipError, match="cannot load"):
+            T.Trainer.load(bad)
+    with pytest.raises(T.HipError, match="cannot load"):
+        T.Trainer.load(str(tmp_path / "missing.ckpt"))
+    assert T.Trainer.load(path).num_params == 97

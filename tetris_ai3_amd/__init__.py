@@ -83,6 +83,8 @@ def lib():
             "ta3h_trainer_last_error": (C.c_char_p, [vp]),
             "ta3h_trainer_run_iteration": (C.c_int, [vp, vp, EVAL_CALLBACK, vp, C.POINTER(IterationReport)]),
             "ta3h_trainer_champion": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double)]),
+            "ta3h_trainer_save": (C.c_int, [vp, C.c_char_p]),
+            "ta3h_trainer_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
             "ta3h_trainer_island_state": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "ta3h_champ_write": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_uint64]),
             "ta3h_champ_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -333,7 +335,11 @@ class Trainer:
         self._h = C.c_void_p()
         if lib().ta3h_trainer_create(C.byref(self.cfg), C.byref(self._h)) != 0:
             raise HipError("trainer: invalid configuration")
-        self.num_params = num_params(kind)
+        self._set_shape(kind, islands, population, games, generations)
+
+    def _set_shape(self, kind, islands, population, games, generations):
+        self.kind, self.num_params = kind, num_params(kind)
+        self.cfg_islands, self.cfg_population, self.cfg_games, self.cfg_generations = islands, population, games, generations
 
     def close(self):
         if self._h:
@@ -379,6 +385,26 @@ class Trainer:
         if lib().ta3h_trainer_island_state(self._h, island, _ptr(mean, C.c_double), C.byref(sigma)) != 0:
             raise HipError("trainer: no such island")
         return mean, sigma.value
+
+    def save(self, path: str) -> None:
+        """checkpoint between iterations (config, seeds, champion, every island's CMA-ES state)"""
+        if lib().ta3h_trainer_save(self._h, os.fsencode(path)) != 0:
+            raise HipError(f"trainer: cannot write {path}")
+
+    @classmethod
+    def load(cls, path: str) -> "Trainer":
+        """a trainer that continues exactly where the checkpoint was saved"""
+        h = C.c_void_p()
+        if lib().ta3h_trainer_load(os.fsencode(path), C.byref(h)) != 0:
+            raise HipError(f"trainer: cannot load {path}")
+        t = cls.__new__(cls)
+        t._h = h
+        t.cfg = None
+        with open(path, "rb") as f:  # the config follows the magic and the version: i32 kind, then 5 x u32
+            head = f.read(36)
+        kind, islands, population, games, _, generations = np.frombuffer(head[12:36], dtype="<i4")
+        t._set_shape(int(kind), int(islands), int(population), int(games), int(generations))
+        return t
 
 
 def champ_write(path: str, w) -> None:

@@ -218,6 +218,57 @@ int generation(ta3h_trainer* t, ta3h_evaluator* ev, ta3h_eval_callback cb, void*
 
 } // namespace
 
+namespace {
+
+constexpr char CKPT_MAGIC[8] = {'T', 'A', '3', 'H', 'T', 'R', 'N', '1'};
+
+struct Writer {
+    FILE* f;
+    bool ok = true;
+    template<class T>
+    void put(T const& v) { ok = ok && std::fwrite(&v, sizeof(T), 1, f) == 1; }
+    void doubles(std::vector<double> const& v) {
+        put(static_cast<uint64_t>(v.size()));
+        ok = ok && (v.empty() || std::fwrite(v.data(), sizeof(double), v.size(), f) == v.size());
+    }
+    void rng(Rng const& r) {
+        for(uint64_t w : r.s) put(w);
+        put(static_cast<uint8_t>(r.have ? 1 : 0));
+        put(r.spare);
+    }
+};
+
+struct Reader {
+    FILE* f;
+    bool ok = true;
+    template<class T>
+    T get() {
+        T v{};
+        ok = ok && std::fread(&v, sizeof(T), 1, f) == 1;
+        return v;
+    }
+    void doubles(std::vector<double>& v, uint64_t maxCount) {
+        uint64_t const n = get<uint64_t>();
+        if(!ok || n > maxCount) {
+            ok = false;
+            return;
+        }
+        v.resize(n);
+        ok = ok && (n == 0 || std::fread(v.data(), sizeof(double), n, f) == n);
+    }
+    void exact(std::vector<double>& v, uint64_t count) {
+        v.resize(count);
+        ok = ok && (count == 0 || std::fread(v.data(), sizeof(double), count, f) == count);
+    }
+    void rng(Rng& r) {
+        for(uint64_t& w : r.s) w = get<uint64_t>();
+        r.have = get<uint8_t>() != 0;
+        r.spare = get<double>();
+    }
+};
+
+} // namespace
+
 extern "C" {
 
 int ta3h_trainer_create(const ta3h_train_config* cfg, ta3h_trainer** out) {
@@ -318,6 +369,103 @@ int ta3h_trainer_island_state(const ta3h_trainer* t, uint32_t island, double* me
     if(!t || island >= t->islands.size()) return TA3H_E_ARG;
     if(mean) std::copy(t->islands[island].mean.begin(), t->islands[island].mean.end(), mean);
     if(sigma) *sigma = t->islands[island].sigma;
+    return TA3H_OK;
+}
+
+// Trainer checkpoint (a format of this library: the reference archives its optimiser state with Boost
+// serialization and zstd, trainer2.cpp:213-249). Little-endian, fields in order:
+//   "TA3HTRN1", u32 version = 1, the config (i32 kind, u32 islands, population, games, max_moves, generations,
+//   f64 sigma0, init_std, u64 seed), u32 params, u64 iteration, the seed generator (4 x u64, u8 spare flag,
+//   f64 spare), f64 champion fitness, u64 n + n f64 champion, then per island: f64 sigma, i64 generation,
+//   its sampler (4 x u64, u8, f64), mean, diagonal covariance, sigma path, covariance path (params f64 each),
+//   f64 best fitness of the iteration, u64 n + n f64 best model of the iteration.
+// Everything else of the state follows from the config. A loaded trainer continues exactly as the saved one.
+
+int ta3h_trainer_save(const ta3h_trainer* t, const char* path) {
+    if(!t || !path) return TA3H_E_ARG;
+    FILE* f = std::fopen(path, "wb");
+    if(!f) return TA3H_E_ARG;
+    Writer w{f};
+    w.ok = std::fwrite(CKPT_MAGIC, 1, sizeof(CKPT_MAGIC), f) == sizeof(CKPT_MAGIC);
+    w.put(static_cast<uint32_t>(1));
+    ta3h_train_config const& c = t->cfg;
+    w.put(static_cast<int32_t>(c.kind));
+    w.put(c.islands);
+    w.put(c.population);
+    w.put(c.games);
+    w.put(c.max_moves);
+    w.put(c.generations);
+    w.put(c.sigma0);
+    w.put(c.init_std);
+    w.put(c.seed);
+    w.put(static_cast<uint32_t>(t->params));
+    w.put(t->iteration);
+    w.rng(t->seedGen);
+    w.put(t->championFitness);
+    w.doubles(t->champion);
+    for(size_t i = 0; i < t->islands.size(); ++i) {
+        SepCma const& isl = t->islands[i];
+        w.put(isl.sigma);
+        w.put(static_cast<int64_t>(isl.gen));
+        w.rng(isl.rng);
+        for(auto const* v : {&isl.mean, &isl.diag, &isl.ps, &isl.pc})
+            w.ok = w.ok && std::fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+        w.put(t->islandBest[i]);
+        w.doubles(t->islandBestX[i]);
+    }
+    bool const ok = w.ok && std::fclose(f) == 0;
+    return ok ? TA3H_OK : TA3H_E_ARG;
+}
+
+int ta3h_trainer_load(const char* path, ta3h_trainer** out) {
+    if(!path || !out) return TA3H_E_ARG;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if(!f) return TA3H_E_ARG;
+    Reader r{f};
+    char magic[sizeof(CKPT_MAGIC)] = {};
+    r.ok = std::fread(magic, 1, sizeof(magic), f) == sizeof(magic) && std::memcmp(magic, CKPT_MAGIC, sizeof(magic)) == 0;
+    r.ok = r.ok && r.get<uint32_t>() == 1u;
+    ta3h_train_config c{};
+    c.kind = r.get<int32_t>();
+    c.islands = r.get<uint32_t>();
+    c.population = r.get<uint32_t>();
+    c.games = r.get<uint32_t>();
+    c.max_moves = r.get<uint32_t>();
+    c.generations = r.get<uint32_t>();
+    c.sigma0 = r.get<double>();
+    c.init_std = r.get<double>();
+    c.seed = r.get<uint64_t>();
+    uint32_t const params = r.get<uint32_t>();
+    ta3h_trainer* t = nullptr;
+    // the config rebuilds the derived state (strategy parameters, buffers); the saved state then replaces
+    // everything that evolves
+    if(!r.ok || c.islands > 1024 || ta3h_trainer_create(&c, &t) != TA3H_OK || params != static_cast<uint32_t>(t->params)) {
+        std::fclose(f);
+        ta3h_trainer_destroy(t);
+        return TA3H_E_ARG;
+    }
+    std::unique_ptr<ta3h_trainer> own(t);
+    t->iteration = r.get<uint64_t>();
+    r.rng(t->seedGen);
+    t->championFitness = r.get<double>();
+    r.doubles(t->champion, params);
+    for(uint32_t i = 0; i < c.islands && r.ok; ++i) {
+        SepCma& isl = t->islands[i];
+        isl.sigma = r.get<double>();
+        isl.gen = static_cast<long>(r.get<int64_t>());
+        r.rng(isl.rng);
+        r.exact(isl.mean, params);
+        r.exact(isl.diag, params);
+        r.exact(isl.ps, params);
+        r.exact(isl.pc, params);
+        t->islandBest[i] = r.get<double>();
+        r.doubles(t->islandBestX[i], params);
+    }
+    bool const ok = r.ok && std::fgetc(f) == EOF; // nothing may follow
+    std::fclose(f);
+    if(!ok) return TA3H_E_ARG;
+    *out = own.release();
     return TA3H_OK;
 }
 

@@ -5,7 +5,9 @@
 Defaults are the reference trainer's setup (4 islands x 50 population x 50 games, 10 generations per
 iteration, README.md:9) with sigma 0.3 and island means ~ N(0, 0.5^2). Prints one JSON line per iteration
 (best / champion fitness, generation latency, placements per second) and writes the champion as
-<out>_<iteration>.champ (u64 count + doubles, the reference's .champ format) and <out>.champ.
+<out>_<iteration>.champ (u64 count + doubles, the reference's .champ format) and <out>.champ, and the whole
+trainer state as <out>.ckpt; `--resume <out>.ckpt` continues a run exactly where it stopped (the checkpoint's
+configuration replaces the command line's).
 """
 from __future__ import annotations
 
@@ -35,13 +37,20 @@ def main(argv=None) -> int:
     ap.add_argument("--init-std", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=0x7E7215)
     ap.add_argument("--garbage", action="store_true", help="start games with 0-4 garbage rows")
-    ap.add_argument("--out", default="", help="champion path prefix (.champ files); empty = none")
+    ap.add_argument("--out", default="", help="path prefix of the .champ files and the .ckpt checkpoint; empty = none")
+    ap.add_argument("--resume", default="", help="continue from a .ckpt checkpoint")
     args = ap.parse_args(argv)
 
-    kind = KINDS[args.kind]
-    trainer = Trainer(kind=kind, islands=args.islands, population=args.population, games=args.games,
-                      max_moves=args.max_moves, generations=args.generations, sigma0=args.sigma0,
-                      init_std=args.init_std, seed=args.seed)
+    if args.resume:
+        trainer = Trainer.load(args.resume)
+        kind = trainer.kind
+        args.islands, args.population = trainer.cfg_islands, trainer.cfg_population
+        args.games, args.generations = trainer.cfg_games, trainer.cfg_generations
+    else:
+        kind = KINDS[args.kind]
+        trainer = Trainer(kind=kind, islands=args.islands, population=args.population, games=args.games,
+                          max_moves=args.max_moves, generations=args.generations, sigma0=args.sigma0,
+                          init_std=args.init_std, seed=args.seed)
     ev = ModelEvaluator(model=kind)
     if args.garbage:
         ev.set_garbage(True)
@@ -64,6 +73,7 @@ def main(argv=None) -> int:
         if args.out:
             champ_write(f"{args.out}_{rep['iteration']:03d}.champ", w)
             champ_write(f"{args.out}.champ", w)
+            trainer.save(f"{args.out}.ckpt")
     ev.close()
     trainer.close()
     return 0
