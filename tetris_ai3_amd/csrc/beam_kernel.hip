@@ -121,13 +121,34 @@ __device__ inline uint64_t bitonic_sort_desc(uint64_t v) {
     return v;
 }
 
+// chunks with at most this many keys above the running K-th best insert them one by one instead of sorting
+constexpr int TOPK_INSERT_MAX = 16;
+
 // Running exact top-K (K <= 32): lanes 0..K-1 hold the best keys seen so far in descending order,
-// other lanes hold 0. Folds one chunk of 64 candidate keys (0 = no candidate) into the list.
+// other lanes hold 0. Folds one chunk of 64 candidate keys (0 = no candidate) into the list. Keys are unique,
+// so the list after a chunk is the same whether its qualifying keys are inserted one at a time (few of them:
+// rank by one ballot, shift the tail down one lane; a key no longer above the K-th is dropped) or the chunk is
+// sorted and merged (many).
 template<int K>
 __device__ inline uint64_t topk_merge(uint64_t running, uint64_t key) {
     uint64_t const thr = readlane_u64(running, K - 1);
-    if(__ballot(key > thr) == 0) return running;
+    uint64_t better = __ballot(key > thr);
+    if(better == 0) return running;
     uint32_t const lane = lane_id();
+    if(__popcll(better) <= TOPK_INSERT_MAX) {
+        uint32_t const up = lane == 0 ? 0u : lane - 1;
+        do {
+            int const src = __builtin_ctzll(better);
+            better &= better - 1;
+            uint64_t const k = readlane_u64(key, src);
+            uint32_t const pos = static_cast<uint32_t>(__popcll(__ballot(running > k))); // keys above k
+            if(pos < static_cast<uint32_t>(K)) {
+                uint64_t const prev = shfl_u64(running, static_cast<int>(up));
+                running = lane < pos ? running : (lane == pos ? k : (lane < static_cast<uint32_t>(K) ? prev : 0));
+            }
+        } while(better != 0);
+        return running;
+    }
     uint64_t const sorted = bitonic_sort_desc(key);
     uint64_t const mirrored = shfl_u64(sorted, static_cast<int>(63 - lane));
     uint64_t v = lane < 32 ? running : mirrored; // descending then ascending: a bitonic sequence
