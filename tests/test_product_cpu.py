@@ -86,3 +86,14 @@ def test_graph_run_without_prepare_is_a_state_error():
     ev._graph_models, ev._graph_items = 1, 1
     with pytest.raises(T.HipError, match="no generation graph"):
         ev.graph_run(np.zeros(97, dtype=np.float32))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 (or a multi-rank launch) exits with a message, before touching
+    a device"""
+    import subprocess
+    import sys
+    for args, env in ((["--gpus", "2"], {}), ([], {"WORLD_SIZE": "4"})):
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, capture_output=True, text=True,
+                           timeout=120, env={**os.environ, **env})
+        assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
