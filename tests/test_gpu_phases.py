@@ -120,3 +120,21 @@ def test_auto_budget_parks_long_games():
     assert one[0].tobytes() == auto[0].tobytes() and (one[1] == auto[1]).all() and (one[2] == auto[2]).all()
     host, host_pl = O.eval_host_mt(O.MODEL_V11, w, 1, seeds, 2100)
     assert auto[0].tobytes() == host.tobytes() and (auto[1] == host_pl).all()
+
+
+def test_graph_prepared_first_with_two_phases():
+    """graph_prepare as the first call of a fresh evaluator, with the two-phase schedule on: the parked-game
+    buffers are allocated before the capture begins, and every replay equals the oracle"""
+    say("case: graph prepared first, two phases")
+    kind, nm, cap = O.MODEL_F16, 2, 60
+    w = np.random.default_rng(56).normal(0, 0.5, nm * O.num_params(kind)).astype(np.float32)
+    seeds = O.expand_seeds(57, 3)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_phases(15)
+    ev.graph_prepare(nm, seeds, cap)
+    host, host_pl = O.eval_host_mt(kind, w, nm, seeds, cap)
+    assert (host_pl > 15).any()  # some game is parked and resumed
+    for rep in range(2):
+        fit, pl = ev.graph_run(w)
+        assert fit.tobytes() == host.tobytes() and (pl == host_pl).all(), rep
+    ev.close()

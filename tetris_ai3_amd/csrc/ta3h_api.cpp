@@ -195,14 +195,12 @@ struct ta3h_evaluator {
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
         if(e != hipSuccess) return fail("device attribute", e);
         numCUs = static_cast<uint32_t>(cus);
-        e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-        if(e != hipSuccess) return fail("create stream", e);
-        e = hipEventCreate(&evStart);
-        if(e != hipSuccess) return fail("create event", e);
-        e = hipEventCreate(&evStop);
-        if(e != hipSuccess) return fail("create event", e);
-        e = hipMalloc(&dQueue, 64);
-        if(e != hipSuccess) return fail("malloc queue", e);
+        // each resource is created once: a call after a partial failure picks up where that one stopped
+        if(!stream && (e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess)
+            return fail("create stream", e);
+        if(!evStart && (e = hipEventCreate(&evStart)) != hipSuccess) return fail("create event", e);
+        if(!evStop && (e = hipEventCreate(&evStop)) != hipSuccess) return fail("create event", e);
+        if(!dQueue && (e = hipMalloc(&dQueue, 64)) != hipSuccess) return fail("malloc queue", e);
         e = ta3h::upload_constants(); // once per process and device
         if(e != hipSuccess) return fail("upload constants", e);
         ready = true;
@@ -363,7 +361,29 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
 
 namespace {
 
-// Enqueues the search of the staged batch on the evaluator's stream (optionally bracketed by the timing events).
+// First-phase move budget of the two-phase schedule for the staged batch; 0 = one phase.
+uint32_t phase_budget(ta3h_evaluator const* ev) {
+    uint32_t const phase = ev->phaseMoves != TA3H_PHASES_AUTO
+                               ? ev->phaseMoves
+                               : (ev->maxMoves / 16 < 128 ? 128u : (ev->maxMoves / 16 > 512 ? 512u : ev->maxMoves / 16));
+    return phase != 0 && ev->persistent && phase < ev->maxMoves ? phase : 0u;
+}
+
+size_t resume_words(ta3h_evaluator const* ev) { // list [items], counter, order [items], per-model counts
+    return 2 * static_cast<size_t>(ev->items) + 1 + ev->items / ev->games;
+}
+
+// Device buffers the launch of the staged batch needs beyond the staged ones (the parked games of the two-phase
+// schedule). Called before enqueue_eval and never while the stream is being captured.
+int prepare_launch(ta3h_evaluator* ev) {
+    if(phase_budget(ev) == 0) return TA3H_OK;
+    int r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked");
+    if(r) return r;
+    return ev->grow(ev->dResume, ev->resumeCap, resume_words(ev), "malloc resume list");
+}
+
+// Enqueues the search of the staged batch on the evaluator's stream (optionally bracketed by the timing events);
+// allocates nothing, so it may run inside a stream capture.
 int enqueue_eval(ta3h_evaluator* ev, bool events) {
     ta3h::EvalArgs a{};
     a.weights = ev->dWeights;
@@ -379,15 +399,12 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.weightStride = stride_of(ev->kind);
     a.persistent = static_cast<uint32_t>(ev->persistent);
     a.garbage = static_cast<uint32_t>(ev->garbage);
-    uint32_t const phase = ev->phaseMoves != TA3H_PHASES_AUTO
-                               ? ev->phaseMoves
-                               : (ev->maxMoves / 16 < 128 ? 128u : (ev->maxMoves / 16 > 512 ? 512u : ev->maxMoves / 16));
-    if(phase != 0 && ev->persistent && phase < ev->maxMoves) {
-        int r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked");
-        if(r) return r;
-        // list [items], counter, order [items], per-model counts [items / games]
-        size_t const words = 2 * static_cast<size_t>(ev->items) + 1 + ev->items / ev->games;
-        if((r = ev->grow(ev->dResume, ev->resumeCap, words, "malloc resume list"))) return r;
+    uint32_t const phase = phase_budget(ev);
+    if(phase != 0) {
+        if(ev->parkedCap < static_cast<size_t>(ev->items) * ta3h::PARK_WORDS || ev->resumeCap < resume_words(ev)) {
+            ev->err = "launch: two-phase buffers not prepared";
+            return TA3H_E_STATE;
+        }
         a.phaseMoves = phase;
         a.parked = ev->dParked;
         a.resumeList = ev->dResume;
@@ -442,9 +459,14 @@ int ta3h_launch(ta3h_evaluator* ev) {
         ev->err = "launch: nothing staged";
         return TA3H_E_STATE;
     }
-    ev->launched = true;
-    if(ev->items == 0) return TA3H_OK;
-    return enqueue_eval(ev, true);
+    if(ev->items == 0) {
+        ev->launched = true;
+        return TA3H_OK;
+    }
+    int r = prepare_launch(ev);
+    if(r == TA3H_OK) r = enqueue_eval(ev, true);
+    ev->launched = r == TA3H_OK; // a failed launch leaves nothing to fetch
+    return r;
 }
 
 int ta3h_sync(ta3h_evaluator* ev) {
@@ -555,7 +577,7 @@ int ta3h_model_values(int kind, const float* weights, const uint32_t* boards, co
     if(n == 0) return TA3H_OK;
     std::vector<float> packed(stride_of(kind));
     repack(kind, weights, packed.data());
-    size_t const bytes = packed.size() * 4 + n * (40 + 4 + 1 + 4) + 64;
+    size_t const bytes = packed.size() * 4 + static_cast<size_t>(n) * (40 + 4 + 1 + 4) + 64;
     uint8_t* d = nullptr;
     if(hipMalloc(&d, bytes) != hipSuccess) return TA3H_E_HIP;
     float* dw = reinterpret_cast<float*>(d);
@@ -636,6 +658,7 @@ int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_par
     ev->maxMoves = max_moves;
     ev->staged = true;
     ev->launched = false;
+    if((r = prepare_launch(ev))) return r; // no allocation inside the capture
     // capture: weights in, search, fitness and placements out
     e = hipStreamBeginCapture(ev->stream, hipStreamCaptureModeThreadLocal);
     if(e != hipSuccess) return ev->fail("begin capture", e);
