@@ -104,6 +104,14 @@ int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 int ta3h_set_timeline(ta3h_evaluator* ev, int enabled);
 int ta3h_fetch_timeline(ta3h_evaluator* ev, uint64_t* out, double* ticks_per_second);
 
+/* Optional move log (no reference counterpart; the per-move record of a game that eval_host's loop plays,
+ * model_eval_host.cpp:37-47): when enabled, the following launches record every committed move of every item;
+ * fetch writes out[item * max_moves + i] for the item's placements i = 0 .. placements - 1 as one byte,
+ * hold << 7 | rotation << 5 | (x + 2) (bytes past an item's placement count are undefined). Used to report the
+ * per-move agreement of device games with the CPU search. max_moves bytes per item of device memory. */
+int ta3h_set_move_log(ta3h_evaluator* ev, int enabled);
+int ta3h_fetch_moves(ta3h_evaluator* ev, uint8_t* out);
+
 /* Device time of the last launch's search kernel in milliseconds (HIP events on the evaluator's stream). */
 int ta3h_last_kernel_ms(ta3h_evaluator* ev, float* ms);
 

@@ -69,6 +69,8 @@ def lib():
             "ta3h_last_kernel_ms": (C.c_int, [vp, f32p]),
             "ta3h_fetch_counters": (C.c_int, [vp, u64p]),
             "ta3h_set_timeline": (C.c_int, [vp, C.c_int]),
+            "ta3h_set_move_log": (C.c_int, [vp, C.c_int]),
+            "ta3h_fetch_moves": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
             "ta3h_fetch_timeline": (C.c_int, [vp, u64p, C.POINTER(C.c_double)]),
             "ta3h_launch_info": (C.c_int, [vp, u32p, u32p, u32p]),
             "ta3h_device_bits_canary": (C.c_int, [C.POINTER(C.c_int)]),
@@ -126,6 +128,7 @@ class ModelEvaluator:
         if lib().ta3h_create(C.byref(self._h)) != 0:
             raise HipError("ta3h_create failed")
         self.model = model
+        self._items, self._max_moves = 0, 0
         self._check(lib().ta3h_set_model(self._h, model))
         self.set_launch(DEFAULT_WAVES_PER_GAME.get(model, 2), True, 0)
 
@@ -181,6 +184,7 @@ class ModelEvaluator:
             num_models = num_models_or_seeds
         w, m, s = self._args(weights, num_models, seeds)
         out = np.zeros(m * s.size, dtype=np.float32)
+        self._items, self._max_moves = m * s.size, int(max_moves)
         self._check(lib().ta3h_eval(self._h, _ptr(w, C.c_float), m, self.num_params, _ptr(s, C.c_uint64), s.size,
                                     int(max_moves), _ptr(out, C.c_float)))
         return out
@@ -189,6 +193,7 @@ class ModelEvaluator:
     def stage(self, weights, num_models, seeds, max_moves):
         w, m, s = self._args(weights, num_models, seeds)
         self._items = m * s.size
+        self._max_moves = int(max_moves)
         self._check(lib().ta3h_stage(self._h, _ptr(w, C.c_float), m, self.num_params, _ptr(s, C.c_uint64), s.size,
                                      int(max_moves)))
 
@@ -221,6 +226,17 @@ class ModelEvaluator:
         t = out.reshape(-1, 2).astype(np.float64)
         return (t - t[:, 0].min()) / rate.value
 
+    def set_move_log(self, enabled: bool = True):
+        """record every committed move of the following launches (fetch_moves)"""
+        self._check(lib().ta3h_set_move_log(self._h, int(enabled)))
+
+    def fetch_moves(self):
+        """move log of the last launch: uint8 [items, max_moves], byte i of an item = its move i as
+        hold << 7 | rotation << 5 | (x + 2), for i below the item's placement count (the rest undefined)"""
+        out = np.zeros(self._items * self._max_moves, dtype=np.uint8)
+        self._check(lib().ta3h_fetch_moves(self._h, _ptr(out, C.c_uint8)))
+        return out.reshape(self._items, self._max_moves)
+
     def last_kernel_ms(self) -> float:
         v = C.c_float(0)
         self._check(lib().ta3h_last_kernel_ms(self._h, C.byref(v)))
@@ -238,6 +254,7 @@ class ModelEvaluator:
         self._graph_items = int(num_models) * s.size
         self._graph_models = int(num_models)
         self._items = self._graph_items
+        self._max_moves = int(max_moves)
         self._check(lib().ta3h_graph_prepare(self._h, int(num_models), self.num_params, _ptr(s, C.c_uint64), s.size,
                                              int(max_moves)))
 

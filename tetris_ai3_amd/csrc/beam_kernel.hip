@@ -215,6 +215,7 @@ struct alignas(16) GameLds {
     uint64_t partMask[4];                     // per-wave kept-node / segment legality masks
     uint32_t partLegal[4];                    // per-wave legal-candidate counts of the item
     EngineState eng;
+    uint32_t moveItem; // the item being played (its move-log row; trace and pruned kernels)
 };
 
 // Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
@@ -1476,6 +1477,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
         LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + rootSlot % 34];
         int const o = e.ox & 3, x = (e.ox >> 2) - 2;
         EngineState& eng = L.eng;
+        if constexpr(MODE != MODE_PLAIN) // the item's move log, one byte per placement (not kept by the plain kernels)
+            if(a.moves != nullptr && eng.placements < a.maxMoves)
+                a.moves[static_cast<uint64_t>(L.moveItem) * a.maxMoves + eng.placements] =
+                    static_cast<uint8_t>((rn.rootHold ? 0x80u : 0u) | (e.ox & 3u) << 5 | (e.ox >> 2));
         if(rn.rootHold) engine_hold(eng);
         int const cleared = engine_place(eng, o, x);
         ++eng.placements;
@@ -1543,6 +1548,8 @@ void beam_eval_kernel(EvalArgs a) {
         uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
         uint64_t nApply = 0, nSlots = 0;
         uint32_t nLegal = 0, mv = 0;
+        if constexpr(MODE != MODE_PLAIN)
+            if(threadIdx.x == 0) L.moveItem = item;
         if(a.resumePhase) { // a parked game: its engine state, counters and move index
             if(wv == 0 && lane < ENGINE_WORDS) reinterpret_cast<uint32_t*>(&L.eng)[lane] = park[lane];
             nApply = uniform(static_cast<uint64_t>(park[ENGINE_WORDS]) | static_cast<uint64_t>(park[ENGINE_WORDS + 1]) << 32);
@@ -1705,7 +1712,8 @@ __global__ void feature_norm_kernel(float* out, uint32_t n) {
 
 template<int KIND>
 hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t stream) {
-    if(a.pruner != nullptr || a.trace != nullptr) { // pruned and trace kernels run 4 waves per game
+    // pruned and trace kernels (the latter also for a move log alone) run 4 waves per game
+    if(a.pruner != nullptr || a.trace != nullptr || a.moves != nullptr) {
         if(wpb != 4) return hipErrorInvalidValue;
         if(a.pruner != nullptr)
             hipLaunchKernelGGL((beam_eval_kernel<KIND, 4, MODE_PRUNED>), dim3(blocks), dim3(256), 0, stream, a);

@@ -32,6 +32,8 @@ struct EvalArgs {
     uint32_t* placements;     // [items] engine.place calls per item (may be null)
     uint64_t* counters;       // [items][3]: candidates fitted, candidates legal, lanes issued (may be null)
     uint64_t* timeline;       // [items][2]: device wall clock at item start / end (may be null)
+    uint8_t* moves;           // [items][maxMoves] committed moves, hold << 7 | rotation << 5 | (x + 2) (may be null;
+                              // written by the trace and pruned kernels)
     uint32_t* queue;          // work-queue counter (persistent mode)
     uint32_t items;
     uint32_t games;

@@ -112,6 +112,10 @@ struct ta3h_evaluator {
     uint64_t* dTimeline = nullptr;
     size_t timelineCap = 0;
     bool timelineOn = false;
+    uint8_t* dMoves = nullptr; // move log [items][maxMoves]
+    size_t movesCap = 0;
+    bool movesOn = false;
+    bool movesLogged = false; // the last launch wrote the move log
     uint32_t* dQueue = nullptr;
     // two-phase schedule
     uint32_t phaseMoves = TA3H_PHASES_AUTO; // first-phase move budget of the two-phase schedule; 0 = one phase
@@ -256,6 +260,7 @@ struct ta3h_evaluator {
         if(dPlacements) static_cast<void>(hipFree(dPlacements));
         if(dCounters) static_cast<void>(hipFree(dCounters));
         if(dTimeline) static_cast<void>(hipFree(dTimeline));
+        if(dMoves) static_cast<void>(hipFree(dMoves));
         if(dQueue) static_cast<void>(hipFree(dQueue));
         for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
                        static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
@@ -376,9 +381,13 @@ size_t resume_words(ta3h_evaluator const* ev) { // list [items], counter, order 
 // Device buffers the launch of the staged batch needs beyond the staged ones (the parked games of the two-phase
 // schedule). Called before enqueue_eval and never while the stream is being captured.
 int prepare_launch(ta3h_evaluator* ev) {
+    int r;
+    if(ev->movesOn
+       && (r = ev->grow(ev->dMoves, ev->movesCap, static_cast<size_t>(ev->items) * ev->maxMoves, "malloc move log")))
+        return r;
     if(phase_budget(ev) == 0) return TA3H_OK;
-    int r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked");
-    if(r) return r;
+    if((r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked")))
+        return r;
     return ev->grow(ev->dResume, ev->resumeCap, resume_words(ev), "malloc resume list");
 }
 
@@ -392,6 +401,14 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.placements = ev->dPlacements;
     a.counters = ev->dCounters;
     a.timeline = ev->timelineOn ? ev->dTimeline : nullptr;
+    if(ev->movesOn) {
+        if(ev->movesCap < static_cast<size_t>(ev->items) * ev->maxMoves) {
+            ev->err = "launch: move log not prepared";
+            return TA3H_E_STATE;
+        }
+        a.moves = ev->dMoves;
+    }
+    ev->movesLogged = ev->movesOn;
     a.queue = ev->dQueue;
     a.items = ev->items;
     a.games = ev->games;
@@ -424,6 +441,9 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
         a.traceCount = ev->dTraceCount;
         a.traceCap = static_cast<uint32_t>(ev->traceCap);
         a.traceEvery = ev->traceEvery;
+        mode = 1;
+        wpb = 4;
+    } else if(ev->movesOn) { // the trace kernel (recording no traces) writes the move log of a full beam
         mode = 1;
         wpb = 4;
     }
@@ -522,6 +542,28 @@ int ta3h_set_timeline(ta3h_evaluator* ev, int enabled) {
     return TA3H_OK;
 }
 
+int ta3h_set_move_log(ta3h_evaluator* ev, int enabled) {
+    if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings and buffers change
+    ev->movesOn = enabled != 0;
+    return TA3H_OK;
+}
+
+int ta3h_fetch_moves(ta3h_evaluator* ev, uint8_t* out) {
+    if(!ev || !out) return TA3H_E_ARG;
+    if(!ev->launched || !ev->movesLogged) {
+        ev->err = "fetch_moves: no launch with the move log enabled";
+        return TA3H_E_STATE;
+    }
+    size_t const bytes = static_cast<size_t>(ev->items) * ev->maxMoves;
+    if(bytes == 0) return TA3H_OK;
+    hipError_t e = hipMemcpyAsync(out, ev->dMoves, bytes, hipMemcpyDeviceToHost, ev->stream);
+    if(e != hipSuccess) return ev->fail("copy move log", e);
+    e = hipStreamSynchronize(ev->stream);
+    if(e != hipSuccess) return ev->fail("kernel run", e);
+    return TA3H_OK;
+}
+
 int ta3h_fetch_timeline(ta3h_evaluator* ev, uint64_t* out, double* ticks_per_second) {
     if(!ev || !out) return TA3H_E_ARG;
     if(!ev->launched || !ev->timelineOn) {
@@ -566,7 +608,7 @@ int ta3h_launch_info(ta3h_evaluator* ev, uint32_t* grid_blocks, uint32_t* waves_
     if(!ev) return TA3H_E_ARG;
     if(grid_blocks) *grid_blocks = ev->lastGrid;
     if(waves_per_game)
-        *waves_per_game = static_cast<uint32_t>(ev->prunerOn || ev->traceEvery != 0 ? 4 : ev->wavesPerBlock);
+        *waves_per_game = static_cast<uint32_t>(ev->prunerOn || ev->traceEvery != 0 || ev->movesOn ? 4 : ev->wavesPerBlock);
     if(num_cus) *num_cus = ev->numCUs;
     return TA3H_OK;
 }
