@@ -45,7 +45,7 @@ def _workload(model: str):
     return w, M, seeds, bench.CAP
 
 
-def run_gpu(out: str, model: str, sample: int, moves: int) -> None:
+def run_gpu(out: str, model: str, sample: int, moves: int, min_length: int) -> None:
     import tetris_ai3_amd as T
     kind = KINDS[model]
     w, M, seeds, cap = _workload(model)
@@ -59,8 +59,9 @@ def run_gpu(out: str, model: str, sample: int, moves: int) -> None:
     log = ev.fetch_moves()
     ev.close()
     items = np.arange(M * len(seeds))
-    if model == "wide":  # an evenly spread sample, first `moves` moves of each
-        items = np.linspace(0, items.size - 1, sample).round().astype(np.int64)
+    if model == "wide":  # an evenly spread sample of the games at least min_length long, first `moves` moves
+        pool = np.nonzero(pl >= min_length)[0]
+        items = pool[np.linspace(0, pool.size - 1, min(sample, pool.size)).round().astype(np.int64)]
         log = log[items, :moves]
     np.savez_compressed(out, model=model, fitness=fit, placements=pl, items=items, moves=log,
                         weights_sum=np.float64(np.asarray(w, dtype=np.float64).sum()))
@@ -165,7 +166,8 @@ def compare_wide(d, threads: int) -> dict:
             _progress(done, items.size, t0, 8)
     return {
         "workload": f"{M} models x {len(seeds)} games, cap {cap}, garbage rows, 32-128-128-1 on BF16 MFMA; "
-                    f"{items.size} games evenly spread over the batch, their first {glog.shape[1]} moves",
+                    f"{items.size} games (at least {int(gpl[items].min())} moves long) evenly spread over the batch, "
+                    f"their first {glog.shape[1]} moves",
         "games_sampled": int(items.size),
         "moves_compared": int(n_moves.sum()),
         "moves_agreeing": int(same.sum()),
@@ -203,13 +205,14 @@ def main() -> int:
     g.add_argument("--model", default="f16", choices=sorted(KINDS))
     g.add_argument("--sample", type=int, default=64, help="wide: games in the sample")
     g.add_argument("--moves", type=int, default=150, help="wide: moves per sampled game")
+    g.add_argument("--min-length", type=int, default=0, help="wide: sample only games at least this long")
     c = sub.add_parser("compare")
     c.add_argument("path")
     c.add_argument("--threads", type=int, default=min(os.cpu_count() or 1, 16))
     c.add_argument("--out", default=None, help="also write the JSON report here")
     a = ap.parse_args()
     if a.cmd == "gpu":
-        run_gpu(a.out, a.model, a.sample, a.moves)
+        run_gpu(a.out, a.model, a.sample, a.moves, a.min_length)
         return 0
     return run_compare(a.path, a.threads, a.out)
 

@@ -83,6 +83,31 @@ def test_move_log_garbage_and_graph():
     ev.close()
 
 
+def test_move_log_with_traces_and_pruned_k34():
+    """the log is kept alongside traces, and by the pruned kernel (k = 34 is the full beam)"""
+    kind, nm, cap = O.MODEL_F16, 2, 60
+    seeds = O.expand_seeds(67, 3)
+    w = np.random.default_rng(68).normal(0, 0.5, nm * O.num_params(kind)).astype(np.float32)
+    want = oracle_logs(kind, w, nm, seeds, cap)
+    ev = T.ModelEvaluator(model=kind)
+    ev.set_move_log(True)
+    for case in ("traces", "pruned k=34"):
+        say(f"case: move log with {case}")
+        if case == "traces":
+            ev.set_trace(2_000_000, every=4)
+        else:
+            ev.set_pruner(T.pruner_init(69), 4.0, k=34, ply_mask=0b1110)
+        ev.stage(w, nm, seeds, cap)
+        ev.launch()
+        fit, pl = ev.fetch()
+        log = ev.fetch_moves()
+        for i, (wm, wf) in enumerate(want):
+            assert pl[i] == wm.size and fit[i] == wf and (log[i, :pl[i]] == wm).all(), (case, i)
+        if case == "traces":
+            assert ev.fetch_trace()[1] > 0
+    ev.close()
+
+
 def test_move_log_off_is_a_state_error():
     ev = T.ModelEvaluator()
     say("case: move log not enabled")
