@@ -87,7 +87,8 @@ def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
     from tetris_ai3_amd import learn
     ev = T.ModelEvaluator(model=kind)
     t0 = time.perf_counter()
-    recs, produced, _ = learn.collect_traces(ev, w, M, seeds, CAP, every=512, cap=3_000_000)
+    # a buffer for every record (~17 M of 44 B): the kept set, the sample and the trained pruner are reproducible
+    recs, produced, _ = learn.collect_traces(ev, w, M, seeds, CAP, every=512, cap=24_000_000)
     rng = np.random.default_rng(1)
     train = recs if recs.shape[0] <= 2_000_000 else recs[rng.choice(recs.shape[0], 2_000_000, replace=False)]
     pw, losses, _ = learn.train_pruner(ev, train, epochs=PRUNE_EPOCHS, batch=512, lr=1e-3, seed=1)
@@ -120,7 +121,8 @@ def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
     out = dict(best)
     out.update({"settings": settings,
                 "trained_on": f"{train.shape[0]} of {produced} full-beam trace records of this generation "
-                              f"(1 move in 512 per game), {PRUNE_EPOCHS} epochs",
+                              f"(1 move in 512 per game{'' if recs.shape[0] == produced else '; buffer full'}), "
+                              f"{PRUNE_EPOCHS} epochs",
                 "training_loss": [float(x) for x in losses],
                 "setup_s": setup_s})
     return out

@@ -184,7 +184,8 @@ def test_training_learns_the_labels(trace_set):
     """a few device epochs on the traces: the loss falls and the trained logits separate the labels
     (ROC AUC on held-out records) far better than the initial network"""
     say("case: pruner training")
-    recs = trace_set["recs"]
+    from tetris_ai3_amd import learn
+    recs = learn.canonical_order(trace_set["recs"])
     rng = np.random.default_rng(12)
     perm = rng.permutation(recs.shape[0])
     n_train = int(0.8 * len(perm))
@@ -216,7 +217,7 @@ def test_trained_pruner_gain_over_untrained():
     beam, whose traces (1 move in 8) train the pruner on the device. On 48 other games the pruned beam (k = 4 at
     plies 2 and 3) with the trained network keeps much more of the full beam's mean score than the same pruned
     beam ranked by the untrained network, at an equal cut in value evaluations. Measured on the MI355X: full beam
-    294.25 with 3,509 value evaluations per placement; trained pruner 291.76 (0.85% loss) with 1,262; untrained
+    294.25 with 3,509 value evaluations per placement; trained pruner 290.67 (1.22% loss) with 1,262; untrained
     155.72 (47% loss) with 1,241."""
     import json
     import os

@@ -97,3 +97,16 @@ def test_bench_refuses_more_than_one_gpu():
         r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, capture_output=True, text=True,
                            timeout=120, env={**os.environ, **env})
         assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_trace_records_canonical_order():
+    """training follows the record order, so records are put in an order that depends on their content alone:
+    any permutation of a record set gives the same array"""
+    from tetris_ai3_amd import learn
+    rng = np.random.default_rng(5)
+    recs = rng.integers(0, 2**32, size=(5000, 11), dtype=np.uint64).astype(np.uint32)
+    recs[100:110] = recs[7]  # duplicates
+    a = learn.canonical_order(recs)
+    b = learn.canonical_order(recs[rng.permutation(recs.shape[0])])
+    assert a.shape == recs.shape and (a == b).all()
+    assert sorted(map(bytes, a)) == sorted(map(bytes, recs))

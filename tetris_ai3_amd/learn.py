@@ -18,15 +18,29 @@ from . import ModelEvaluator, pruner_init
 PLY_SHIFT, LABEL_SHIFT = 9, 11
 
 
+def canonical_order(recs):
+    """The records sorted by content. The device appends them in no particular order and training batches follow
+    the record order, so sorting makes training a function of the record set alone. Key: a 64-bit hash of the 11
+    words (records with equal keys are equal, but for a hash collision, odds ~n^2 / 2^65)."""
+    recs = np.ascontiguousarray(recs, dtype=np.uint32)
+    h = np.zeros(recs.shape[0], dtype=np.uint64)
+    for j in range(recs.shape[1]):  # multiply-xorshift mixing, wrapping mod 2^64
+        h = (h ^ recs[:, j].astype(np.uint64)) * np.uint64(0x9E3779B97F4A7C15)
+        h ^= h >> np.uint64(31)
+    return recs[np.argsort(h, kind="stable")]
+
+
 def collect_traces(ev: ModelEvaluator, weights, num_models: int, seeds, max_moves: int, every: int, cap: int):
-    """Plays the batch with the full beam and returns (records uint32 [n, 11], produced, fitness)."""
+    """Plays the batch with the full beam and returns (records uint32 [n, 11] in canonical order, produced,
+    fitness). Which records a full buffer keeps depends on the order of work: with cap >= produced the set, and
+    so everything trained from it, is reproducible."""
     ev.set_trace(cap, every)
     ev.stage(weights, num_models, seeds, max_moves)
     ev.launch()
     fit, _ = ev.fetch()
     recs, produced = ev.fetch_trace()
     ev.set_trace(0)
-    return recs, produced, fit
+    return canonical_order(recs), produced, fit
 
 
 def label_stats(recs) -> dict:
