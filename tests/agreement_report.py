@@ -87,7 +87,7 @@ def _progress(done: int, total: int, t0: float, every: int) -> None:
         print(f"compare: {done}/{total} items, {time.perf_counter() - t0:.0f} s", flush=True)
 
 
-def compare_exact(d, model: str, threads: int) -> dict:
+def compare_exact(d, model: str, threads: int, every: int = 500) -> dict:
     """f16 / v11: every game replayed on the oracle, compared move by move"""
     import oracle as O
     kind = KINDS[model]
@@ -115,7 +115,7 @@ def compare_exact(d, model: str, threads: int) -> dict:
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as pool:
         for done, _ in enumerate(pool.map(one, order.tolist()), 1):
-            _progress(done, items, t0, 500)
+            _progress(done, items, t0, every)
     same = same_moves & (gfit.view(np.uint32) == hfit.view(np.uint32))
     return {
         "workload": f"{M} models x {len(seeds)} games, cap {cap}, depth-4 beam 16/32/32, {model}",
@@ -181,12 +181,12 @@ def compare_wide(d, threads: int) -> dict:
     }
 
 
-def run_compare(path: str, threads: int, out: str | None) -> int:
+def run_compare(path: str, threads: int, out: str | None, every: int) -> int:
     d = np.load(path)
     model = str(d["model"])
     w, _, _, _ = _workload(model)
     assert float(np.asarray(w, dtype=np.float64).sum()) == float(d["weights_sum"]), "workload differs"
-    report = compare_wide(d, threads) if model == "wide" else compare_exact(d, model, threads)
+    report = compare_wide(d, threads) if model == "wide" else compare_exact(d, model, threads, every)
     text = json.dumps(report, indent=1)
     print(text, flush=True)
     if out:
@@ -210,11 +210,12 @@ def main() -> int:
     c.add_argument("path")
     c.add_argument("--threads", type=int, default=min(os.cpu_count() or 1, 16))
     c.add_argument("--out", default=None, help="also write the JSON report here")
+    c.add_argument("--progress", type=int, default=500, help="print a progress line every this many games")
     a = ap.parse_args()
     if a.cmd == "gpu":
         run_gpu(a.out, a.model, a.sample, a.moves, a.min_length)
         return 0
-    return run_compare(a.path, a.threads, a.out)
+    return run_compare(a.path, a.threads, a.out, a.progress)
 
 
 if __name__ == "__main__":
