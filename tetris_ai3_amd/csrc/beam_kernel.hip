@@ -683,8 +683,17 @@ __device__ inline float relu_bits(float x) {
     return __builtin_bit_cast(float, b > 0 ? b : 0);
 }
 
-template<int CB>
-__device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave) {
+// a load from LDS at byte address addr
+template<typename T>
+__device__ inline T lds_load(uint32_t addr) {
+    return *reinterpret_cast<T const __attribute__((address_space(3)))*>(static_cast<uintptr_t>(addr));
+}
+
+// LDSW: the model's weights were staged in LDS at byte address wlds (wb unused); else they are read from wb in
+// device memory. Fragment reads are lane-major (lane l reads 16 bytes at (fragment group) * 1024 + 16 l), so a
+// wave reads 1 KB in one piece from either.
+template<int CB, bool LDSW = false>
+__device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave, uint32_t wlds = 0) {
     static_assert(CB == 1 || CB == 2 || CB == 4);
     uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
     bf16x8 const* w1f = reinterpret_cast<bf16x8 const*>(wb + f32w::W1F);
@@ -692,7 +701,23 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
     f32x4 const* b1 = reinterpret_cast<f32x4 const*>(wb + f32w::B1);
     f32x4 const* b2 = reinterpret_cast<f32x4 const*>(wb + f32w::B2);
     f32x4 const* w3 = reinterpret_cast<f32x4 const*>(wb + f32w::W3);
-    float const b3 = *reinterpret_cast<float const*>(wb + f32w::B3);
+    if constexpr(LDSW) asm volatile("" : "+s"(wlds)); // re-read the staged weights for every chunk
+    auto frag1 = [&](uint32_t t) -> bf16x8 {
+        if constexpr(LDSW) return lds_load<bf16x8>(wlds + f32w::W1F + (t * 64 + lane) * 16);
+        else {
+            auto const w1l = opaque_global(w1f); // one tile's fragment in flight at a time
+            return w1l[t * 64 + lane];
+        }
+    };
+    auto frag2 = [&](uint32_t g) -> bf16x8 {
+        if constexpr(LDSW) return lds_load<bf16x8>(wlds + f32w::W2F + (g * 64 + lane) * 16);
+        else return w2f[g * 64 + lane];
+    };
+    auto vec4 = [&](f32x4 const* gp, uint32_t off, uint32_t i) -> f32x4 {
+        if constexpr(LDSW) return lds_load<f32x4>(wlds + off + i * 16);
+        else return gp[i];
+    };
+    float const b3 = LDSW ? lds_load<float>(wlds + f32w::B3) : *reinterpret_cast<float const*>(wb + f32w::B3);
     float mine = 0.0f;
 #pragma unroll 1
     for(uint32_t cb = 0; cb < 4; cb += CB) {
@@ -703,9 +728,8 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
             for(int c = 0; c < CB; ++c) x[c] = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * (cb + c) + r16, q)]);
 #pragma unroll
             for(uint32_t t = 0; t < 8; ++t) {
-                auto const w1l = opaque_global(w1f); // one tile's fragment in flight at a time
-                bf16x8 const a = w1l[(t * 16 + r16) * 4 + q];
-                f32x4 const bias = b1[4 * t + q]; // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
+                bf16x8 const a = frag1(t);
+                f32x4 const bias = vec4(b1, f32w::B1, 4 * t + q); // D rows 4 q + j of tile t start at b1[16 t + 4 q + j]
 #pragma unroll
                 for(int c = 0; c < CB; ++c) {
                     f32x4 const acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c], bias, 0, 0, 0);
@@ -720,17 +744,17 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
 #pragma unroll 1
         for(uint32_t t = 0; t < 8; ++t) {
             f32x4 acc[CB];
-            f32x4 const bias = b2[4 * t + q];
+            f32x4 const bias = vec4(b2, f32w::B2, 4 * t + q);
 #pragma unroll
             for(int c = 0; c < CB; ++c) acc[c] = bias;
 #pragma unroll
             for(uint32_t kb = 0; kb < 4; ++kb) {
-                bf16x8 const a = w2f[((t * 4 + kb) * 16 + r16) * 4 + q];
+                bf16x8 const a = frag2(t * 4 + kb);
 #pragma unroll
                 for(int c = 0; c < CB; ++c)
                     acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][kb]), acc[c], 0, 0, 0);
             }
-            f32x4 const w = w3[4 * t + q];
+            f32x4 const w = vec4(w3, f32w::W3, 4 * t + q);
 #pragma unroll
             for(int c = 0; c < CB; ++c)
 #pragma unroll
@@ -863,9 +887,10 @@ struct WideArgs {
     uint16_t* feat;
     uint64_t* segBest;
     bool finite; // every weight of the game's model is finite (enables the exact zero skipping of value_f16)
+    uint32_t wlds; // LDS byte address of the model's staged weights (wide network, plain kernel)
 };
 
-template<int KIND, int STAGE, int K, int G, bool TRACE = false, int WIDE_CB = 4>
+template<int KIND, int STAGE, int K, int G, bool TRACE = false, int WIDE_CB = 4, bool LDSW = false>
 __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
                                      uint64_t* mask, uint32_t& nLegal, WideArgs const& wide) {
     uint32_t const lane = lane_id();
@@ -900,7 +925,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         }
         if constexpr(KIND == MODEL_F32W) { // the whole wave runs the network over the chunk's feature tile
             wave_sync();
-            float const v = value_f32w<WIDE_CB>(wide.wb, wide.feat);
+            float const v = value_f32w<WIDE_CB, LDSW>(wide.wb, wide.feat, wide.wlds);
             if(legal) score = ordered_bits(v);
             wave_sync(); // the tile is rewritten by the next chunk
         }
@@ -1323,6 +1348,12 @@ struct alignas(16) FeatLds {
     uint16_t v[G][64 * 32];
 };
 
+// the wide network's weights of the game's model, in their device layout
+struct alignas(16) WideWeightsLds {
+    uint8_t b[f32w::BYTES];
+};
+static_assert(f32w::BYTES % 16 == 0, "copied as 16-byte words");
+
 
 enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
 
@@ -1354,7 +1385,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
     nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
+    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
     nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
     top = combine_parts<G, K0, true>(L, top, mask, wv);
     uint32_t count = __popcll(__ballot(top != 0));
@@ -1386,8 +1417,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
             }
         }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
-                         : run_stage<KIND, STAGE_MID, K2, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
+                         : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
@@ -1443,10 +1474,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
             group_sync<G>();
             leaf = run_stage<KIND, STAGE_LEAF, 1, G, true, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         } else {
-            leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+            leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         }
     } else {
-        leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
     }
     nSlots += (evalTotal + 63u) & ~63u;
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
@@ -1502,7 +1533,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
-           : (KIND == MODEL_F32W ? 3
+           : (KIND == MODEL_F32W ? (MODE == MODE_PLAIN && G == 4 ? 2 : 3)
                                  : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two)
 
@@ -1517,6 +1548,9 @@ void beam_eval_kernel(EvalArgs a) {
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
     __shared__ std::conditional_t<KIND == MODEL_F32W, FeatLds<G>, uint32_t> featStore; // wide network only
+    // the wide network's weights, staged once per item (plain kernel): every chunk reads its fragments from LDS
+    constexpr bool WIDE_LDS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
+    __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
@@ -1537,6 +1571,13 @@ void beam_eval_kernel(EvalArgs a) {
         WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr,
                       MODE == MODE_TRACE ? pruneLds->segBest : nullptr, false};
         if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
+        if constexpr(WIDE_LDS) { // all waves copy the model's 42,512 bytes; the group sync below publishes them
+            uint4 const* const src = reinterpret_cast<uint4 const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride);
+            uint4* const dst = reinterpret_cast<uint4*>(&wideWeights);
+            for(uint32_t k = threadIdx.x; k < f32w::BYTES / 16; k += 64 * G) dst[k] = src[k];
+            wide.wlds = static_cast<uint32_t>(
+                reinterpret_cast<uintptr_t>((uint8_t __attribute__((address_space(3)))*)(&wideWeights.b[0])));
+        }
         if constexpr(KIND == MODEL_F16) { // each wave checks that the model's weights are all finite
             uint32_t const* const wbits =
                 reinterpret_cast<uint32_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride);

@@ -14,8 +14,8 @@ enum ModelKind : int {
     MODEL_F32W = 3, // 32 features -> 128 ReLU -> 128 ReLU -> 1 on BF16 MFMA (20865 params)
 };
 
-// Device weight layout of the wide network, per model (bytes): w1f [8 t][16 r][4 q] x 8 bf16 = MFMA A fragment
-// (rows 16 t + r, features 8 q .. + 7) of layer 1; w2f [8 t][4 kb][16 r][4 q] x 8 bf16 = A fragment of layer 2
+// Device weight layout of the wide network, per model (bytes): w1f [8 t][4 q][16 r] x 8 bf16 = MFMA A fragment
+// (rows 16 t + r, features 8 q .. + 7) of layer 1; w2f [8 t][4 kb][4 q][16 r] x 8 bf16 = A fragment of layer 2
 // for output rows 16 t + r, K block kb, whose element e is hidden unit 32 kb + 4 q + e (e < 4) or
 // 32 kb + 16 + 4 q + (e - 4) -- the units lane quarter q holds in layer 1's output tiles 2 kb and 2 kb + 1, so
 // layer 1's outputs are layer 2's B operands in place; then b1[128], b2[128], W3[128], b3 as float.

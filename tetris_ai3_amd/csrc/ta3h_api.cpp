@@ -46,8 +46,9 @@ uint16_t bf16_bits(float v) { // round to nearest even
     return static_cast<uint16_t>(u >> 16);
 }
 
-// wide network, host layout W1[128][32], b1, W2[128][128], b2, W3[128], b3 -> MFMA fragment layout:
-// w1f[t][r][q][e] = W1[16 t + r][8 q + e]; w2f[t][kb][r][q][e] = W2[16 t + r][unit(kb, q, e)] with
+// wide network, host layout W1[128][32], b1, W2[128][128], b2, W3[128], b3 -> MFMA fragment layout, lane-major
+// (lane 16 q + r of a wave reads the 16 bytes [.][q][r]):
+// w1f[t][q][r][e] = W1[16 t + r][8 q + e]; w2f[t][kb][q][r][e] = W2[16 t + r][unit(kb, q, e)] with
 // unit = 32 kb + 4 q + e (e < 4) or 32 kb + 16 + 4 q + e - 4 (the units lane quarter q holds in layer 1's
 // output tiles 2 kb, 2 kb + 1); biases and W3 as float.
 void repack_f32w(float const* src, uint8_t* dst) {
@@ -55,7 +56,7 @@ void repack_f32w(float const* src, uint8_t* dst) {
     for(int t = 0; t < 8; ++t)
         for(int r = 0; r < 16; ++r)
             for(int q = 0; q < 4; ++q)
-                for(int e = 0; e < 8; ++e) w1f[((t * 16 + r) * 4 + q) * 8 + e] = bf16_bits(src[(16 * t + r) * 32 + 8 * q + e]);
+                for(int e = 0; e < 8; ++e) w1f[((t * 4 + q) * 16 + r) * 8 + e] = bf16_bits(src[(16 * t + r) * 32 + 8 * q + e]);
     float const* w2 = src + 4224;
     uint16_t* w2f = reinterpret_cast<uint16_t*>(dst + wide::W2F);
     for(int t = 0; t < 8; ++t)
@@ -64,7 +65,7 @@ void repack_f32w(float const* src, uint8_t* dst) {
                 for(int q = 0; q < 4; ++q)
                     for(int e = 0; e < 8; ++e) {
                         int const unit = e < 4 ? 32 * kb + 4 * q + e : 32 * kb + 16 + 4 * q + (e - 4);
-                        w2f[(((t * 4 + kb) * 16 + r) * 4 + q) * 8 + e] = bf16_bits(w2[(16 * t + r) * 128 + unit]);
+                        w2f[(((t * 4 + kb) * 4 + q) * 16 + r) * 8 + e] = bf16_bits(w2[(16 * t + r) * 128 + unit]);
                     }
     std::memcpy(dst + wide::B1, src + 4096, 128 * sizeof(float));
     std::memcpy(dst + wide::B2, src + 4224 + 16384, 128 * sizeof(float));
