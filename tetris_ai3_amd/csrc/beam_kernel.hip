@@ -1535,7 +1535,8 @@ constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
            : (KIND == MODEL_F32W ? (MODE == MODE_PLAIN && G == 4 ? 2 : 3)
                                  : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
-// (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two)
+// (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
+// 4-wave plain kernel at 2, the limit of its LDS with the model's weights staged there)
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
