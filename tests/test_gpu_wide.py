@@ -103,3 +103,23 @@ def test_garbage_games_bit_exact(kind):
     assert fit.tobytes() == want.tobytes() and (pl == want_pl).all()
     plain, _ = O.eval_host_mt(kind, w, nm, seeds, 80)
     assert fit.tobytes() != plain.tobytes()
+
+
+def test_wide_waves_per_game_agree():
+    """the 4-wave kernel reads the weights staged in LDS, the 1- and 2-wave kernels read them from device memory:
+    the same games either way"""
+    nm = 3
+    w = np.random.default_rng(12).normal(0, 0.3, nm * 20865).astype(np.float32)
+    seeds = O.expand_seeds(13, 4)
+    ev = T.ModelEvaluator(model=T.MODEL_F32W)
+    ev.set_garbage(True)
+    got = {}
+    for wpg in (4, 2, 1):
+        say(f"case: wide waves per game {wpg}")
+        ev.set_launch(wpg, True, 0)
+        ev.stage(w, nm, seeds, 60)
+        ev.launch()
+        fit, pl = ev.fetch()
+        got[wpg] = (fit.tobytes(), pl.tobytes())
+    ev.close()
+    assert got[4] == got[2] == got[1]
