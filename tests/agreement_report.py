@@ -2,8 +2,8 @@
 
 Two steps, because the oracle needs hours of CPU time where the device needs seconds:
 
-    python tests/agreement_report.py gpu gpurun_out/agreement_f16.npz [--model f16|v11|wide]   # with the GPU
-    python tests/agreement_report.py compare gpurun_out/agreement_f16.npz [--out report.json]  # host cores
+    python tests/agreement_report.py gpu agreement_f16.npz [--model f16|v11|wide]   # with the GPU
+    python tests/agreement_report.py compare agreement_f16.npz [--out report.json]  # on host cores
 
 `gpu` plays bench.py's workload with the device's move log on and stores every item's fitness, placement count
 and moves (hold, rotation, x of each placement).
