@@ -46,7 +46,8 @@ C_MLP = {0: 184, 1: 1697}
 VALU_PEAK_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz (non-packed 32-bit VALU ops)
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 F32W_FLOPS = 2 * 20865      # per leaf evaluation of the wide network (on MFMA)
-PRUNE_EPOCHS = 8
+PRUNE_EPOCHS = 16
+PRUNE_RECORDS = 8_000_000  # training sample of the generation's ~17 M trace records
 # (k, ply mask) settings of the pruned leg: mask bits 1, 2 = middle levels, 3 = leaves
 PRUNE_SETTINGS = ((8, 0b1100), (8, 0b1000), (16, 0b1000), (16, 0b1100))
 
@@ -90,7 +91,7 @@ def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
     # a buffer for every record (~17 M of 44 B): the kept set, the sample and the trained pruner are reproducible
     recs, produced, _ = learn.collect_traces(ev, w, M, seeds, CAP, every=512, cap=24_000_000)
     rng = np.random.default_rng(1)
-    train = recs if recs.shape[0] <= 2_000_000 else recs[rng.choice(recs.shape[0], 2_000_000, replace=False)]
+    train = recs if recs.shape[0] <= PRUNE_RECORDS else recs[rng.choice(recs.shape[0], PRUNE_RECORDS, replace=False)]
     pw, losses, _ = learn.train_pruner(ev, train, epochs=PRUNE_EPOCHS, batch=512, lr=1e-3, seed=1)
     act = learn.calibrate_act_range(ev, pw, train)
     setup_s = time.perf_counter() - t0
