@@ -40,6 +40,10 @@ class Counters(C.Structure):
         ("legal", C.c_uint64),
         ("reapplies", C.c_uint64),
         ("placements", C.c_uint64),
+        ("dead_mid", C.c_uint64),
+        ("dead_leaf", C.c_uint64),
+        ("fallback_mid_best", C.c_uint64),
+        ("fallback_leaf_best", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -93,6 +97,8 @@ def lib():
             "ta3o_set_garbage": (None, [C.c_int]),
             "ta3o_search_beam": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, u32p, C.c_int,
                                            ip, ip, ip, f32p, C.POINTER(Counters)]),
+            "ta3o_select_top": (C.c_uint32, [u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
+            "ta3o_set_select_linear": (None, [C.c_int]),
             "ta3o_search_exhaustive": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, C.c_int, ip, ip, ip, f32p]),
             "ta3o_eval_host": (None, [C.c_int, f32p, u64p, C.c_uint32, C.c_uint32, f32p, C.POINTER(Counters)]),
             "ta3o_eval_host_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
@@ -305,6 +311,28 @@ def search_beam(e: Engine, kind: int, w, widths=REF_WIDTHS, counters: Counters |
     if r == 0:
         return None
     return dict(hold=bool(h.value), o=o.value, x=x.value, value=v.value)
+
+
+def select_top(scores, k: int):
+    """select_top (beam.hpp:244-265): (slots, scores, scores after the call with the selected ones zeroed)"""
+    sc = np.array(scores, dtype=np.uint32)
+    slots = np.zeros(max(k, 1), dtype=np.uint32)
+    top = np.zeros(max(k, 1), dtype=np.uint32)
+    n = lib().ta3o_select_top(_p(sc, C.c_uint32), sc.size, k, _p(slots, C.c_uint32), _p(top, C.c_uint32))
+    return slots[:n].tolist(), top[:n].tolist(), sc
+
+
+class select_linear:
+    """context manager: select_top runs the repeated-argmax loop for every beam width (else widths above 64
+    take an equivalent single sort)"""
+
+    def __enter__(self):
+        lib().ta3o_set_select_linear(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().ta3o_set_select_linear(0)
+        return False
 
 
 def search_exhaustive(e: Engine, kind: int, w, depth: int):

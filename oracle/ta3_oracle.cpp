@@ -731,12 +731,14 @@ inline uint64_t pack_key(uint32_t score, uint32_t pos, uint32_t span) { // :217-
 inline uint32_t key_pos(uint64_t key, uint32_t span) { return span - static_cast<uint32_t>(key & 0xFFFFFFFFull); } // :221-223
 
 // select_top (:244-265): repeatedly take the max nonzero score, lowest slot on ties, zero it
-uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlots, uint32_t* outScores) {
-    if(k > 64) {
+std::atomic<int> g_selectLinear{0}; // 1: the argmax loop for every width
+
+uint32_t select_top(uint32_t* scores, uint32_t size, uint32_t k, uint32_t* outSlots, uint32_t* outScores) {
+    if(k > 64 && g_selectLinear.load(std::memory_order_relaxed) == 0) {
         // very wide beams (only the full-width checks use them): the same total order -- score
         // descending, slot ascending, zeros excluded -- by one sort instead of k linear scans
         std::vector<uint32_t> idx;
-        for(uint32_t i = 0; i < scores.size(); ++i)
+        for(uint32_t i = 0; i < size; ++i)
             if(scores[i] != 0) idx.push_back(i);
         std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
             return scores[a] != scores[b] ? scores[a] > scores[b] : a < b;
@@ -752,7 +754,7 @@ uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlot
     uint32_t count = 0;
     for(; count < k; ++count) {
         uint32_t best = 0, bestSlot = 0;
-        for(uint32_t i = 0; i < scores.size(); ++i)
+        for(uint32_t i = 0; i < size; ++i)
             if(scores[i] > best) {
                 best = scores[i];
                 bestSlot = i;
@@ -763,6 +765,9 @@ uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlot
         scores[bestSlot] = 0;
     }
     return count;
+}
+uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlots, uint32_t* outScores) {
+    return select_top(scores.data(), static_cast<uint32_t>(scores.size()), k, outSlots, outScores);
 }
 
 struct Step { Board board; uint32_t clears; bool legal; };
@@ -791,12 +796,17 @@ struct SearchResult { bool some; bool hold; int o, x; float value; };
 
 struct Counters {
     uint64_t moves = 0, applies = 0, legal = 0, reapplies = 0, placements = 0;
+    uint64_t deadMid = 0, deadLeaf = 0, fallbackMidBest = 0, fallbackLeafBest = 0; // dead ends, and moves they decided
     void add(Counters const& c) {
         moves += c.moves;
         applies += c.applies;
         legal += c.legal;
         reapplies += c.reapplies;
         placements += c.placements;
+        deadMid += c.deadMid;
+        deadLeaf += c.deadLeaf;
+        fallbackMidBest += c.fallbackMidBest;
+        fallbackLeafBest += c.fallbackLeafBest;
     }
 };
 
@@ -1064,9 +1074,13 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     if(rootCount == 0) return {false, false, 0, 0, 0.0f};
 
     uint64_t bestKey = 0;
-    auto consider = [&](uint32_t score, uint32_t order) {
+    int bestFallback = 0; // where the best key so far came from (counted, not used): 0 leaf, 1 mid / 2 leaf dead end
+    auto consider = [&](uint32_t score, uint32_t order, int fallback = 0) {
         uint64_t const key = pack_key(score, order, orderCount);
-        bestKey = key > bestKey ? key : bestKey;
+        if(key > bestKey) {
+            bestKey = key;
+            bestFallback = fallback;
+        }
     };
 
     ws.kept[0].resize(kMax);
@@ -1144,7 +1158,10 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
             }
         }
         for(uint32_t k = 0; k < keptCount; ++k) // dead-end kept nodes become depth-`level` leaves (:549-551)
-            if(!ws.hasChild[k]) consider(F.score[k], leafOrderCount + F.rootRank[k]);
+            if(!ws.hasChild[k]) {
+                ++cnt.deadMid;
+                consider(F.score[k], leafOrderCount + F.rootRank[k], 1);
+            }
         // advance_frontier (:379-403)
         Frontier& N = ws.kept[cur ^ 1];
         for(uint32_t j = 0; j < nextCount; ++j) {
@@ -1207,7 +1224,10 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                             Placement const pl = LUT.nth(piece, i);
                             trace.add(F.board[j], static_cast<int>(piece), pl.o, pl.x, leafPly, i == listArg);
                         }
-                if(!expanded) consider(F.score[j], base);
+                if(!expanded) {
+                    ++cnt.deadLeaf;
+                    consider(F.score[j], base, 2);
+                }
             }
         }
     }
@@ -1219,6 +1239,8 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     Node const& rn = tree.layers[0][ws.topRootSlots[rootRank] / MP];
     Placement const pl0 = LUT.nth(slot[rn.slot], ws.topRootSlots[rootRank] % MP);
     ++cnt.moves;
+    if(bestFallback == 1) ++cnt.fallbackMidBest;
+    if(bestFallback == 2) ++cnt.fallbackLeafBest;
     return {true, rn.rootHold, pl0.o, pl0.x, ordered_bits_inv(static_cast<uint32_t>(bestKey >> 32))};
 }
 
@@ -1392,6 +1414,10 @@ void to_counters(Counters const& c, ta3o_counters* out) {
     out->legal += c.legal;
     out->reapplies += c.reapplies;
     out->placements += c.placements;
+    out->dead_mid += c.deadMid;
+    out->dead_leaf += c.deadLeaf;
+    out->fallback_mid_best += c.fallbackMidBest;
+    out->fallback_leaf_best += c.fallbackLeafBest;
 }
 
 Board board_from(uint32_t const cols[10]) {
@@ -1571,6 +1597,12 @@ int ta3o_search_beam(const ta3o_engine* e, int kind, const float* w, const uint3
     *value = r.value;
     return 1;
 }
+
+uint32_t ta3o_select_top(uint32_t* scores, uint32_t n, uint32_t k, uint32_t* out_slots, uint32_t* out_scores) {
+    return select_top(scores, n, k, out_slots, out_scores);
+}
+
+void ta3o_set_select_linear(int enabled) { g_selectLinear.store(enabled ? 1 : 0, std::memory_order_relaxed); }
 
 int ta3o_search_exhaustive(const ta3o_engine* e, int kind, const float* w, int depth, int* hold, int* o, int* x,
                            float* value) {

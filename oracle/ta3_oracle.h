@@ -44,6 +44,10 @@ typedef struct {
     uint64_t legal;       /* candidates that fit and were scored by the model */
     uint64_t reapplies;   /* frontier re-applies (seed/advance) */
     uint64_t placements;  /* engine.place calls */
+    uint64_t dead_mid;    /* kept nodes of a middle level with no legal child in any list (beam.hpp:549-551) */
+    uint64_t dead_leaf;   /* leaf lists with no legal placement (beam.hpp:439-440) */
+    uint64_t fallback_mid_best;  /* moves whose best key is a middle-level dead-end fallback */
+    uint64_t fallback_leaf_best; /* moves whose best key is a leaf-list dead-end fallback */
 } ta3o_counters;
 
 int ta3o_num_params(int kind);
@@ -97,6 +101,13 @@ void ta3o_features(int kind, uint32_t clears, const uint32_t cols[10], int held_
 float ta3o_model_value(int kind, const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i);
 /* the wide network in plain FP32 (fused multiply-adds in dense() order, no bfloat16 rounding) */
 float ta3o_f32w_value_fp32(const float* w, uint32_t clears, const uint32_t cols[10], int held_is_i);
+
+/* select_top (beam.hpp:244-265) over n scores: the k best nonzero scores, highest first, lowest slot on ties;
+ * writes their slots and scores, zeroes them in `scores` and returns how many were selected */
+uint32_t ta3o_select_top(uint32_t* scores, uint32_t n, uint32_t k, uint32_t* out_slots, uint32_t* out_scores);
+/* process-wide option: 1 = select_top always runs the reference's repeated-argmax loop; 0 (default) = beams wider
+ * than 64 take an equivalent single sort (only the full-width checks use such beams) */
+void ta3o_set_select_linear(int enabled);
 
 /* one beamed search move (beam.hpp:483-570) at runtime depth = nwidths + 1.
  * returns 1 and writes the move, or 0 when none. */

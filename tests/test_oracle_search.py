@@ -4,6 +4,11 @@ Known answers and invariants from the reference tests:
   ta3_ai/tests/src/search/search_test.cpp (LUT counts :60-67, naive hold replay :132-229),
   ta3_ai/tests/src/search/beam_test.cpp (tree structure :44-100, keys :103-107,
   full-width beam == exhaustive :113-141, determinism :144-169).
+What the mathematics fixes beyond them:
+  select_top (beam.hpp:244-265) == a stable sort by score descending, slot ascending, zeros dropped;
+  the dead-end fallbacks (beam.hpp:439-440 per leaf list, :549-551 per kept node with no legal child) on
+  near-top-out boards: at depth 2 the full-width beam equals the exhaustive search (search.hpp:117-150), and at
+  depth 3 it equals a brute-force walk of the plan tree that scores every leaf and every dead end.
 """
 import numpy as np
 import pytest
@@ -216,9 +221,10 @@ def test_beam_plays_deterministic_games():  # beam_test.cpp:144-169
     assert play(3) != play(42)
 
 
-def test_beam_move_is_legal_and_value_is_a_leaf_value():
-    """the decoded root move fits the committed board and its value equals the best leaf found by
-    replaying the search at full width restricted to the chosen root (sanity of decode_move)"""
+def test_beam_move_is_legal_and_value_bounded_by_exhaustive():
+    """the decoded root move fits the committed board (sanity of decode_move), and a narrow beam's value is at
+    most the exhaustive search's at the same depth: every key the beam considers is a leaf of the full tree or a
+    dead end that the exhaustive search also scores"""
     rng = np.random.default_rng(8)
     w = rng.uniform(-2, 2, 97).astype(np.float32)
     e = O.engine(4)
@@ -226,6 +232,8 @@ def test_beam_move_is_legal_and_value_is_a_leaf_value():
         r = O.search_beam(e, O.MODEL_V11, w)
         if r is None:
             break
+        narrow = O.search_beam(e, O.MODEL_V11, w, (16, 32))
+        assert narrow["value"] <= O.search_exhaustive(e, O.MODEL_V11, w, 3)["value"]
         piece = e.held if r["hold"] and e.held != 7 else (e.queue[1] if r["hold"] else e.queue[0])
         b = O.engine_board(e)
         legal, _ = b.fit(piece, r["o"], r["x"])
@@ -234,3 +242,163 @@ def test_beam_move_is_legal_and_value_is_a_leaf_value():
             O.engine_hold(e)
         if O.engine_place(e, r["o"], r["x"]) == -1:
             break
+
+
+# --- select_top against a brute-force stable sort (beam.hpp:244-265) ----------------------------------
+@pytest.mark.parametrize("linear", [False, True])
+def test_select_top_matches_stable_sort_with_ties(linear):
+    """the k best nonzero scores, highest first, the LOWEST slot first among equal scores; the selected scores
+    are zeroed. Scores come from a handful of values so that ties are everywhere, at every width the search uses
+    (16, 32) and wider than the argmax loop's range (65, 100)."""
+    rng = np.random.default_rng(2024)
+    for trial in range(300):
+        n = int(rng.choice([1, 3, 34, 68, 100, 1088, 2176]))
+        k = int(rng.choice([1, 2, 16, 32, 64, 65, 100]))
+        scores = rng.choice(np.array([0, 0, 1, 7, 0x80000000, 0xBF800000, 0xFFFFFFFF], dtype=np.uint32), n)
+        want = sorted((i for i in range(n) if scores[i] != 0), key=lambda i: (-int(scores[i]), i))[:k]
+        if linear:
+            with O.select_linear():
+                slots, top, after = O.select_top(scores, k)
+        else:
+            slots, top, after = O.select_top(scores, k)
+        assert slots == want, (trial, n, k)
+        assert top == [int(scores[i]) for i in want]
+        rest = scores.copy()
+        rest[want] = 0
+        assert np.array_equal(after, rest)
+
+
+def test_full_width_argmax_path_matches_exhaustive():
+    """the full-width equality above, with every select_top run as the repeated-argmax loop that the reference
+    widths (<= 64) take"""
+    with O.select_linear():
+        assert play_compare(2, 7, 25) == 25
+        assert play_compare(3, 7, 6) == 6
+
+
+# --- dead ends: near-top-out boards ------------------------------------------------------------------
+FIELD = (1 << 23) - 1
+
+
+def near_top_out(seed, rng):
+    """an engine of seed `seed` (hold used or not) whose board is filled to 1-3 rows below the top, one gap per
+    row (no row is full), so that pieces soon find no room: kept nodes without a legal child and leaf lists
+    without a legal placement occur"""
+    e = O.engine(seed)
+    if rng.integers(0, 2):
+        O.engine_hold(e)
+    cols = [FIELD & ~((1 << int(rng.integers(1, 4))) - 1) for _ in range(10)]
+    for y in range(23):
+        cols[(y * 7 + seed) % 10] &= ~(1 << y)
+    for x in range(10):
+        e.cols[x] = cols[x]
+    e.game_over = 0 if O.Board(cols).available(e.queue[0], 0, 3, 0) else 1
+    return e
+
+
+def _hist_added(h, cleared):  # clear_hist_t::added (model_defs.hpp:108-110)
+    return h if cleared == 0 else h + (1 << (8 * (cleared - 1)))
+
+
+def plan_tree_walk(e, kind, w, depth):
+    """Brute force over the whole plan tree (beam.hpp:69-126) at full width: the best of every leaf's value, of
+    the depth-(depth-2) node's own value for each of its leaf lists with no legal placement (beam.hpp:439-440),
+    and of a shallower node's own value when none of its child lists has a legal placement (beam.hpp:549-551).
+    No selection, no keys, no orders: plain enumeration with the board primitives and the model."""
+    slots = [e.held, e.queue[0]] + [e.queue[1 + i] for i in range(depth)]
+    tree = O.tree(depth, e.held == 7)
+    luts = {p: O.lut(p) for p in range(7)}
+    best = []
+
+    def held_i(n):
+        return n["held_slot"] != 7 and slots[n["held_slot"]] == 0
+
+    def children(board, clears, piece):
+        for o, x in luts[piece]:
+            b = O.Board(board)
+            legal, d = b.fit(piece, o, x)
+            if legal:
+                b.place(piece, o, x, d)
+                cleared = b.clear_lines()  # before the board is read
+                yield list(b.cols), _hist_added(clears, cleared)
+
+    def walk(level, node, board, clears):
+        v = O.model_value(kind, w, clears, board, held_i(node))
+        if level == depth - 1:
+            best.append(v)
+            return
+        any_child = False
+        for ci in range(node["child_begin"], node["child_end"]):
+            child = tree[level + 1][ci]
+            list_has = False
+            for b2, c2 in children(board, clears, slots[child["slot"]]):
+                list_has = any_child = True
+                walk(level + 1, child, b2, c2)
+            if level + 1 == depth - 1 and not list_has:
+                best.append(v)
+        if level + 1 < depth - 1 and not any_child:
+            best.append(v)
+
+    for root in tree[0]:
+        for b2, c2 in children(list(e.cols), 0, slots[root["slot"]]):
+            walk(0, root, b2, c2)
+    return max(best) if best else None
+
+
+def _dead_end_cases(n):
+    rng = np.random.default_rng(77)
+    w = rng.uniform(-2, 2, 97).astype(np.float32)
+    for seed in range(n):
+        e = near_top_out(seed, rng)
+        if not e.game_over:
+            yield e, O.MODEL_HEIGHT, []
+            yield e, O.MODEL_V11, w
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_leaf_dead_ends_full_width_depth2_equal_exhaustive(linear):
+    """at depth 2 (roots, then leaves) the exhaustive search scores a root whose next piece does not fit exactly
+    as the beam's leaf-list fallback does, so the values agree on boards where that decides the move"""
+    total = O.Counters()
+    cases = 0
+    for e, kind, w in _dead_end_cases(60):
+        ref = O.search_exhaustive(e, kind, w, 2)
+        if linear:
+            with O.select_linear():
+                beam = O.search_beam(e, kind, w, (68,), total)
+        else:
+            beam = O.search_beam(e, kind, w, (68,), total)
+        assert (ref is None) == (beam is None)
+        if ref is not None:
+            assert beam["value"] == ref["value"]
+            cases += 1
+    assert cases >= 40
+    assert total.dead_leaf > 0 and total.fallback_leaf_best >= 5  # the fallback decided moves
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_dead_ends_full_width_depth3_equal_plan_tree_walk(linear):
+    """at depth 3 the full-width beam equals the brute-force plan-tree walk on near-top-out boards where both
+    fallbacks (kept node without a legal child, leaf list without a legal placement) decide moves. The exhaustive
+    search is no judge here: it scores a node whenever ONE of its plans dead-ends, the beam only when all of its
+    child lists do (beam.hpp:549-551), and on some of these boards that changes the value."""
+    total = O.Counters()
+    cases = differ = 0
+    for e, kind, w in _dead_end_cases(45):
+        brute = plan_tree_walk(e, kind, w, 3)
+        if linear:
+            with O.select_linear():
+                beam = O.search_beam(e, kind, w, (68, 68 * 68), total)
+        else:
+            beam = O.search_beam(e, kind, w, (68, 68 * 68), total)
+        assert (brute is None) == (beam is None)
+        if brute is None:
+            continue
+        assert beam["value"] == brute
+        cases += 1
+        ref = O.search_exhaustive(e, kind, w, 3)
+        differ += ref["value"] != brute
+    assert cases >= 30
+    assert total.dead_mid > 0 and total.fallback_mid_best >= 2
+    assert total.dead_leaf > 0 and total.fallback_leaf_best >= 2
+    assert differ >= 1  # the reference's per-node rule is visible on these boards
