@@ -131,7 +131,8 @@ struct ta3h_evaluator {
     uint32_t pruneK = 34, pruneMask = 0;
     // full-beam traces
     uint32_t* dTrace = nullptr;
-    size_t traceCap = 0; // records
+    size_t traceWords = 0; // capacity of dTrace in 32-bit words
+    size_t traceCap = 0;   // records the launches keep (set_trace's cap_records; traceCap * 11 <= traceWords)
     uint32_t* dTraceCount = nullptr;
     uint64_t traceRecords = 0; // records the trace buffer holds after a launch (0 = none requested)
     uint32_t traceEvery = 0;   // 0 = off
@@ -166,6 +167,11 @@ struct ta3h_evaluator {
     size_t hFitnessCap = 0;
     uint32_t* hPlacements = nullptr;
     size_t hPlacementsCap = 0;
+    uint64_t* hSeeds = nullptr; // pinned staging of the seeds
+    size_t hSeedsCap = 0;
+    // copies from the pinned staging buffers may still be in flight (no stream synchronisation since they were
+    // enqueued): the next stage waits before it rewrites the buffers
+    bool stagingInFlight = false;
     uint32_t graphModels = 0;
 
     int kind = TA3H_MODEL_V11;
@@ -255,6 +261,7 @@ struct ta3h_evaluator {
         if(hWeights) static_cast<void>(hipHostFree(hWeights));
         if(hFitness) static_cast<void>(hipHostFree(hFitness));
         if(hPlacements) static_cast<void>(hipHostFree(hPlacements));
+        if(hSeeds) static_cast<void>(hipHostFree(hSeeds));
         if(dWeights) static_cast<void>(hipFree(dWeights));
         if(dSeeds) static_cast<void>(hipFree(dSeeds));
         if(dFitness) static_cast<void>(hipFree(dFitness));
@@ -344,6 +351,11 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
     }
     int r = ev->ensure_ready();
     if(r) return r;
+    hipError_t e;
+    if(ev->stagingInFlight) { // a stage without a synchronisation since: its copies may still read the buffers
+        if((e = hipStreamSynchronize(ev->stream)) != hipSuccess) return ev->fail("stage", e);
+        ev->stagingInFlight = false;
+    }
     uint32_t const stride = stride_of(ev->kind);
     size_t const wCount = static_cast<size_t>(num_models) * stride;
     if((r = ev->grow(ev->dWeights, ev->weightsCap, wCount, "malloc weights"))) return r;
@@ -352,15 +364,17 @@ int ta3h_stage(ta3h_evaluator* ev, const float* weights, uint32_t num_models, ui
     if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
     if((r = ev->grow(ev->dCounters, ev->countersCap, 3 * items, "malloc counters"))) return r;
     if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
-    ev->packed.resize(wCount);
+    // inputs go through pinned staging buffers, so the two copies are asynchronous (model_eval.cu:123-134)
+    if((r = ev->grow_pinned(ev->hWeights, ev->hWeightsCap, wCount, "pinned weights"))) return r;
+    if((r = ev->grow_pinned(ev->hSeeds, ev->hSeedsCap, games, "pinned seeds"))) return r;
     for(uint32_t m = 0; m < num_models; ++m)
-        repack(ev->kind, weights + static_cast<size_t>(m) * num_params, ev->packed.data() + static_cast<size_t>(m) * stride);
-    hipError_t e = hipMemcpyAsync(ev->dWeights, ev->packed.data(), wCount * sizeof(float), hipMemcpyHostToDevice, ev->stream);
+        repack(ev->kind, weights + static_cast<size_t>(m) * num_params, ev->hWeights + static_cast<size_t>(m) * stride);
+    std::memcpy(ev->hSeeds, seeds, games * sizeof(uint64_t));
+    e = hipMemcpyAsync(ev->dWeights, ev->hWeights, wCount * sizeof(float), hipMemcpyHostToDevice, ev->stream);
     if(e != hipSuccess) return ev->fail("copy weights", e);
-    e = hipMemcpyAsync(ev->dSeeds, seeds, games * sizeof(uint64_t), hipMemcpyHostToDevice, ev->stream);
+    ev->stagingInFlight = true;
+    e = hipMemcpyAsync(ev->dSeeds, ev->hSeeds, games * sizeof(uint64_t), hipMemcpyHostToDevice, ev->stream);
     if(e != hipSuccess) return ev->fail("copy seeds", e);
-    e = hipStreamSynchronize(ev->stream); // the host staging vector may be reused by the next stage
-    if(e != hipSuccess) return ev->fail("stage", e);
     ev->staged = true;
     return TA3H_OK;
 }
@@ -495,6 +509,7 @@ int ta3h_sync(ta3h_evaluator* ev) {
     if(!ev->ready) return TA3H_OK;
     hipError_t const e = hipStreamSynchronize(ev->stream);
     if(e != hipSuccess) return ev->fail("kernel run", e);
+    ev->stagingInFlight = false;
     return TA3H_OK;
 }
 
@@ -517,6 +532,7 @@ int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements)
     }
     e = hipStreamSynchronize(ev->stream);
     if(e != hipSuccess) return ev->fail("kernel run", e);
+    ev->stagingInFlight = false;
     return TA3H_OK;
 }
 
@@ -586,12 +602,24 @@ int ta3h_fetch_timeline(ta3h_evaluator* ev, uint64_t* out, double* ticks_per_sec
     return TA3H_OK;
 }
 
+// As the reference's eval (model_eval.cu:97-153): two asynchronous input copies, the search, one result copy,
+// then exactly one host synchronisation.
 int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uint32_t num_params,
               const uint64_t* seeds, uint32_t games, uint32_t max_moves, float* out_fitness) {
     int r = ta3h_stage(ev, weights, num_models, num_params, seeds, games, max_moves);
     if(r) return r;
     if((r = ta3h_launch(ev))) return r;
-    return ta3h_fetch(ev, out_fitness, nullptr);
+    if(ev->items == 0) return TA3H_OK;
+    if(!out_fitness) return ev->fail_arg("eval: null output");
+    if((r = ev->grow_pinned(ev->hFitness, ev->hFitnessCap, ev->items, "pinned fitness"))) return r;
+    hipError_t e = hipMemcpyAsync(ev->hFitness, ev->dFitness, ev->items * sizeof(float), hipMemcpyDeviceToHost,
+                                  ev->stream);
+    if(e != hipSuccess) return ev->fail("copy fitness", e);
+    e = hipStreamSynchronize(ev->stream); // the one host synchronisation of the call
+    if(e != hipSuccess) return ev->fail("kernel run", e);
+    ev->stagingInFlight = false;
+    std::memcpy(out_fitness, ev->hFitness, ev->items * sizeof(float));
+    return TA3H_OK;
 }
 
 int ta3h_last_kernel_ms(ta3h_evaluator* ev, float* ms) {
@@ -689,11 +717,16 @@ int ta3h_graph_prepare(ta3h_evaluator* ev, uint32_t num_models, uint32_t num_par
     if((r = ev->grow(ev->dPlacements, ev->placementsCap, items, "malloc placements"))) return r;
     if((r = ev->grow(ev->dCounters, ev->countersCap, 3 * items, "malloc counters"))) return r;
     if(ev->timelineOn && (r = ev->grow(ev->dTimeline, ev->timelineCap, 2 * items, "malloc timeline"))) return r;
+    hipError_t e;
+    if(ev->stagingInFlight) { // copies of an earlier stage may still read the pinned weights
+        if((e = hipStreamSynchronize(ev->stream)) != hipSuccess) return ev->fail("stage", e);
+        ev->stagingInFlight = false;
+    }
     if((r = ev->grow_pinned(ev->hWeights, ev->hWeightsCap, wCount, "pinned weights"))) return r;
     if((r = ev->grow_pinned(ev->hFitness, ev->hFitnessCap, items, "pinned fitness"))) return r;
     if((r = ev->grow_pinned(ev->hPlacements, ev->hPlacementsCap, items, "pinned placements"))) return r;
     std::memset(ev->hWeights, 0, wCount * sizeof(float)); // padding floats stay zero
-    hipError_t e = hipMemcpyAsync(ev->dSeeds, seeds, games * sizeof(uint64_t), hipMemcpyHostToDevice, ev->stream);
+    e = hipMemcpyAsync(ev->dSeeds, seeds, games * sizeof(uint64_t), hipMemcpyHostToDevice, ev->stream);
     if(e == hipSuccess) e = hipStreamSynchronize(ev->stream);
     if(e != hipSuccess) return ev->fail("copy seeds", e);
     ev->items = static_cast<uint32_t>(items);
@@ -805,7 +838,7 @@ int ta3h_set_trace(ta3h_evaluator* ev, uint64_t cap_records, uint32_t every) {
     if(cap_records > 0xFFFFFFFFull) return ev->fail_arg("set_trace: at most 2^32 - 1 records");
     int r = ev->ensure_ready();
     if(r) return r;
-    if((r = ev->grow(ev->dTrace, ev->traceCap, static_cast<size_t>(cap_records) * 11, "malloc trace"))) return r;
+    if((r = ev->grow(ev->dTrace, ev->traceWords, static_cast<size_t>(cap_records) * 11, "malloc trace"))) return r;
     ev->traceCap = static_cast<size_t>(cap_records); // records (the buffer may be larger)
     if(ev->dTraceCount == nullptr) {
         hipError_t const e = hipMalloc(&ev->dTraceCount, 64);
