@@ -97,6 +97,8 @@ def lib():
             "ta3o_set_garbage": (None, [C.c_int]),
             "ta3o_search_beam": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, u32p, C.c_int,
                                            ip, ip, ip, f32p, C.POINTER(Counters)]),
+            "ta3o_search_beam_pruned": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, f32p, C.c_float, C.c_int, C.c_uint,
+                                                  ip, ip, ip, f32p]),
             "ta3o_select_top": (C.c_uint32, [u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
             "ta3o_set_select_linear": (None, [C.c_int]),
             "ta3o_search_exhaustive": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, C.c_int, ip, ip, ip, f32p]),
@@ -308,6 +310,19 @@ def search_beam(e: Engine, kind: int, w, widths=REF_WIDTHS, counters: Counters |
                                C.byref(h), C.byref(o), C.byref(x), C.byref(v), C.byref(cnt))
     if r < 0:
         raise ValueError("bad beam widths")
+    if r == 0:
+        return None
+    return dict(hold=bool(h.value), o=o.value, x=x.value, value=v.value)
+
+
+def search_beam_pruned(e: Engine, kind: int, w, pruner_w, act_range: float, k: int, ply_mask: int):
+    """one pruned search move at the reference widths (None when there is no move)"""
+    w = _f32(w if len(w) else np.zeros(1))
+    pw = _f32(pruner_w)
+    h, o, x = C.c_int(0), C.c_int(0), C.c_int(0)
+    v = C.c_float(0)
+    r = lib().ta3o_search_beam_pruned(C.byref(e), kind, _p(w, C.c_float), _p(pw, C.c_float), float(act_range), int(k),
+                                      int(ply_mask), C.byref(h), C.byref(o), C.byref(x), C.byref(v))
     if r == 0:
         return None
     return dict(hold=bool(h.value), o=o.value, x=x.value, value=v.value)

@@ -1598,6 +1598,28 @@ int ta3o_search_beam(const ta3o_engine* e, int kind, const float* w, const uint3
     return 1;
 }
 
+int ta3o_search_beam_pruned(const ta3o_engine* e, int kind, const float* w, const float* pruner_w, float act_range,
+                            int k, unsigned ply_mask, int* hold, int* o, int* x, float* value) {
+    std::vector<uint32_t> const ks{16u, 32u, 32u};
+    Engine const g = from_c(e);
+    ModelRef const m{kind, w};
+    auto q = std::make_unique<PrunerQ>();
+    pruner_quantize(pruner_w, act_range, *q);
+    PrunerRef pr;
+    pr.q = q.get();
+    pr.k = k;
+    pr.plyMask = ply_mask;
+    BeamWorkspace ws;
+    Counters c;
+    SearchResult const r = search_beam(g, m, ks, ws, c, pr);
+    if(!r.some) return 0;
+    *hold = r.hold ? 1 : 0;
+    *o = r.o;
+    *x = r.x;
+    *value = r.value;
+    return 1;
+}
+
 uint32_t ta3o_select_top(uint32_t* scores, uint32_t n, uint32_t k, uint32_t* out_slots, uint32_t* out_scores) {
     return select_top(scores, n, k, out_slots, out_scores);
 }

@@ -144,6 +144,10 @@ void ta3o_pruner_quant_info(const float* w, float act_range, int* info);
  * rounded = apply the inference bfloat16 roundings (straight-through) */
 double ta3o_pruner_loss_grad(const float* w, const uint32_t* samples, int n, double pos_weight, int rounded,
                              double* grad);
+/* one pruned beamed search move at the reference widths {16, 32, 32} (what eval_host_pruned_mt plays per move);
+ * returns 1 and writes the move, or 0 when none */
+int ta3o_search_beam_pruned(const ta3o_engine* e, int kind, const float* w, const float* pruner_w, float act_range,
+                            int k, unsigned ply_mask, int* hold, int* o, int* x, float* value);
 /* eval_host_mt with the pruned beam: at plies in ply_mask (bit 1, 2: middle levels, bit 3: leaves) only
  * the pruner's top-k legal placements of every list are scored by the value network */
 void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, const uint64_t* seeds, uint32_t games,
