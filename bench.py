@@ -4,20 +4,24 @@ Workload (one "step" = one generation): 4 islands x 50 population x 50 games = 1
 simulations, depth-4 beam search with widths 16/32/32 over every placement, games capped at 2,000
 pieces, value network 16 -> 32 -> 32 -> 1 in FP32. Parameter vectors are sampled per island like a
 fresh CMA-ES generation (island mean ~ N(0, 0.5^2), sigma 0.3, identity covariance). Game seeds come
-from simulate()'s seed expansion. Inputs are staged into device memory before the timed region.
+from simulate()'s seed expansion. Inputs are staged into device memory before the timed region; the launch
+shape is autotuned on the batch first (untimed; results never depend on it).
 
-Prints one JSON line. `value` is the full beam's rate over the K timed generations. `pruned` reports the same
-generation searched with the learned pruner: the pruner is trained first (untimed) on full-beam traces of this
-generation, then the pruned generation is timed at each (k, plies) setting of PRUNE_SETTINGS and its mean
-score compared with the full beam's (the <=1% score-loss bar); `pruned` holds the fastest setting within the
-bar, or the one with the smallest loss, and `pruned.settings` all of them. `roofline` prices the dominant kernel's
-algorithmic integer/FP32 VALU work (exact per-item candidate counts from the kernel x a fixed per-candidate op
-model) against the MI355X VALU peak; `traffic` is the profiled HBM bytes per launch when a PMC summary for the
-workload is committed under profiles/. `cpu_baseline` times the CPU oracle on a bounded sample of the same
-workload on the host cores.
+Prints one JSON line. `value` is the full beam's rate over the K timed generations. `pruned` reports the learned
+pruner: trained (untimed) on full-beam traces of this generation, then judged on HELD-OUT game seeds -- the same
+population on a new seed set -- against the full beam there, at each setting of PRUNE_SPECS; beside it the
+non-learned way to spend fewer value evaluations, narrower beams, and a narrower beam with the pruner. `pruned`
+holds the fastest learned setting within the <=1% mean-score bar (else the one with the smallest loss),
+`pruned.narrow_best` the same for narrower beams, `pruned.champion` the same study on a trained 16-32-32-1
+champion (its own pruner from its own games on training seeds, judged on 1,000 held-out games). `roofline` prices
+the dominant kernel's algorithmic integer/FP32 VALU work (exact per-item candidate counts from the kernel x a
+fixed per-candidate op model) against the MI355X VALU peak; `traffic` is the profiled HBM bytes per launch when a
+PMC summary for the workload is committed under profiles/. `cpu_baseline` times the CPU oracle on a fixed sample
+of the same workload (items spread over the generation) on all host cores, and a smaller sample on one thread.
 
 `--workload wide` measures the scaled configuration instead: 256 models x 100 games, cap 5,000 pieces,
-32-128-128-1 network on BF16 MFMA, garbage rows.
+32-128-128-1 network on BF16 MFMA, garbage rows, with its own pruned leg (pruner trained on that generation's
+traces, judged on held-out seeds).
 """
 from __future__ import annotations
 
@@ -36,20 +40,29 @@ ISLANDS, POP, GAMES, CAP = 4, 50, 50, 2000
 GEN_SEED = 0x7E7215
 WIDE_MODELS, WIDE_GAMES, WIDE_CAP = 256, 100, 5000
 
-# Algorithmic VALU lane-ops per candidate placement of the reference algorithm (10-column board loops):
-C_FIT = 120                 # spawn collision + drop distance over 10 columns
-C_PLACE_CLEAR = 45          # lock-in, full-row mask, compaction
-C_FEAT = {0: 355, 1: 425}   # fused column metrics + normalisation (V11 / 16-feature set)
-# network: V11 = 88 multiply-adds as separate mul + add (reference rounding) + 8 ReLU;
-# 16-32-32-1 = 1,633 fused multiply-adds + 64 ReLU
-C_MLP = {0: 184, 1: 1697}
+# Algorithmic VALU lane-ops per candidate placement, frozen from the instrumented scalar restatement of one
+# candidate (oracle/opcount.cpp; tests/test_oracle_opcount.py re-derives them): every scalar operation of fit
+# (over the columns the piece occupies), place + clear, the fused feature pass and the dense network, averaged over
+# the legal candidates of full-beam games (fit and place + clear depend on the data, the rest does not).
+C_FIT = 54                   # spawn collision + drop distance, per fitted candidate
+C_PLACE_CLEAR = {0: 33, 1: 32}  # lock-in, full-row mask, row compaction, histogram bump (V11 / 16-32-32-1 games)
+C_FEAT = {0: 470, 1: 611}    # fused column metrics + normalisation (V11 / 16-feature set)
+# network: V11 = 88 multiply-adds as separate mul + add (reference rounding) + 8 ReLU = 184;
+# 16-32-32-1 = 1,568 fused multiply-adds + 64 ReLU = 1,632 (every multiply-add of the network, including those the
+# kernel skips exactly when an input is zero in every lane of a chunk)
+C_MLP = {0: 184, 1: 1632}
 VALU_PEAK_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz (non-packed 32-bit VALU ops)
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 F32W_FLOPS = 2 * 20865      # per leaf evaluation of the wide network (on MFMA)
 PRUNE_EPOCHS = 16
 PRUNE_RECORDS = 8_000_000  # training sample of the generation's ~17 M trace records
-# (k, ply mask) settings of the pruned leg: mask bits 1, 2 = middle levels, 3 = leaves
-PRUNE_SETTINGS = ((8, 0b1100), (8, 0b1000), (16, 0b1000), (16, 0b1100))
+HELDOUT_SEED = GEN_SEED + 1  # the pruner is judged on games of seeds it never saw
+# settings of the pruned leg: p<k>:<mask> = learned pruner keeps k of every list at the plies of mask (bits 1, 2 =
+# middle levels, 3 = leaves); n<k0>-<k1>-<k2> = narrower beam (no pruner); both joined by +
+PRUNE_SPECS = ("p16:8", "p8:8", "p8:12", "p4:12", "n12-24-24", "n8-16-16", "n4-8-8", "n8-16-16+p8:8")
+CHAMP_PATH = os.path.join(ROOT, "tests", "golden", "f16_champion.champ")  # a trained 16-32-32-1 model
+CHAMP_TRAIN_SEED, CHAMP_HELDOUT_SEED, CHAMP_GAMES = 0x5EED, 0x5EEE, 1000
+CPU_SAMPLE = 128  # items of the generation the CPU baseline plays, evenly spaced through it (model-major)
 
 
 def workload(model_kind: int):
@@ -81,77 +94,119 @@ def traffic_from_profiles(name: str):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def pruned_leg(T, kind, w, M, seeds, full_fit, full_pl, full_ms, steps):
-    """Trains the pruner on this generation's full-beam traces (untimed), then times the pruned generation at
-    each (k, ply mask) of PRUNE_SETTINGS. Reports every setting and picks the fastest one within 1% mean-score
-    loss of the full beam, or the one with the smallest loss when none is."""
-    from tetris_ai3_amd import learn
-    ev = T.ModelEvaluator(model=kind)
-    t0 = time.perf_counter()
-    # a buffer for every record (~17 M of 44 B): the kept set, the sample and the trained pruner are reproducible
-    recs, produced, _ = learn.collect_traces(ev, w, M, seeds, CAP, every=512, cap=24_000_000)
-    rng = np.random.default_rng(1)
-    train = recs if recs.shape[0] <= PRUNE_RECORDS else recs[rng.choice(recs.shape[0], PRUNE_RECORDS, replace=False)]
-    pw, losses, _ = learn.train_pruner(ev, train, epochs=PRUNE_EPOCHS, batch=512, lr=1e-3, seed=1)
-    act = learn.calibrate_act_range(ev, pw, train)
-    setup_s = time.perf_counter() - t0
-    full_rate = float(full_pl.sum()) / (full_ms / 1000.0)
-    settings = []
-    for k, mask in PRUNE_SETTINGS:
-        ev.set_pruner(pw, act, k=k, ply_mask=mask)
-        ev.stage(w, M, seeds, CAP)
-        ev.launch()
-        ev.sync()
-        t1 = time.perf_counter()
-        for _ in range(steps):
-            ev.launch()
-            ev.sync()
-        elapsed = time.perf_counter() - t1
-        fit, pl = ev.fetch()
-        cnt = ev.fetch_counters()
-        rate = float(pl.sum()) * steps / elapsed
-        loss = 1.0 - float(fit.mean()) / float(full_fit.mean()) if full_fit.mean() > 0 else \
-            float(full_fit.mean() - fit.mean()) / abs(float(full_fit.mean()))
-        settings.append({"k": k, "ply_mask": mask, "placements_per_s": rate, "ms_per_step": elapsed * 1000 / steps,
-                         "speedup_vs_full": rate / full_rate, "mean_score": float(fit.mean()),
-                         "mean_score_loss_pct": 100.0 * loss, "meets_1pct": bool(loss <= 0.01),
-                         "value_evals_per_placement": float(cnt[:, 1].sum()) / max(float(pl.sum()), 1.0)})
-    ev.close()
-    ok = [x for x in settings if x["meets_1pct"]]
-    best = max(ok, key=lambda x: x["placements_per_s"]) if ok else min(settings, key=lambda x: x["mean_score_loss_pct"])
-    out = dict(best)
-    out.update({"settings": settings,
-                "trained_on": f"{train.shape[0]} of {produced} full-beam trace records of this generation "
-                              f"(1 move in 512 per game{'' if recs.shape[0] == produced else '; buffer full'}), "
-                              f"{PRUNE_EPOCHS} epochs",
-                "training_loss": [float(x) for x in losses],
-                "setup_s": setup_s})
+def _best(rows):
+    """the fastest row within the 1% bar, else the one with the smallest loss"""
+    ok = [r for r in rows if r["meets_1pct"]]
+    return max(ok, key=lambda r: r["placements_per_s"]) if ok else min(rows, key=lambda r: r["mean_score_loss_pct"])
+
+
+def _summary(rows, note):
+    full, rest = rows[0], rows[1:]
+    learned = [r for r in rest if r["k"] and r["widths"] == [16, 32, 32]]
+    narrow = [r for r in rest if not r["k"]]
+    out = dict(_best(learned)) if learned else {}
+    out.update({"judged_on": note, "full_beam": full, "settings": rest})
+    if narrow:
+        out["narrow_best"] = _best(narrow)
     return out
 
 
-def cpu_baseline(model_kind: int, w, num_models: int, seeds, budget_s: float):
+def pruned_leg(T, kind, w, M, seeds, cap, steps, every=512, cap_records=24_000_000, heldout_seed=HELDOUT_SEED):
+    """Pruner trained on this generation's full-beam traces (untimed); every setting judged against the full beam
+    on the same population playing held-out seeds."""
+    from tetris_ai3_amd import learn
+    ev = T.ModelEvaluator(model=kind)
+    if kind == T.MODEL_F32W:
+        ev.set_garbage(True)
+    pw, act, info = learn.fit_pruner(ev, w, M, seeds, cap, every=every, cap_records=cap_records,
+                                     records=PRUNE_RECORDS, epochs=PRUNE_EPOCHS)
+    held = T.expand_seeds(heldout_seed, len(seeds))
+    rows = learn.compare_settings(ev, w, M, held, cap, PRUNE_SPECS, pw, act, steps)
+    ev.close()
+    out = _summary(rows, f"the same {M} models on {len(seeds)} held-out seeds (simulate seed {heldout_seed:#x})")
+    out["trained_on"] = (f"{info['trained_on']} of {info['trace_records']} full-beam trace records of this "
+                         f"generation (1 move in {every} per game), {info['epochs']} epochs")
+    out["training_loss"] = info["training_loss"]
+    out["setup_s"] = info["setup_s"]
+    return out
+
+
+def champion_leg(T, steps):
+    """The same study on a trained 16-32-32-1 champion: its pruner trained on its own full-beam games on training
+    seeds, every setting judged on 1,000 held-out games."""
+    if not os.path.exists(CHAMP_PATH):
+        return None
+    from tetris_ai3_amd import learn
+    w = T.champ_read(CHAMP_PATH).astype(np.float32)
+    ev = T.ModelEvaluator(model=T.MODEL_F16)
+    pw, act, info = learn.fit_pruner(ev, w, 1, T.expand_seeds(CHAMP_TRAIN_SEED, 200), CAP, every=64,
+                                     cap_records=16_000_000, records=4_000_000, epochs=10)
+    rows = learn.compare_settings(ev, w, 1, T.expand_seeds(CHAMP_HELDOUT_SEED, CHAMP_GAMES), CAP, PRUNE_SPECS, pw,
+                                  act, steps)
+    ev.close()
+    out = _summary(rows, f"{CHAMP_GAMES} held-out games (simulate seed {CHAMP_HELDOUT_SEED:#x}) of "
+                         f"{os.path.relpath(CHAMP_PATH, ROOT)}")
+    out["trained_on"] = (f"{info['trained_on']} of {info['trace_records']} trace records of 200 training games "
+                         f"(1 move in 64), {info['epochs']} epochs")
+    return out
+
+
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, cores
+
+
+def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_pl):
+    """The oracle (this build's scalar C++ restatement of eval_host, model_eval_host.cpp:26-54) on CPU_SAMPLE items
+    spread evenly over the generation, one item per task on every host core, each played to completion; then
+    the first of those items (about 2,500 placements by the GPU's count) on one thread, as eval_host runs. The
+    sampled items' fitness must equal the GPU's bit for bit."""
     import oracle as O
-    P = O.num_params(model_kind)
-    threads = min(os.cpu_count() or 1, 16)
-    items = threads
+    from concurrent.futures import ThreadPoolExecutor
+    P, G = O.num_params(model_kind), len(seeds)
+    items = num_models * G
+    sample = [(k * items) // CPU_SAMPLE for k in range(CPU_SAMPLE)]
+
+    def one(i):
+        m, g = divmod(i, G)
+        f, p = O.eval_host_mt(model_kind, w[m * P:(m + 1) * P], 1, seeds[g:g + 1], cap, threads=1)
+        return f[0], int(p[0])
+
+    model, cores = cpu_info()
     t0 = time.perf_counter()
-    # first `items` (model, game) items of the generation, played to completion: model-major order
-    fit_all, pl_all = [], []
-    done = 0
-    while done < items:
-        m, g = divmod(done, len(seeds))
-        n = min(items - done, len(seeds) - g)
-        f, pl = O.eval_host_mt(model_kind, w[m * P:(m + 1) * P], 1, seeds[g:g + n], CAP, threads=threads)
-        fit_all.append(f)
-        pl_all.append(pl)
-        done += n
-        if time.perf_counter() - t0 > budget_s:
-            break
+    with ThreadPoolExecutor(cores) as pool:
+        res = list(pool.map(one, sample))
     dt = time.perf_counter() - t0
-    placements = int(np.concatenate(pl_all).sum())
-    return {"value": placements / dt, "unit": "placements/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} (model, game) items of the generation played to completion on {threads} "
-                      f"threads ({placements} placements, {dt:.1f} s)"}, np.concatenate(fit_all), done
+    placements = sum(p for _, p in res)
+    single, acc = [], 0
+    for i in sample:
+        if single and acc + int(gpu_pl[i]) > 2500:
+            break
+        single.append(i)
+        acc += int(gpu_pl[i])
+    t1 = time.perf_counter()
+    res1 = [one(i) for i in single]
+    dt1 = time.perf_counter() - t1
+    p1 = sum(p for _, p in res1)
+    fit = np.array([f for f, _ in res], dtype=np.float32)
+    ok = fit.tobytes() == gpu_fit[sample].tobytes() and [p for _, p in res] == [int(gpu_pl[i]) for i in sample]
+    return {"value": placements / dt, "unit": "placements/s", "cores": cores, "kind": "port",
+            "sample": f"{CPU_SAMPLE} (model, game) items evenly spaced over the generation (item k * {items} / "
+                      f"{CPU_SAMPLE}), each played to completion, one per task on {cores} threads "
+                      f"({placements} placements, {dt:.1f} s)",
+            "one_thread": {"value": p1 / dt1, "unit": "placements/s",
+                           "sample": f"the first {len(single)} of those items on 1 thread ({p1} placements, {dt1:.1f} s)"},
+            "nproc": os.cpu_count(), "cpu_model": model,
+            "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}, len(sample), ok
 
 
 def main():
@@ -165,9 +220,10 @@ def main():
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="copy 1 GiB device-to-device first: a kernel of known traffic to scale FETCH_SIZE / "
                          "WRITE_SIZE by in the same profiled run")
-    ap.add_argument("--waves-per-game", type=int, default=0, help="0: the network's tuned default")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--waves-per-game", type=int, default=0,
+                    help="fixed waves per game (1, 2, 4) instead of the autotuned launch shape")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prune-steps", type=int, default=3, help="timed launches per pruned-leg setting")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build evaluates on a single MI355X; run with --gpus 1",
@@ -193,9 +249,12 @@ def main():
     ev = T.ModelEvaluator(model=kind)
     if wide:
         ev.set_garbage(True)
+    tuned = None
     if args.waves_per_game:
         ev.set_launch(args.waves_per_game, True, 0)
-    ev.stage(w, M, seeds, cap)
+        ev.stage(w, M, seeds, cap)
+    else:  # launch shape and first-phase budget tuned on this batch (untimed; results never depend on them)
+        tuned = T.autotune(ev, w, M, seeds, cap, candidates=((2, 0), (4, 0)), phases=(None, 256), use_cache=False)
 
     for _ in range(args.warmup):
         ev.launch()
@@ -225,7 +284,7 @@ def main():
                 "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic_from_profiles("traffic_f32w_wide"),
                 "flops_per_launch": ops}
     else:
-        ops = applies * C_FIT + legal * (C_PLACE_CLEAR + C_FEAT[kind] + C_MLP[kind])
+        ops = applies * C_FIT + legal * (C_PLACE_CLEAR[kind] + C_FEAT[kind] + C_MLP[kind])
         achieved = ops / (kms / 1000.0) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                 "frac": achieved / VALU_PEAK_TOPS,
@@ -265,17 +324,25 @@ def main():
             "candidates_per_placement": applies / max(placements, 1),
             "lane_utilisation": legal / max(lanes, 1),
             "launch": info,
+            "autotune": tuned,
             "kernel_ms": kms,
         },
         "roofline": roof,
     }
-    if not wide and not args.no_pruned:
-        out["pruned"] = pruned_leg(T, kind, w, M, seeds, fit, pl, ms_per_step, args.steps)
+    if not args.no_pruned:
+        if wide:
+            out["pruned"] = pruned_leg(T, kind, w, M, seeds, cap, args.prune_steps, every=1024,
+                                       heldout_seed=GEN_SEED + 5)
+        else:
+            out["pruned"] = pruned_leg(T, kind, w, M, seeds, cap, args.prune_steps)
+            champ = champion_leg(T, args.prune_steps) if kind == T.MODEL_F16 else None
+            if champ is not None:
+                out["pruned"]["champion"] = champ
     if not args.no_cpu and not wide:
-        base, cpu_fit, n = cpu_baseline(kind, w, M, seeds, args.cpu_budget)
+        base, n, ok = cpu_baseline(kind, w, M, seeds, cap, fit, pl)
         out["cpu_baseline"] = base
         out["cpu_parity_items"] = n
-        out["cpu_parity_ok"] = bool(cpu_fit.tobytes() == fit[:n].tobytes())
+        out["cpu_parity_ok"] = bool(ok)
     print(json.dumps(out), flush=True)
 
 

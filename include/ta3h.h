@@ -69,6 +69,11 @@ int ta3h_set_phases(ta3h_evaluator* ev, uint32_t first_phase_moves);
  * build-defined start-position extension). */
 int ta3h_set_garbage(ta3h_evaluator* ev, int enabled);
 
+/* Beam widths of the following launches: each level keeps its k0 / k1 / k2 best nodes (1..16, 1..32, 1..32).
+ * The reference fixes them at compile time to {16, 32, 32} (ta3_ai/incl/ta3/ai/search/variation_sequences.hpp:22),
+ * the default here; narrower beams give the moves of the reference's search built with those widths. */
+int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t k2);
+
 /* Launch shape: waves cooperating on one game (1, 2 or 4; a workgroup plays one game at a time),
  * persistent work-queue mode (1) or one workgroup per item (0), and the persistent grid in workgroups
  * (0 = fill the device at the occupancy limit). Results never depend on these knobs. */

@@ -75,7 +75,7 @@ private:
 
 namespace detail {
     // xoshiro256** seeded through splitmix64, the stream gpu::simulate expands seeds with
-    // (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:99-124)
+    // (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:20-45)
     struct seed_stream {
         std::uint64_t s[4];
         explicit seed_stream(std::uint64_t seed) {

@@ -95,10 +95,13 @@ def lib():
             "ta3o_model_value": (C.c_float, [C.c_int, f32p, C.c_uint32, u32p, C.c_int]),
             "ta3o_f32w_value_fp32": (C.c_float, [f32p, C.c_uint32, u32p, C.c_int]),
             "ta3o_set_garbage": (None, [C.c_int]),
+            "ta3o_set_widths": (None, [u32p]),
             "ta3o_search_beam": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, u32p, C.c_int,
                                            ip, ip, ip, f32p, C.POINTER(Counters)]),
             "ta3o_search_beam_pruned": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, f32p, C.c_float, C.c_int, C.c_uint,
                                                   ip, ip, ip, f32p]),
+            "ta3o_candidate_ops": (None, [C.c_int, f32p, u32p, C.c_int, u64p, C.POINTER(C.c_int32), u32p, f32p,
+                                          f32p]),
             "ta3o_select_top": (C.c_uint32, [u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
             "ta3o_set_select_linear": (None, [C.c_int]),
             "ta3o_search_exhaustive": (C.c_int, [C.POINTER(Engine), C.c_int, f32p, C.c_int, ip, ip, ip, f32p]),
@@ -293,6 +296,21 @@ class garbage:
         return False
 
 
+class widths:
+    """context manager: the games the oracle plays inside it search with these beam widths (k0, k1, k2)"""
+
+    def __init__(self, ks):
+        self.ks = np.array(ks, dtype=np.uint32)
+
+    def __enter__(self):
+        lib().ta3o_set_widths(_p(self.ks, C.c_uint32))
+        return self
+
+    def __exit__(self, *exc):
+        lib().ta3o_set_widths(None)
+        return False
+
+
 def model_value(kind: int, w, clears: int, cols, held_is_i: bool) -> float:
     w = _f32(w if len(w) else np.zeros(1))
     c = np.array(cols, dtype=np.uint32)
@@ -457,6 +475,22 @@ def eval_host_pruned_mt(kind, w, num_models, seeds, max_moves, pruner_w, act_ran
                                    _p(pl, C.c_uint32),
                                    C.byref(cnt))
     return out, pl, cnt
+
+
+def candidate_ops(kind, w, samples):
+    """instrumented op counts of candidates given as trace records [n, 11]: (ops per part summed (fit, place +
+    clear, features, network), drop or -1 per record, boards [n, 10], features [n, 16], values [n])"""
+    w = _f32(w)
+    smp = np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1, 11)
+    n = smp.shape[0]
+    ops = np.zeros(4, dtype=np.uint64)
+    legal = np.zeros(n, dtype=np.int32)
+    boards = np.zeros((n, 10), dtype=np.uint32)
+    feats = np.zeros((n, 16), dtype=np.float32)
+    vals = np.zeros(n, dtype=np.float32)
+    lib().ta3o_candidate_ops(kind, _p(w, C.c_float), _p(smp, C.c_uint32), n, _p(ops, C.c_uint64),
+                             _p(legal, C.c_int32), _p(boards, C.c_uint32), _p(feats, C.c_float), _p(vals, C.c_float))
+    return ops, legal, boards, feats, vals
 
 
 def trace_records(kind, w, seed, max_moves, every):

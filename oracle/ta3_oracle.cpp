@@ -16,7 +16,7 @@
  *    height_model test double of ta3_ai/tests/src/search/beam_test.cpp:20-35.
  *
  * Floating point: compile with -ffp-contract=off; every multiply and add is rounded separately, as in
- * the reference's host build (ta3_ai/incl/ta3/ai/models/models/utility.hpp:340-345).
+ * the reference's host build (ta3_ai/incl/ta3/ai/models/models/utility.hpp:39-44).
  */
 #include "ta3_oracle.h"
 
@@ -219,7 +219,7 @@ inline Board canonical(Board const& b) { // :132-143: the lexicographically smal
 }
 
 // ---------------------------------------------------------------------------------------------------
-// xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:99-127)
+// xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:20-48)
 // ---------------------------------------------------------------------------------------------------
 inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
@@ -485,7 +485,7 @@ const Trees TREES;
 // ---------------------------------------------------------------------------------------------------
 // value models
 // ---------------------------------------------------------------------------------------------------
-// clear histogram (ta3_ai/incl/ta3/ai/model_defs.hpp:79-113): four 8-bit lanes, lane k = (k+1)-line clears
+// clear histogram (ta3_ai/incl/ta3/ai/model_defs.hpp:37-58): four 8-bit lanes, lane k = (k+1)-line clears
 inline uint32_t hist_count(uint32_t h, int k) { return (h >> (8 * k)) & 0xFFu; }
 inline uint32_t hist_added(uint32_t h, int cleared) { return cleared == 0 ? h : h + (1u << (8 * (cleared - 1))); }
 inline uint32_t hist_lines(uint32_t h) {
@@ -543,7 +543,7 @@ Metrics metrics(Board const& b) {
 // fsqrt (v_board_metrics.hpp:23-37): 0 for x <= 0, else the correctly rounded sqrt
 inline float fsqrt(float x) { return x <= 0 ? 0.0f : std::sqrt(x); }
 
-// ModelV11::extractInputs (ta3_ai/incl/ta3/ai/models/models/model_v11.hpp:178-206)
+// ModelV11::extractInputs (ta3_ai/incl/ta3/ai/models/models/model_v11.hpp:166-180)
 void features_v11(uint32_t clears, Board const& b, bool heldIsI, float* out) {
     for(int k = 0; k < 4; ++k) out[k] = static_cast<float>(hist_count(clears, k)) * 0.25f; // CLEAR_COUNT_NORM
     Metrics const m = metrics(b);
@@ -649,7 +649,7 @@ float value_f32w(float const* w, float const* in) {
     return ((p[0] + p[1]) + (p[2] + p[3])) + w3[128];
 }
 
-// dense layer (ta3_ai/incl/ta3/ai/models/models/utility.hpp:330-347): W[o*In + i], biases after the
+// dense layer (ta3_ai/incl/ta3/ai/models/models/utility.hpp:29-46): W[o*In + i], biases after the
 // matrix, v = bias then += in[i] * w in ascending i, ReLU = std::max<float>(0, v)
 void dense(float const* in, int nIn, int nOut, bool relu, float const* w, float* out) {
     for(int o = 0; o < nOut; ++o) {
@@ -678,7 +678,7 @@ struct ModelRef {
 
     float value(uint32_t clears, Board const& b, bool heldIsI) const {
         switch(kind) {
-            case TA3O_MODEL_V11: { // ModelV11::forward (model_v11.hpp:281-295): 10 -> 8 ReLU -> 1
+            case TA3O_MODEL_V11: { // ModelV11::forward (model_v11.hpp:166-180): 10 -> 8 ReLU -> 1
                 float in[10], h1[8], out[1];
                 features_v11(clears, b, heldIsI, in);
                 dense(in, 10, 8, true, w, h1);
@@ -771,7 +771,7 @@ uint32_t select_top(std::vector<uint32_t>& scores, uint32_t k, uint32_t* outSlot
 }
 
 struct Step { Board board; uint32_t clears; bool legal; };
-inline Step apply(Board const& b, uint32_t clears, Placement const& pl) { // search.hpp:294-302
+inline Step apply(Board const& b, uint32_t clears, Placement const& pl) { // search.hpp:61-69
     Fit const f = fit(b, pl.shape);
     if(!f.legal) return {b, clears, false};
     Board n = placed(b, pl.shape, f.drop);
@@ -1315,7 +1315,7 @@ SearchResult search_exhaustive(Engine const& g, ModelRef const& model, int depth
 }
 
 // ---------------------------------------------------------------------------------------------------
-// stats_v4 / score_v1 (ta3_ai/incl/ta3/ai/models/metrics/tetris_stats.hpp:157-162, tetris_stats_v4.hpp:8-15,
+// stats_v4 / score_v1 (ta3_ai/incl/ta3/ai/models/metrics/tetris_stats.hpp:19-23, tetris_stats_v4.hpp:8-15,
 // metrics.hpp:94-127, score_v1.hpp:8-34)
 // ---------------------------------------------------------------------------------------------------
 struct Stats {
@@ -1350,6 +1350,7 @@ struct Stats {
 };
 
 const std::vector<uint32_t> REF_WIDTHS = {16, 32, 32}; // variation_sequences.hpp:22
+uint32_t g_widths[3] = {16, 32, 32}; // widths the game loops search with (set before a run, read by its threads)
 
 // eval_host inner loop for one (model, seed) game (model_eval_host.cpp:33-52)
 float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWorkspace& ws, Counters& cnt,
@@ -1358,6 +1359,7 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
     Engine g;
     engine_reset(g, seed);
     if(g_garbage) add_garbage_rows(g, seed);
+    std::vector<uint32_t> const widths(g_widths, g_widths + 3);
     Stats stats;
     int rec = 0;
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
@@ -1365,7 +1367,7 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
         tr.out = traceOut;
         // traced moves: index congruent to the game seed modulo traceEvery (spreads them over games)
         tr.on = traceOut != nullptr && traceEvery != 0 && i % traceEvery == seed % traceEvery;
-        SearchResult const r = search_beam(g, model, REF_WIDTHS, ws, cnt, pr, tr);
+        SearchResult const r = search_beam(g, model, widths, ws, cnt, pr, tr);
         if(!r.some) break;
         if(r.hold) engine_hold(g);
         int const cleared = engine_place(g, r.o, r.x);
@@ -1622,6 +1624,10 @@ int ta3o_search_beam_pruned(const ta3o_engine* e, int kind, const float* w, cons
 
 uint32_t ta3o_select_top(uint32_t* scores, uint32_t n, uint32_t k, uint32_t* out_slots, uint32_t* out_scores) {
     return select_top(scores, n, k, out_slots, out_scores);
+}
+
+void ta3o_set_widths(const uint32_t* widths) {
+    for(int l = 0; l < 3; ++l) g_widths[l] = widths ? widths[l] : REF_WIDTHS[l];
 }
 
 void ta3o_set_select_linear(int enabled) { g_selectLinear.store(enabled ? 1 : 0, std::memory_order_relaxed); }

@@ -52,7 +52,7 @@ typedef struct {
 
 int ta3o_num_params(int kind);
 
-/* xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:99-127) */
+/* xoshiro256** (ta3_sim/incl/ta3/sim/utility/xoshiro256ss.hpp:20-48) */
 void ta3o_rng_seed(uint64_t seed, uint64_t state[4]);
 uint64_t ta3o_rng_next(uint64_t state[4]);
 /* gpu::simulate seed expansion (ta3_gpu/incl/ta3/gpu/model_eval.hpp:92-95) */
@@ -74,6 +74,10 @@ void ta3o_piece_shape(int type, int o, uint32_t* cols_nibbles, int* left);
  * but for one hole, drawn from xoshiro256** seeded with seed ^ 0x6A09E667F3BCC908: n = next() % 5, then per
  * row y = 22, 21, ...: hole = next() % 10 */
 void ta3o_set_garbage(int enabled);
+
+/* process-wide option: beam widths of the games the oracle plays (eval_host*, traces; 3 values; NULL = the reference's
+ * {16, 32, 32}, variation_sequences.hpp:22) */
+void ta3o_set_widths(const uint32_t* widths);
 
 /* TetrisEngine (tetris_engine.hpp:103-161); place returns cleared rows or -1 for DIED */
 void ta3o_engine_reset(ta3o_engine* e, uint64_t seed);
@@ -159,6 +163,14 @@ void ta3o_eval_host_pruned_mt(int kind, const float* w, uint32_t num_models, con
  * (copies at most cap_records) */
 uint64_t ta3o_trace_records(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint32_t every,
                             uint32_t* out, uint64_t cap_records);
+
+/* Instrumented op counts of n candidates given as trace records (parent board + piece / rotation / column): ops[0]
+ * = scalar operations of fit summed over the records, ops[1] of placing + clearing (legal records), ops[2] of the
+ * features, ops[3] of the value network (kind V11 or F16, weights w). legal[r] = drop distance or -1; boards[10 r]
+ * = the board after placing and clearing; features[16 r] (V11: first 10) and values[r] as the restatement
+ * computes them, for checking against the oracle (oracle/opcount.cpp). */
+void ta3o_candidate_ops(int kind, const float* w, const uint32_t* samples, int n, uint64_t* ops, int32_t* legal,
+                        uint32_t* boards, float* features, float* values);
 
 #ifdef __cplusplus
 }

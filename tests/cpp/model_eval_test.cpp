@@ -1,5 +1,5 @@
 // C++ caller of the drop-in header: the reference's GPU acceptance pattern
-// (ta3_gpu/tests/src/parity_test.cpp:198-255) run against expected values passed on the command line.
+// (ta3_gpu/tests/src/parity_test.cpp:43-101) run against expected values passed on the command line.
 // Usage: model_eval_test <weights.f32> <expected.f32>   (3 V11 models, seeds {1,2,3,4}, 200 moves)
 // Prints "ok" and exits 0 when: the bits canary passes, batched == per-model single evaluation, both equal
 // the expected (oracle) fitness bit for bit, a wrong weight count throws, and empty seeds give no result.

@@ -34,7 +34,7 @@ def parity_cases():
     import oracle as O
     cases = []
     rng = np.random.Generator(np.random.PCG64(20261203))
-    seeds = [1, 2, 3, 4]  # parity_test.cpp:191-194
+    seeds = [1, 2, 3, 4]  # parity_test.cpp:36-39
     for kind, P in ((O.MODEL_V11, 97), (O.MODEL_F16, 1633)):
         for ms in range(3):
             w = rng.uniform(-2.0, 2.0, size=P).astype(np.float32)

@@ -64,7 +64,7 @@ def test_golden_parity_cases(ev):
 
 @pytest.mark.parametrize("kind", [T.MODEL_V11, T.MODEL_F16])
 def test_single_model_matches_oracle(kind):
-    """parity_test.cpp:206-221: three weight vectors x seeds {1,2,3,4} x 200 moves, device == host exactly"""
+    """parity_test.cpp:49-66: three weight vectors x seeds {1,2,3,4} x 200 moves, device == host exactly"""
     e = T.ModelEvaluator(model=kind)
     P = T.num_params(kind)
     seeds = [1, 2, 3, 4]
@@ -135,7 +135,7 @@ def test_work_counters():
 
 
 def test_multi_model_batch_matches_single_and_host(ev):
-    """parity_test.cpp:225-255"""
+    """parity_test.cpp:68-101"""
     rng = np.random.default_rng(100)
     ws = [rng.uniform(-2, 2, 97).astype(np.float32) for _ in range(3)]
     seeds = [1, 2, 3, 4]

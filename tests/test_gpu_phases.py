@@ -94,11 +94,28 @@ def test_two_phase_pruned_and_autotune():
         two = run(ev, w, nm, seeds, 90)
         assert one[0].tobytes() == two[0].tobytes() and (one[1] == two[1]).all() and (one[2] == two[2]).all()
     ev.set_pruner(None, 0.0)
-    timings = T.autotune(ev, w, nm, seeds, 90, candidates=((2, 0), (4, 0)), phases=(0, 20))
-    assert set(timings) == {(2, 0, 0), (2, 0, 20), (4, 0, 0), (4, 0, 20)}
+    T._TUNED.clear()
+    tuned = T.autotune(ev, w, nm, seeds, 90, candidates=((2, 0), (4, 0)), phases=(0, 20))
+    assert not tuned["cached"]
+    assert set(tuned["timings_ms"]) == {"wpg2_grid0_phase0", "wpg2_grid0_phase20", "wpg4_grid0_phase0",
+                                        "wpg4_grid0_phase20"}
+    assert tuned["chosen"][0] in (2, 4) and tuned["chosen"][2] in (0, 20)
     host, host_pl = O.eval_host_mt(kind, w, nm, seeds, 90)
     fit, pl = run(ev, w, nm, seeds, 90)[:2]
     assert fit.tobytes() == host.tobytes() and (pl == host_pl).all()
+    # the same batch shape again: the cached choice is applied without timing, on another evaluator too
+    ev2 = T.ModelEvaluator(model=kind)
+    again = T.autotune(ev2, w, nm, seeds, 90, candidates=((2, 0), (4, 0)), phases=(0, 20))
+    assert again["cached"] and again["chosen"] == list(tuned["chosen"]) and again["key"] == tuned["key"]
+    assert ev2.launch_info()["waves_per_game"] == tuned["chosen"][0]
+    fit2, pl2 = run(ev2, w, nm, seeds, 90)[:2]
+    assert fit2.tobytes() == host.tobytes()
+    # a pruned evaluator tunes its own key (its kernel always runs 4 waves per game)
+    ev2.set_pruner(T.pruner_init(3), 4.0, k=6, ply_mask=0b1110)
+    pruned = T.autotune(ev2, w, nm, seeds, 90, phases=(0, 20))
+    assert not pruned["cached"] and pruned["key"] != tuned["key"]
+    assert set(pruned["timings_ms"]) == {"wpg4_grid0_phase0", "wpg4_grid0_phase20"}
+    ev2.close()
     ev.close()
 
 

@@ -43,8 +43,12 @@ def test_leaf_values_bit_exact(kind):
 
 
 def test_wide_leaf_values_within_tolerance():
-    """device vs the oracle's device-arithmetic restatement (exact sums): within 1e-4 of the value spread;
-    vs the plain FP32 network: within 5% of the spread (bfloat16 operands)"""
+    """device vs the oracle's device-arithmetic restatement: within 1e-5 of the value spread (the 1e-5 relative
+    bound for FP32 leaf values, taken against the scale of the outputs because values cross zero). Both round
+    the same bfloat16 operands and accumulate in FP32; only the MFMA's summation order differs from the
+    restatement's, which moves a sum of n terms by at most ~n 2^-24 of its terms' magnitudes (measured: ~1e-6 of
+    the spread). vs the plain FP32 network: within 5% of the spread (the bfloat16 rounding of weights and
+    features, 2^-9 relative per operand, is what the wide network is defined with)."""
     say("case: wide leaf values")
     boards, clears, held = board_set(5, 300)
     w = np.random.default_rng(6).normal(0, 0.2, 20865).astype(np.float32)
@@ -54,23 +58,37 @@ def test_wide_leaf_values_within_tolerance():
     spread = fp32.std()
     say(f"max |device - emulation| = {np.abs(got - emu).max():.3g}, max |device - fp32| = "
         f"{np.abs(got - fp32).max():.3g}, spread {spread:.3g}")
-    assert np.abs(got - emu).max() <= 1e-4 * spread
+    assert np.abs(got - emu).max() <= 1e-5 * spread
     assert np.abs(got - fp32).max() <= 0.05 * spread
 
 
 def test_wide_games_agree_with_oracle():
-    """whole games: fitness agreement rate with the oracle's restatement (bit-identical games)"""
+    """whole games and every move of them: the rate of games identical to the oracle's (fitness bits) is
+    reported, and along the device's games the oracle's FP32 search picks the device's move at >= 99% of the
+    states (BF16 MFMA values differ from the FP32 restatement by ~1e-6 of their spread, so only near-ties can
+    flip a move)"""
+    from test_gpu_configs import replay_agreement
+    from concurrent.futures import ThreadPoolExecutor
     say("case: wide games 4 models x 6 seeds x 60")
-    nm = 4
-    w = np.random.default_rng(7).normal(0, 0.3, nm * 20865).astype(np.float32)
+    nm, P = 4, 20865
+    w = np.random.default_rng(7).normal(0, 0.3, nm * P).astype(np.float32)
     seeds = O.expand_seeds(8, 6)
     ev = T.ModelEvaluator(model=T.MODEL_F32W)
-    got = ev.eval(w, nm, seeds, 60)
+    ev.set_move_log(True)
+    ev.stage(w, nm, seeds, 60)
+    ev.launch()
+    got, pl = ev.fetch()
+    log = ev.fetch_moves()
     ev.close()
     want, _ = O.eval_host_mt(O.MODEL_F32W, w, nm, seeds, 60)
-    rate = float((got == want).mean())
-    say(f"identical games: {rate:.3f}")
-    assert rate >= 0.75
+    say(f"identical games: {float((got == want).mean()):.3f}")
+    rows = [(m, g, m * len(seeds) + g) for m in range(nm) for g in range(len(seeds))]
+    with ThreadPoolExecutor(16) as pool:
+        same = list(pool.map(lambda t: replay_agreement(T.MODEL_F32W, w[t[0] * P:(t[0] + 1) * P], seeds[t[1]],
+                                                        log[t[2]], int(pl[t[2]]), O.search_beam), rows))
+    rate = sum(same) / float(pl.sum())
+    say(f"wide placement agreement {rate:.4f} over {int(pl.sum())} moves")
+    assert rate >= 0.99
 
 
 def test_wide_pruned_k34_is_the_wide_full_beam():

@@ -21,7 +21,7 @@ def test_xoshiro256ss_published_vector():
 
 
 def test_splitmix64_seeding_published_vector():
-    # seed(s) fills the state with successive splitmix64 outputs (xoshiro256ss.hpp:99-107);
+    # seed(s) fills the state with successive splitmix64 outputs (xoshiro256ss.hpp:20-28);
     # splitmix64 from 0 is the published e220a8397b1dcdaf, 6e789e6aa1b965f4, 06c45d188009454f, f88bb8a8724c81ec.
     import ctypes as C
     st = (C.c_uint64 * 4)()

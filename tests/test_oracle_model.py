@@ -3,7 +3,7 @@
 - Features: brute-force cell-grid definitions of the reference metrics
   (ta3_ai/incl/ta3/ai/models/metrics/v_board_metrics.hpp:130-232), normalised with numpy float32
   (IEEE correctly rounded division and sqrt), compared bit-exactly.
-- MLP: the dense layout of ta3_ai/incl/ta3/ai/models/models/utility.hpp:330-347 evaluated with numpy
+- MLP: the dense layout of ta3_ai/incl/ta3/ai/models/models/utility.hpp:29-46 evaluated with numpy
   float32 scalars (separately rounded multiply and add), compared bit-exactly.
 - score_v1 (ta3_ai/incl/ta3/ai/models/metrics/score_v1.hpp:20-34) recomputed from a recorded game.
 - End to end: the reference's shipped v11 champion (models/v11_16_32_32_score_296.champ, mean score 296
@@ -162,7 +162,7 @@ def fma_dense(x, n_in, n_out, relu, w):
 
 
 def test_f16_value_matches_exact_fma_reference():
-    """the build-defined 16-32-32-1 network: dense layout of utility.hpp:330-347 with each
+    """the build-defined 16-32-32-1 network: dense layout of utility.hpp:29-46 with each
     multiply-add fused (one IEEE rounding), checked against exact rational arithmetic"""
     rng = np.random.default_rng(5)
     for cols in random_boards(4, 2):
@@ -176,7 +176,7 @@ def test_f16_value_matches_exact_fma_reference():
 
 
 def test_layer_layout_biases_after_matrix():
-    """weights W[o*In+i] then the Out biases (utility.hpp:341): a zero matrix leaves bias-only outputs"""
+    """weights W[o*In+i] then the Out biases (utility.hpp:40): a zero matrix leaves bias-only outputs"""
     w = np.zeros(97, dtype=np.float32)
     w[80:88] = np.arange(1, 9, dtype=np.float32)  # layer-1 biases
     w[88:96] = 1.0                               # layer-2 weights
@@ -237,7 +237,7 @@ def test_v11_champion_plays_near_perfect_games():
     assert (fit > 280.0).sum() >= 6
 
 
-def test_batched_equals_per_model():  # parity_test.cpp:225-255 pattern, on the oracle itself
+def test_batched_equals_per_model():  # parity_test.cpp:68-101 pattern, on the oracle itself
     rng = np.random.default_rng(10)
     ws = [rng.uniform(-2, 2, 97).astype(np.float32) for _ in range(3)]
     seeds = [1, 2, 3, 4]

@@ -296,7 +296,7 @@ def near_top_out(seed, rng):
     return e
 
 
-def _hist_added(h, cleared):  # clear_hist_t::added (model_defs.hpp:108-110)
+def _hist_added(h, cleared):  # clear_hist_t::added (model_defs.hpp:53-55)
     return h if cleared == 0 else h + (1 << (8 * (cleared - 1)))
 
 

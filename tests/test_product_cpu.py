@@ -68,6 +68,10 @@ def test_argument_errors_without_gpu():
     with pytest.raises(T.HipError, match="21249"):
         ev.set_pruner(pw[:-1], 2.0)
     ev.set_pruner(None, 0.0)  # back to the full beam: no device work
+    for bad in ((0, 32, 32), (17, 32, 32), (16, 33, 32), (16, 32, 0)):
+        with pytest.raises(T.HipError, match="set_beam_widths"):
+            ev.set_beam_widths(bad)
+    ev.set_beam_widths((4, 8, 8))
 
 
 def test_pruner_quantization_matches_oracle():

@@ -61,6 +61,7 @@ def lib():
             "ta3h_num_params": (C.c_int, [C.c_int]),
             "ta3h_set_model": (C.c_int, [vp, C.c_int]),
             "ta3h_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint32]),
+            "ta3h_set_beam_widths": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
             "ta3h_eval": (C.c_int, [vp, f32p, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32, f32p]),
             "ta3h_stage": (C.c_int, [vp, f32p, C.c_uint32, C.c_uint32, u64p, C.c_uint32, C.c_uint32]),
             "ta3h_launch": (C.c_int, [vp]),
@@ -129,6 +130,8 @@ class ModelEvaluator:
             raise HipError("ta3h_create failed")
         self.model = model
         self._items, self._max_moves = 0, 0
+        self._garbage, self._pruned, self._traced, self._logged = False, False, False, False
+        self._widths = (16, 32, 32)
         self._check(lib().ta3h_set_model(self._h, model))
         self.set_launch(DEFAULT_WAVES_PER_GAME.get(model, 2), True, 0)
 
@@ -160,6 +163,13 @@ class ModelEvaluator:
     def set_garbage(self, enabled: bool = True):
         """start every game of the following launches with 0-4 seeded garbage rows"""
         self._check(lib().ta3h_set_garbage(self._h, int(enabled)))
+        self._garbage = bool(enabled)
+
+    def set_beam_widths(self, widths=(16, 32, 32)):
+        """beam widths per level for the following launches (at most 16 / 32 / 32, the reference's)"""
+        k0, k1, k2 = (int(v) for v in widths)
+        self._check(lib().ta3h_set_beam_widths(self._h, k0, k1, k2))
+        self._widths = (k0, k1, k2)
 
     def set_launch(self, waves_per_game: int = 2, persistent: bool = True, grid_blocks: int = 0):
         """waves cooperating on one game (1, 2 or 4), persistent work queue or one workgroup per item,
@@ -229,6 +239,7 @@ class ModelEvaluator:
     def set_move_log(self, enabled: bool = True):
         """record every committed move of the following launches (fetch_moves)"""
         self._check(lib().ta3h_set_move_log(self._h, int(enabled)))
+        self._logged = bool(enabled)
 
     def fetch_moves(self):
         """move log of the last launch: uint8 [items, max_moves], byte i of an item = its move i as
@@ -274,14 +285,19 @@ class ModelEvaluator:
         pruner_w None restores the full beam"""
         if pruner_w is None:
             self._check(lib().ta3h_set_pruner(self._h, None, 1.0, 34, 0))
+            self._pruned = False
             return
         w = _pruner_w(pruner_w)
         self._check(lib().ta3h_set_pruner(self._h, _ptr(w, C.c_float), float(act_range), int(k), int(ply_mask)))
+        self._pruned, self._traced = True, False
 
     def set_trace(self, cap_records: int, every: int = 1):
         """full-beam traces of every `every`-th move for the following launches (cap_records 0 = off)"""
         self._check(lib().ta3h_set_trace(self._h, int(cap_records), int(every)))
         self._trace_cap = int(cap_records)
+        self._traced = int(cap_records) > 0 and int(every) > 0
+        if self._traced:
+            self._pruned = False
 
     def fetch_trace(self, cap_records: int | None = None):
         """(records uint32 [n, 11], produced) of the last launch; n <= the capacity set by set_trace"""
@@ -352,11 +368,12 @@ class Trainer:
         self._h = C.c_void_p()
         if lib().ta3h_trainer_create(C.byref(self.cfg), C.byref(self._h)) != 0:
             raise HipError("trainer: invalid configuration")
-        self._set_shape(kind, islands, population, games, generations)
+        self._set_shape(kind, islands, population, games, max_moves, generations)
 
-    def _set_shape(self, kind, islands, population, games, generations):
+    def _set_shape(self, kind, islands, population, games, max_moves, generations):
         self.kind, self.num_params = kind, num_params(kind)
         self.cfg_islands, self.cfg_population, self.cfg_games, self.cfg_generations = islands, population, games, generations
+        self.cfg_max_moves = max_moves
 
     def close(self):
         if self._h:
@@ -419,8 +436,8 @@ class Trainer:
         t.cfg = None
         with open(path, "rb") as f:  # the config follows the magic and the version: i32 kind, then 5 x u32
             head = f.read(36)
-        kind, islands, population, games, _, generations = np.frombuffer(head[12:36], dtype="<i4")
-        t._set_shape(int(kind), int(islands), int(population), int(games), int(generations))
+        kind, islands, population, games, max_moves, generations = np.frombuffer(head[12:36], dtype="<i4")
+        t._set_shape(int(kind), int(islands), int(population), int(games), int(max_moves), int(generations))
         return t
 
 
@@ -479,28 +496,79 @@ def pruner_init(seed: int = 0) -> np.ndarray:
     return w
 
 
-# launch shapes measured fastest on MI355X per network (full CMA-ES generation, see autotune())
+# launch shapes measured fastest on MI355X per network (full CMA-ES generation); autotune() refines them per batch
 DEFAULT_WAVES_PER_GAME = {MODEL_V11: 2, MODEL_F16: 4, MODEL_F32W: 4}  # V11 at 5 waves/SIMD, F16 at 4
+
+# tuned launch shapes: (network, models, games, max_moves, garbage, kernel mode) -> (waves per game, grid, phase)
+_TUNED: dict = {}
+
+
+def _tune_key(evaluator: "ModelEvaluator", num_models: int, games: int, max_moves: int) -> str:
+    mode = "pruned" if evaluator._pruned else ("trace" if evaluator._traced or evaluator._logged else "plain")
+    widths = "-".join(str(k) for k in evaluator._widths)
+    return (f"{evaluator.model}:{int(num_models)}x{int(games)}:{int(max_moves)}:g{int(evaluator._garbage)}:{mode}"
+            f":w{widths}")
+
+
+def _cache_path() -> str:
+    return os.environ.get("TA3H_AUTOTUNE_CACHE", "")
+
+
+def _load_cache() -> dict:
+    path = _cache_path()
+    if path and os.path.exists(path):
+        import json
+        with open(path) as f:
+            return {k: tuple(v) for k, v in json.load(f).items()}
+    return {}
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
-             candidates=((1, 0), (2, 0), (4, 0)), phases=(None,)):
-    """Times one launch per launch shape (waves per game, grid) and first-phase budget on the given batch and
-    keeps the fastest on `evaluator`. Neither changes results; returns {(wpg, grid, phase): kernel_ms}."""
+             candidates=((1, 0), (2, 0), (4, 0)), phases=(None, 64, 256), repeats: int = 2, use_cache: bool = True):
+    """Picks the launch shape (waves per game, grid) and first-phase budget for this batch shape and keeps it on
+    `evaluator` (none of them changes a result). The choice is cached per (network, models x games, max_moves,
+    garbage rows, kernel mode) in this process and, when TA3H_AUTOTUNE_CACHE names a JSON file, across processes;
+    a cached shape is applied without timing. Otherwise every candidate is timed `repeats` times on the batch
+    (device time of the search) and the fastest kept. The pruned and trace kernels always run 4 waves per game, so
+    only their phase budget is tuned. Leaves the batch staged. Returns {"key", "chosen", "cached", "timings_ms"}."""
+    key = _tune_key(evaluator, num_models, len(seeds), max_moves)
+    if use_cache:
+        if key not in _TUNED:
+            _TUNED.update({k: v for k, v in _load_cache().items() if k not in _TUNED})
+        if key in _TUNED:
+            wpg, grid, phase = _TUNED[key]
+            evaluator.set_launch(wpg, True, grid)
+            evaluator.set_phases(phase)
+            evaluator.stage(weights, num_models, seeds, max_moves)
+            return {"key": key, "chosen": list(_TUNED[key]), "cached": True, "timings_ms": {}}
+    if evaluator._pruned or evaluator._traced or evaluator._logged:
+        candidates = ((4, 0),)
+    evaluator.stage(weights, num_models, seeds, max_moves)
     timings = {}
     for wpg, grid in candidates:
         for phase in phases:
             evaluator.set_launch(wpg, True, grid)
             evaluator.set_phases(phase)
-            evaluator.stage(weights, num_models, seeds, max_moves)
-            evaluator.launch()
-            evaluator.sync()
-            timings[(wpg, grid, phase)] = evaluator.last_kernel_ms()
-    best = min(timings, key=timings.get)
-    evaluator.set_launch(best[0], True, best[1])
-    evaluator.set_phases(best[2])
-    evaluator.stage(weights, num_models, seeds, max_moves)
-    return timings
+            best = float("inf")
+            for _ in range(max(1, int(repeats))):
+                evaluator.launch()
+                evaluator.sync()
+                best = min(best, evaluator.last_kernel_ms())
+            timings[(wpg, grid, phase)] = best
+    chosen = min(timings, key=timings.get)
+    evaluator.set_launch(chosen[0], True, chosen[1])
+    evaluator.set_phases(chosen[2])
+    _TUNED[key] = chosen
+    path = _cache_path()
+    if use_cache and path:
+        import json
+        merged = _load_cache()
+        merged[key] = chosen
+        with open(path, "w") as f:
+            json.dump({k: list(v) for k, v in merged.items()}, f, indent=1, sort_keys=True)
+    return {"key": key, "chosen": list(chosen), "cached": False,
+            "timings_ms": {f"wpg{k[0]}_grid{k[1]}_phase{'auto' if k[2] is None else k[2]}": v
+                           for k, v in timings.items()}}
 
 
 def expand_seeds(seed: int, games: int) -> np.ndarray:
@@ -508,7 +576,7 @@ def expand_seeds(seed: int, games: int) -> np.ndarray:
     mask = (1 << 64) - 1
     s = seed & mask
     st = []
-    for _ in range(4):  # splitmix64 seeding (xoshiro256ss.hpp:99-107)
+    for _ in range(4):  # splitmix64 seeding (xoshiro256ss.hpp:20-28)
         s = (s + 0x9E3779B97F4A7C15) & mask
         z = s
         z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask

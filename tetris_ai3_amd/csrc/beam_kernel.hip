@@ -199,7 +199,7 @@ static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 6 <= PARK_WORDS && 
 
 struct alignas(16) GameLds {
     uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
-    uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:91-113)
+    uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:37-58)
     uint32_t score[COMMITTED + 1];            // ordered_bits score of the kept node
     uint32_t info[COMMITTED + 1];             // tree node | root rank << 8
     uint32_t segOff[64];                      // exclusive prefix offsets of the stage's segments
@@ -229,7 +229,7 @@ struct alignas(16) PruneLds {
 };
 
 
-__device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:112-124
+__device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:33-45
     uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
     uint64_t const t = s[1] << 17;
     s[2] ^= s[0];
@@ -280,7 +280,7 @@ __device__ void engine_reset(EngineState& e, uint64_t seed) { // tetris_engine.h
     e.held = NO_PIECE;
     e.gameOver = 0;
     e.bagPos = 0;
-    for(int i = 0; i < 4; ++i) { // splitmix64 seeding (xoshiro256ss.hpp:99-107)
+    for(int i = 0; i < 4; ++i) { // splitmix64 seeding (xoshiro256ss.hpp:20-28)
         seed += 0x9e3779b97f4a7c15ULL;
         uint64_t z = seed;
         z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -368,7 +368,7 @@ __device__ void engine_hold(EngineState& e) { // tetris_engine.hpp:148-157
     update_game_over(e);
 }
 
-// stats_v4::advance for the fields score_v1 reads (tetris_stats.hpp:157-162, metrics.hpp:94-127)
+// stats_v4::advance for the fields score_v1 reads (tetris_stats.hpp:19-23, metrics.hpp:94-127)
 __device__ void stats_advance(EngineState& e, int cleared) {
     ++e.pieces;
     ++e.clears[cleared];
@@ -405,7 +405,7 @@ __device__ float final_score(EngineState const& e) { // score_v1.hpp:20-34; mode
 }
 
 // ---------------------------------------------------------------------------------------------------
-// candidate evaluation: apply (search.hpp:294-302) + features + value network
+// candidate evaluation: apply (search.hpp:61-69) + features + value network
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -446,7 +446,7 @@ __device__ inline float relu(float v) { return (0.0f < v) ? v : 0.0f; } // std::
 __device__ inline float mad_rn(float acc, float a, float b) { return __fadd_rn(acc, __fmul_rn(a, b)); }
 __device__ inline float fma_rn(float acc, float a, float b) { return __fmaf_rn(a, b, acc); }
 
-// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:281-295; utility.hpp:330-347). The zero skipping of the
+// V11 value: 10 features -> 8 ReLU -> 1 (model_v11.hpp:166-180; utility.hpp:29-46). The zero skipping of the
 // 16-feature network does not pay here: with three variants the V11 generation ran 4% slower, with one ballot
 // and two copies (skipping the multi-line clear counts) 5% slower.
 __device__ inline float value_v11(wptr w, float const (&f)[16]) {
@@ -817,7 +817,7 @@ __device__ inline int place_into(uint32_t const* parent, uint32_t cols, int col0
     return n;
 }
 
-__device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t::added (model_defs.hpp:108-110)
+__device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t::added (model_defs.hpp:53-55)
     return cleared == 0 ? h : h + (1u << (8 * (cleared - 1)));
 }
 
@@ -1388,6 +1388,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
     nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
     top = combine_parts<G, K0, true>(L, top, mask, wv);
+    // a narrower beam keeps the first beamWidth[0] keys of the exact, descending top-K0 list
+    top = lane < a.beamWidth[0] ? top : 0;
     uint32_t count = __popcll(__ballot(top != 0));
     if(count == 0) return false;
     if(wv == 0) {
@@ -1421,6 +1423,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                          : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
+        top = lane < a.beamWidth[level] ? top : 0; // narrower beam: the first beamWidth[level] keys
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
         if(lane < count && ((mask >> lane) & 1ull) == 0) {

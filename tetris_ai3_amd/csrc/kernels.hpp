@@ -39,6 +39,10 @@ struct EvalArgs {
     uint32_t games;
     uint32_t maxMoves;
     uint32_t weightStride;
+    // beam widths per level, at most the compiled 16 / 32 / 32 (variation_sequences.hpp:22): a level keeps the
+    // first beamWidth[l] keys of its exact top-K list, which is the top-beamWidth[l] selection; the order ids
+    // keep their relative order, so moves equal those of a beam built with these widths
+    uint32_t beamWidth[3];
     uint32_t persistent;
     uint32_t garbage; // start every game with 0-4 garbage rows (seeded from the game seed)
     // Two-phase schedule (persistent mode): the first launch plays every item for at most phaseMoves moves and

@@ -72,7 +72,7 @@ void repack_f32w(float const* src, uint8_t* dst) {
     std::memcpy(dst + wide::W3, src + 4224 + 16512, 129 * sizeof(float)); // W3 and b3
 }
 
-// host layout (reference dense(), utility.hpp:330-347) -> device layout of the selected network
+// host layout (reference dense(), utility.hpp:29-46) -> device layout of the selected network
 void repack(int kind, float const* src, float* dst) {
     int const P = params_of(kind);
     if(kind == TA3H_MODEL_F32W) {
@@ -176,6 +176,7 @@ struct ta3h_evaluator {
 
     int kind = TA3H_MODEL_V11;
     int garbage = 0;
+    uint32_t beamWidth[3] = {16, 32, 32}; // beam widths per level (variation_sequences.hpp:22)
     int wavesPerBlock = 2; // waves cooperating on one game
     int persistent = 1;
     uint32_t gridBlocks = 0; // 0: occupancy-derived
@@ -319,6 +320,17 @@ int ta3h_set_garbage(ta3h_evaluator* ev, int enabled) {
     return TA3H_OK;
 }
 
+int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t k2) {
+    if(!ev) return TA3H_E_ARG;
+    ev->drop_graph(); // captured settings change
+    if(k0 < 1 || k0 > 16 || k1 < 1 || k1 > 32 || k2 < 1 || k2 > 32)
+        return ev->fail_arg("set_beam_widths: widths must be in [1, 16], [1, 32], [1, 32]");
+    ev->beamWidth[0] = k0;
+    ev->beamWidth[1] = k1;
+    ev->beamWidth[2] = k2;
+    return TA3H_OK;
+}
+
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
     ev->drop_graph(); // captured settings and buffers change
@@ -429,6 +441,7 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.games = ev->games;
     a.maxMoves = ev->maxMoves;
     a.weightStride = stride_of(ev->kind);
+    for(int l = 0; l < 3; ++l) a.beamWidth[l] = ev->beamWidth[l];
     a.persistent = static_cast<uint32_t>(ev->persistent);
     a.garbage = static_cast<uint32_t>(ev->garbage);
     uint32_t const phase = phase_budget(ev);

@@ -75,9 +75,91 @@ def train_pruner(ev: ModelEvaluator, recs, epochs: int = 3, batch: int = 512, lr
     return w, losses, time.perf_counter() - t0
 
 
+FULL_WIDTHS = (16, 32, 32)
+
+
+def parse_setting(spec: str) -> dict:
+    """'p<k>:<mask>' = pruned beam (k of every list at the plies of mask: bit 1, 2 middle levels, bit 3 leaves),
+    'n<k0>-<k1>-<k2>' = narrower beam, 'n...+p...' both, 'full' = the full beam"""
+    out = {"name": spec, "widths": FULL_WIDTHS, "k": 0, "mask": 0}
+    for part in spec.split("+"):
+        if part == "full":
+            continue
+        if part.startswith("p"):
+            k, mask = part[1:].split(":")
+            out["k"], out["mask"] = int(k), int(mask)
+        elif part.startswith("n"):
+            out["widths"] = tuple(int(v) for v in part[1:].split("-"))
+        else:
+            raise ValueError(f"unknown setting {spec!r}")
+    return out
+
+
+def run_setting(ev: ModelEvaluator, w, num_models: int, seeds, max_moves: int, setting: dict, pruner=None,
+                act_range: float = 0.0, steps: int = 3) -> dict:
+    """One warm launch, then `steps` timed launches (wall clock, launch + synchronisation) of the batch under a
+    setting; returns its rate, score and value evaluations per placement. Leaves the full beam set."""
+    ev.set_beam_widths(setting["widths"])
+    if setting["k"]:
+        ev.set_pruner(pruner, act_range, k=setting["k"], ply_mask=setting["mask"])
+    else:
+        ev.set_pruner(None, 0.0)
+    ev.stage(w, num_models, seeds, max_moves)
+    ev.launch()
+    ev.sync()
+    t0 = time.perf_counter()
+    for _ in range(max(1, steps)):
+        ev.launch()
+        ev.sync()
+    elapsed = (time.perf_counter() - t0) / max(1, steps)
+    fit, pl = ev.fetch()
+    cnt = ev.fetch_counters()
+    ev.set_pruner(None, 0.0)
+    ev.set_beam_widths(FULL_WIDTHS)
+    placements = int(pl.sum())
+    return {"setting": setting["name"], "widths": list(setting["widths"]), "k": setting["k"],
+            "ply_mask": setting["mask"], "ms_per_step": elapsed * 1e3, "placements": placements,
+            "placements_per_s": placements / elapsed, "mean_score": float(fit.mean()),
+            "value_evals_per_placement": float(cnt[:, 1].sum()) / max(placements, 1), "fitness": fit}
+
+
+def compare_settings(ev: ModelEvaluator, w, num_models: int, seeds, max_moves: int, specs, pruner=None,
+                     act_range: float = 0.0, steps: int = 3) -> list:
+    """The full beam and every setting of `specs` on the same batch: speed-up over the full beam's placements/s,
+    mean-score loss relative to the full beam's |mean| (positive = worse), share of games with identical fitness."""
+    full = run_setting(ev, w, num_models, seeds, max_moves, parse_setting("full"), steps=steps)
+    out = []
+    for spec in specs:
+        r = run_setting(ev, w, num_models, seeds, max_moves, parse_setting(spec), pruner, act_range, steps)
+        fm = full["mean_score"]
+        r["speedup_vs_full"] = r["placements_per_s"] / full["placements_per_s"]
+        r["mean_score_loss_pct"] = 100.0 * (fm - r["mean_score"]) / abs(fm) if fm != 0 else 0.0
+        r["meets_1pct"] = bool(r["mean_score_loss_pct"] <= 1.0)
+        r["same_games"] = float((r.pop("fitness") == full["fitness"]).mean())
+        out.append(r)
+    full.pop("fitness")
+    return [full] + out
+
+
+def fit_pruner(ev: ModelEvaluator, w, num_models: int, seeds, max_moves: int, every: int, cap_records: int,
+               records: int, epochs: int, seed: int = 1):
+    """Traces of the full beam on (w, seeds), a sample of `records` of them, training on the GPU and the
+    activation-range calibration: (pruner weights, act_range, info)."""
+    t0 = time.perf_counter()
+    recs, produced, _ = collect_traces(ev, w, num_models, seeds, max_moves, every=every, cap=cap_records)
+    rng = np.random.default_rng(seed)
+    train = recs if recs.shape[0] <= records else recs[rng.choice(recs.shape[0], records, replace=False)]
+    pw, losses, secs = train_pruner(ev, train, epochs=epochs, batch=512, lr=1e-3, seed=seed)
+    act = calibrate_act_range(ev, pw, train)
+    return pw, act, {"trace_records": int(produced), "kept": int(recs.shape[0]), "trained_on": int(train.shape[0]),
+                     "every": every, "epochs": epochs, "train_s": secs, "setup_s": time.perf_counter() - t0,
+                     "training_loss": [float(x) for x in losses]}
+
+
 def _main(argv=None) -> int:
-    """Pruner study on a trained model (cfg3 shape): train on the model's own full-beam traces from one seed
-    set, then judge pruned vs full beam on `--games` fresh games. Prints one JSON line per setting."""
+    """Pruner study on a trained model (cfg3 shape): train on the model's own full-beam traces from one seed set,
+    then judge every setting against the full beam on `--games` held-out games. Prints one JSON line per
+    setting."""
     import argparse
     import json
 
@@ -87,14 +169,16 @@ def _main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=_main.__doc__)
     ap.add_argument("--champ", required=True, help=".champ weights of the model (or models: count = n x params)")
     ap.add_argument("--kind", choices=sorted(kinds), default="f16")
-    ap.add_argument("--games", type=int, default=10000)
+    ap.add_argument("--games", type=int, default=1000)
     ap.add_argument("--max-moves", type=int, default=2000)
     ap.add_argument("--trace-games", type=int, default=200)
-    ap.add_argument("--every", type=int, default=8)
-    ap.add_argument("--records", type=int, default=2_000_000)
+    ap.add_argument("--every", type=int, default=64)
+    ap.add_argument("--records", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=10)
-    ap.add_argument("--settings", default="8:12,8:14,4:12,4:14",
-                    help="comma list of k:ply_mask (mask bits 1, 2 = middle levels, 3 = leaves)")
+    ap.add_argument("--garbage", action="store_true")
+    ap.add_argument("--settings", default="p8:8,p8:12,p4:12,n8-16-16,n4-8-8,n8-16-16+p8:8",
+                    help="comma list: p<k>:<ply mask> pruned, n<k0>-<k1>-<k2> narrower beam, n...+p... both")
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--save", default="", help="write the trained pruner (npy: weights then act_range)")
     args = ap.parse_args(argv)
@@ -102,46 +186,18 @@ def _main(argv=None) -> int:
     kind = kinds[args.kind]
     w = champ_read(args.champ).astype(np.float32)
     ev = ModelEvaluator(model=kind)
+    if args.garbage:
+        ev.set_garbage(True)
     nm = w.size // ev.num_params
-    trace_seeds = expand_seeds(args.seed, args.trace_games)
-    recs, produced, _ = collect_traces(ev, w, nm, trace_seeds, args.max_moves, args.every, args.records * 2)
-    rng = np.random.default_rng(args.seed)
-    if recs.shape[0] > args.records:
-        recs = recs[rng.choice(recs.shape[0], args.records, replace=False)]
-    pw, losses, secs = train_pruner(ev, recs, epochs=args.epochs, seed=1)
-    act = calibrate_act_range(ev, pw, recs)
-    print(json.dumps({"traces": int(produced), "trained_on": int(recs.shape[0]), "train_s": secs,
-                      "epoch_loss": [float(x) for x in losses], "act_range": act,
-                      "label_stats": label_stats(recs)}), flush=True)
+    pw, act, info = fit_pruner(ev, w, nm, expand_seeds(args.seed, args.trace_games), args.max_moves, args.every,
+                               args.records * 4, args.records, args.epochs)
+    print(json.dumps(dict(info, act_range=act)), flush=True)
     if args.save:
         np.save(args.save, np.r_[pw, np.float32(act)])
-    seeds = expand_seeds(args.seed + 1, args.games)
-
-    def run():
-        ev.stage(w, nm, seeds, args.max_moves)
-        ev.launch()
-        ev.sync()
-        ms = ev.last_kernel_ms()
-        fit, pl = ev.fetch()
-        cnt = ev.fetch_counters()
-        return fit, pl, cnt, ms
-
-    ev.set_pruner(None, 0.0)
-    full, full_pl, full_cnt, full_ms = run()
-    base = {"beam": "full", "mean_score": float(full.mean()), "placements": int(full_pl.sum()), "ms": full_ms,
-            "placements_per_s": float(full_pl.sum()) / full_ms * 1e3,
-            "value_evals_per_placement": float(full_cnt[:, 1].sum()) / float(full_pl.sum())}
-    print(json.dumps(base), flush=True)
-    for item in args.settings.split(","):
-        k, mask = (int(v) for v in item.split(":"))
-        ev.set_pruner(pw, act, k=k, ply_mask=mask)
-        fit, pl, cnt, ms = run()
-        rate = float(pl.sum()) / ms * 1e3
-        print(json.dumps({"beam": "pruned", "k": k, "ply_mask": mask, "mean_score": float(fit.mean()),
-                          "mean_score_loss_pct": 100.0 * (1.0 - float(fit.mean()) / float(full.mean())),
-                          "same_games": float((fit == full).mean()), "placements": int(pl.sum()), "ms": ms,
-                          "placements_per_s": rate, "speedup": rate / base["placements_per_s"],
-                          "value_evals_per_placement": float(cnt[:, 1].sum()) / float(pl.sum())}), flush=True)
+    rows = compare_settings(ev, w, nm, expand_seeds(args.seed + 1, args.games), args.max_moves,
+                            args.settings.split(","), pw, act, args.steps)
+    for r in rows:
+        print(json.dumps(r), flush=True)
     ev.close()
     return 0
 

@@ -19,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import MODEL_F16, MODEL_F32W, MODEL_V11, ModelEvaluator, Trainer, champ_write
+from . import MODEL_F16, MODEL_F32W, MODEL_V11, ModelEvaluator, Trainer, autotune, champ_write, expand_seeds
 
 KINDS = {"v11": MODEL_V11, "f16": MODEL_F16, "f32w": MODEL_F32W}
 
@@ -39,6 +39,8 @@ def main(argv=None) -> int:
     ap.add_argument("--garbage", action="store_true", help="start games with 0-4 garbage rows")
     ap.add_argument("--out", default="", help="path prefix of the .champ files and the .ckpt checkpoint; empty = none")
     ap.add_argument("--resume", default="", help="continue from a .ckpt checkpoint")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the network's default launch shape instead of tuning it on a first-generation batch")
     args = ap.parse_args(argv)
 
     if args.resume:
@@ -56,6 +58,16 @@ def main(argv=None) -> int:
         ev.set_garbage(True)
     if args.out:
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    if not args.no_autotune:
+        # launch shape and first-phase budget tuned on a batch shaped like a generation: island means ~ N(0, init_std^2)
+        # plus sigma0 noise, the seeds of a generation (results never depend on the shape)
+        M = args.islands * args.population
+        rng = np.random.default_rng(args.seed)
+        w0 = (rng.normal(0.0, args.init_std, (args.islands, 1, trainer.num_params))
+              + args.sigma0 * rng.normal(0.0, 1.0, (args.islands, args.population, trainer.num_params)))
+        tuned = autotune(ev, w0.astype(np.float32).reshape(-1), M, expand_seeds(args.seed, args.games),
+                         trainer.cfg_max_moves)
+        print(json.dumps({"autotune": tuned}), flush=True)
     ev._items = args.islands * args.population * args.games  # the trainer's generation batch
     for it in range(args.iterations):
         t0 = time.perf_counter()
