@@ -24,7 +24,8 @@ enum {
     TA3H_OK = 0,
     TA3H_E_ARG = 1,  /* invalid argument (sizes, model kind, launch shape) */
     TA3H_E_HIP = 2,  /* HIP runtime error */
-    TA3H_E_STATE = 3 /* call order (e.g. launch before stage) */
+    TA3H_E_STATE = 3, /* call order (e.g. launch before stage) */
+    TA3H_E_SCRATCH = 4 /* ta3h_eval_noalloc: the batch needs more than its static scratch array */
 };
 
 /* value networks the search scores leaves with */
@@ -87,6 +88,25 @@ int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint
  * One host synchronisation per call. */
 int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uint32_t num_params,
               const uint64_t* seeds, uint32_t games, uint32_t max_moves, float* out_fitness);
+
+/* Bytes of the static device array that ta3h_eval_noalloc keeps its scratch buffers in (one per GPU, held until
+ * the process ends). */
+#define TA3H_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): ta3h_eval on device buffers the caller owns, enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) without allocating and without synchronising. d_weights: num_models
+ * x ta3h_num_params(kind) floats in the host layout ta3h_eval takes; d_seeds: games seeds; d_fitness: num_models x
+ * games floats, model-major; d_placements: the same count of engine.place calls (may be NULL). Launch shape and
+ * two-phase budget are the defaults. The search's scratch (work-queue counters, the weights repacked into the
+ * network's device layout, parked games, resume lists) lives in one static device array of
+ * TA3H_NOALLOC_SCRATCH_BYTES per GPU: 256 B + models x the layout's bytes (V11 448, 16-32-32-1 6,592, wide 42,560,
+ * each rounded up to 256 B per section) + 256 B per (model, game) + 8 B per (model, game) + 4 B per model; a
+ * larger batch returns TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1: 4.0 MB; 256 x 100 wide: 17.7 MB). Calls
+ * that overlap on one GPU share that array and race: serialise them (one stream, or events). Returns TA3H_OK,
+ * TA3H_E_ARG, TA3H_E_SCRATCH or TA3H_E_HIP (there is no evaluator to hold an error message). */
+int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_models, const uint64_t* d_seeds, uint32_t games,
+                      uint32_t max_moves, float* d_fitness, uint32_t* d_placements, void* stream);
 
 /* The same evaluation split into its device steps, for callers that keep inputs resident and time or
  * overlap the device work: stage copies inputs into device buffers; launch enqueues the search on the

@@ -74,6 +74,17 @@ def test_argument_errors_without_gpu():
     ev.set_beam_widths((4, 8, 8))
 
 
+def test_noalloc_refuses_batches_beyond_its_scratch_without_gpu():
+    """ta3h_eval_noalloc checks sizes before any HIP call: a batch whose scratch exceeds the static array is
+    refused with TA3H_E_SCRATCH, bad arguments with TA3H_E_ARG, an empty batch succeeds"""
+    fake = 1 << 20  # never dereferenced: the checks come first
+    assert T.eval_noalloc(T.MODEL_F16, fake, 1000, fake, 1000, 100, fake) == T.E_SCRATCH
+    assert T.eval_noalloc(T.MODEL_F32W, fake, 1024, fake, 1, 100, fake) == T.E_SCRATCH  # 43.6 MB of weights
+    assert T.eval_noalloc(2, fake, 4, fake, 4, 100, fake) == 1
+    assert T.eval_noalloc(T.MODEL_V11, 0, 4, fake, 4, 100, fake) == 1
+    assert T.eval_noalloc(T.MODEL_V11, fake, 0, fake, 4, 100, fake) == 0
+
+
 def test_pruner_quantization_matches_oracle():
     """the library's grid exponents for the integer inference form equal the oracle's over weights and
     activation ranges spanning many binades"""

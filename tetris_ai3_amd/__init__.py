@@ -101,6 +101,7 @@ def lib():
                                             C.c_float, C.c_uint64, f32p]),
             "ta3h_pruner_hidden": (C.c_int, [vp, f32p, u32p, C.c_uint32, f32p]),
             "ta3h_pruner_logits": (C.c_int, [vp, f32p, C.c_float, u32p, C.c_uint32, f32p]),
+            "ta3h_eval_noalloc": (C.c_int, [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
             "ta3h_tables": (None, [C.POINTER(C.c_int8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
                                    C.POINTER(C.c_uint8)]),
         }
@@ -569,6 +570,20 @@ def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_m
     return {"key": key, "chosen": list(chosen), "cached": False,
             "timings_ms": {f"wpg{k[0]}_grid{k[1]}_phase{'auto' if k[2] is None else k[2]}": v
                            for k, v in timings.items()}}
+
+
+E_SCRATCH = 4
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def eval_noalloc(kind: int, d_weights: int, num_models: int, d_seeds: int, games: int, max_moves: int,
+                 d_fitness: int, d_placements: int = 0, stream: int = 0) -> int:
+    """ta3h_eval_noalloc on device addresses (integers, e.g. torch's data_ptr()) and a HIP stream handle (0 = the
+    null stream): enqueues the evaluation without allocating or synchronising; returns the status code (0 = ok,
+    E_SCRATCH = the batch exceeds the static scratch array)."""
+    return lib().ta3h_eval_noalloc(int(kind), C.c_void_p(d_weights), int(num_models), C.c_void_p(d_seeds), int(games),
+                                   int(max_moves), C.c_void_p(d_fitness), C.c_void_p(d_placements or None),
+                                   C.c_void_p(stream or None))
 
 
 def expand_seeds(seed: int, games: int) -> np.ndarray:

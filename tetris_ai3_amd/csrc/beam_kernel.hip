@@ -1904,13 +1904,13 @@ hipError_t max_active_blocks(int kind, int wavesPerBlock, int mode, int* blocksP
 // kernels run on the same device never rewrites a table those kernels read.
 hipError_t upload_constants() {
     static std::mutex mu;
-    static std::vector<int> done; // devices whose table is in place
+    static bool done[MAX_DEVICES] = {}; // devices whose table is in place (a fixed table: no host allocation)
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if(e != hipSuccess) return e;
+    if(dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
     std::lock_guard<std::mutex> const lock(mu);
-    for(int d : done)
-        if(d == dev) return hipSuccess;
+    if(done[dev]) return hipSuccess;
     // fsqrt (v_board_metrics.hpp:23-37) then the /100 normaliser (:229), both IEEE on the host
     float lut[SV_MAX + 1];
     for(int sv = 0; sv <= SV_MAX; ++sv) {
@@ -1919,7 +1919,7 @@ hipError_t upload_constants() {
         lut[sv] = r / 100.0f;
     }
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_svNorm), lut, sizeof(lut), 0, hipMemcpyHostToDevice);
-    if(e == hipSuccess) done.push_back(dev);
+    if(e == hipSuccess) done[dev] = true;
     return e;
 }
 

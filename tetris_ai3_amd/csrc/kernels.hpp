@@ -24,6 +24,7 @@ inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 4198
 }
 
 inline constexpr uint32_t PARK_WORDS = 64; // parked game: engine state + counters + move index (256 B)
+inline constexpr int MAX_DEVICES = 64;      // per-device set-up tables of the library (fixed size)
 
 struct EvalArgs {
     float const* weights;     // device layout per model, weightStride floats apart
