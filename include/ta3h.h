@@ -242,6 +242,14 @@ int ta3h_trainer_load(const char* path, ta3h_trainer** out);
 int ta3h_champ_write(const char* path, const double* w, uint64_t count);
 int ta3h_champ_read(const char* path, double* w, uint64_t cap, uint64_t* count);
 
+/* Host replay of a move log (no GPU), for previews: the game of `seed` (with the garbage rows of ta3h_set_garbage
+ * when `garbage` is set) played with moves[0 .. n) in the move-log byte format (hold << 7 | rotation << 5 | (x + 2));
+ * boards[10 i .. 10 i + 9] = the board after move i, pieces[i] = the piece it placed, cleared[i] = rows it cleared
+ * (each output may be NULL). Returns the moves replayed (fewer than n after a top-out) or -1 for a move that does
+ * not fit. Replaces the board snapshots the reference's preview renders (ta3_ai/src/preview.cpp:40-94). */
+int ta3h_replay(uint64_t seed, int garbage, const uint8_t* moves, uint32_t n, uint32_t* boards, uint8_t* pieces,
+                uint8_t* cleared);
+
 /* ---- learned pruner (placement-proposal network) and its training traces -------------------------------
  * No reference counterpart: the reference scores every placement of every kept node (ai/search/beam.hpp:337-372,
  * 407-443). The network: 265 inputs (board slot 24c + r = bit 3 + r of column c, r < 20; one-hots piece 240 + p,

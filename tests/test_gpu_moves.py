@@ -160,3 +160,29 @@ def test_wide_placement_agreement():
     rate = sum(same) / float(pl.sum())
     say(f"wide placement agreement {rate:.4f} over {int(pl.sum())} moves")
     assert rate >= 0.99
+
+
+def test_preview_boards_equal_oracle_games(tmp_path):
+    """the preview (tetris_ai3_amd.preview): games played on the GPU with the move log, boards rebuilt by the host
+    replay, equal to the oracle's own games board by board; the command writes its frames"""
+    import json
+    import os
+    from tetris_ai3_amd import preview
+    here = os.path.dirname(os.path.abspath(__file__))
+    w = np.array(json.load(open(os.path.join(here, "golden", "v11_champion.json")))["weights_f64"], dtype=np.float32)
+    seeds = O.expand_seeds(71, 3)
+    say("case: preview, v11 champion 3 games x 120 moves")
+    ev = T.ModelEvaluator(model=O.MODEL_V11)
+    games = preview.play(ev, w, 1, seeds, 120, False)
+    ev.close()
+    for g, s in zip(games, seeds):
+        moves, boards, fit = O.play_trace(O.MODEL_V11, w, int(s), 120)
+        assert g["placements"] == moves.shape[0] and np.float32(g["fitness"]) == np.float32(fit)
+        assert np.array_equal(g["boards"], boards)
+    champ = tmp_path / "v11.champ"
+    T.champ_write(str(champ), w.astype(np.float64))
+    out = tmp_path / "frames.txt"
+    assert preview.main(["--champ", str(champ), "--games", "2", "--moves", "30", "--seed", "5", "--every", "10",
+                         "--out", str(out)]) == 0
+    text = out.read_text()
+    assert text.count("game 0") >= 4 and "fitness" in text

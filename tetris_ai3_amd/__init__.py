@@ -102,6 +102,8 @@ def lib():
             "ta3h_pruner_hidden": (C.c_int, [vp, f32p, u32p, C.c_uint32, f32p]),
             "ta3h_pruner_logits": (C.c_int, [vp, f32p, C.c_float, u32p, C.c_uint32, f32p]),
             "ta3h_eval_noalloc": (C.c_int, [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
+            "ta3h_replay": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_uint8), C.c_uint32, u32p,
+                                      C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
             "ta3h_tables": (None, [C.POINTER(C.c_int8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
                                    C.POINTER(C.c_uint8)]),
         }
@@ -584,6 +586,20 @@ def eval_noalloc(kind: int, d_weights: int, num_models: int, d_seeds: int, games
     return lib().ta3h_eval_noalloc(int(kind), C.c_void_p(d_weights), int(num_models), C.c_void_p(d_seeds), int(games),
                                    int(max_moves), C.c_void_p(d_fitness), C.c_void_p(d_placements or None),
                                    C.c_void_p(stream or None))
+
+
+def replay(seed: int, moves, garbage: bool = False):
+    """host replay of one game's move log (bytes hold << 7 | rotation << 5 | (x + 2)): (boards uint32 [n, 10] after
+    each move, pieces placed [n], rows cleared [n]); n < len(moves) after a top-out"""
+    m = np.ascontiguousarray(moves, dtype=np.uint8).reshape(-1)
+    boards = np.zeros((max(m.size, 1), 10), dtype=np.uint32)
+    pieces = np.zeros(max(m.size, 1), dtype=np.uint8)
+    cleared = np.zeros(max(m.size, 1), dtype=np.uint8)
+    n = lib().ta3h_replay(int(seed) & ((1 << 64) - 1), int(garbage), _ptr(m, C.c_uint8), m.size, _ptr(boards, C.c_uint32),
+                          _ptr(pieces, C.c_uint8), _ptr(cleared, C.c_uint8))
+    if n < 0:
+        raise HipError("replay: a move does not fit")
+    return boards[:n], pieces[:n], cleared[:n]
 
 
 def expand_seeds(seed: int, games: int) -> np.ndarray:
