@@ -546,7 +546,9 @@ __device__ inline uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16
 // Feature pass over the candidate board (one streaming pass over the columns, v_board_metrics.hpp:130-225; a
 // column's well flag is settled once its right neighbour is known). Fills f[0..9] (V11) or f[0..15] (16-feature
 // set); the 32-feature set also writes its features 16..31 (bfloat16) straight into the lane's row of featTile.
-template<int KIND>
+// STRIDE: distance between the board's columns, SCR_STRIDE for a lane's LDS board, 1 for a board held in registers
+// (the column loop is then unrolled completely so that the array stays in registers).
+template<int KIND, int STRIDE = SCR_STRIDE>
 __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool heldIsI, float* f,
                                      uint16_t* featTile) {
     constexpr bool WIDE = KIND != MODEL_V11;
@@ -556,9 +558,9 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
     uint32_t holeRows = 0, cPrev = 0;
     int hPrev2 = 0, hPrev = 0;
     uint32_t const lane = lane_id();
-#pragma unroll 2
+#pragma unroll(STRIDE == 1 ? WIDTH : 2)
     for(int x = 0; x < WIDTH; ++x) {
-        uint32_t const c = col[x * SCR_STRIDE];
+        uint32_t const c = col[x * STRIDE];
         int const top = ctz_nz(c | FLOOR);
         int const hx = HEIGHT - top;
         int const n = popc(c);
@@ -817,6 +819,29 @@ __device__ inline int place_into(uint32_t const* parent, uint32_t cols, int col0
     return n;
 }
 
+// placed + clearLines into a board held in registers (b[WIDTH], every loop over columns unrolled)
+__device__ inline int place_regs(uint32_t const* parent, uint32_t cols, int col0, int drop, uint32_t (&b)[WIDTH]) {
+    uint32_t full = FIELD;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        b[x] = parent[x] | (column_of(cols, col0, x) << drop);
+        full &= b[x];
+    }
+    if(full == 0) return 0;
+    int const n = popc(full);
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        uint32_t c = b[x], rows = full;
+        for(int i = 0; i < n; ++i) {
+            uint32_t const row = rows & (~rows + 1); // top-most remaining full row
+            c = ((c & (row - 1)) << 1) | (c & ~((row << 1) - 1));
+            rows ^= row;
+        }
+        b[x] = c;
+    }
+    return n;
+}
+
 __device__ inline uint32_t hist_added(uint32_t h, int cleared) { // clear_hist_t::added (model_defs.hpp:53-55)
     return cleared == 0 ? h : h + (1u << (8 * (cleared - 1)));
 }
@@ -912,14 +937,24 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             int drop = 0;
             legal = fit_span(parent, e.cols, e.col0, drop);
             if(legal) {
-                int const cleared = place_into(parent, e.cols, e.col0, drop, col);
-                uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
-                if constexpr(KIND == MODEL_F32W) {
+                if constexpr(KIND == MODEL_F32W && LDSW) {
+                    // the 4-wave plain kernel (256 VGPRs) keeps the candidate board in registers: no LDS round trip
+                    uint32_t b[WIDTH];
+                    int const cleared = place_regs(parent, e.cols, e.col0, drop, b);
+                    uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
                     float ft[16];
-                    leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
+                    leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
                     store_features16(wide.feat, ft);
                 } else {
-                    score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl, wide.finite));
+                    int const cleared = place_into(parent, e.cols, e.col0, drop, col);
+                    uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
+                    if constexpr(KIND == MODEL_F32W) {
+                        float ft[16];
+                        leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
+                        store_features16(wide.feat, ft);
+                    } else {
+                        score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl, wide.finite));
+                    }
                 }
             }
         }
