@@ -411,6 +411,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef short i16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Weights are read through the constant address space: the per-model base address is wave-uniform, so
@@ -545,12 +546,13 @@ __device__ inline uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16
 
 // Feature pass over the candidate board (one streaming pass over the columns, v_board_metrics.hpp:130-225; a
 // column's well flag is settled once its right neighbour is known). Fills f[0..9] (V11) or f[0..15] (16-feature
-// set); the 32-feature set also writes its features 16..31 (bfloat16) straight into the lane's row of featTile.
+// set); the 32-feature set also writes its features 16..31 (bfloat16) straight into the lane's row of featTile or,
+// for a board held in registers, into g[0..15] (float; the caller rounds them to bfloat16).
 // STRIDE: distance between the board's columns, SCR_STRIDE for a lane's LDS board, 1 for a board held in registers
 // (the column loop is then unrolled completely so that the array stays in registers).
 template<int KIND, int STRIDE = SCR_STRIDE>
 __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool heldIsI, float* f,
-                                     uint16_t* featTile) {
+                                     uint16_t* featTile, float* g = nullptr) {
     constexpr bool WIDE = KIND != MODEL_V11;
     constexpr bool F32 = KIND == MODEL_F32W;
     int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
@@ -584,8 +586,12 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
             ytr += popc((yc ^ (yc >> 1)) & FIELD);
         }
         if(F32) {
-            uint32_t const k = 16 + static_cast<uint32_t>(x);
-            featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(div20(hx));
+            if constexpr(STRIDE == 1) {
+                g[x] = div20(hx);
+            } else {
+                uint32_t const k = 16 + static_cast<uint32_t>(x);
+                featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(div20(hx));
+            }
             uint32_t const hole = FIELD & ~((1u << top) - 1u) & ~c; // empty cells under the column's top
             holeRows |= hole;
             holeCols += hole != 0 ? 1 : 0;
@@ -627,13 +633,16 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         int const d9 = hPrev2 - hPrev; // column 9 against column 8
         deepest = d9 > deepest ? d9 : deepest;
         rowTr += popc(~cPrev & FIELD); // the right wall
-        uint16_t* const c3 = &featTile[feat_index(lane, 3)]; // features 24..31
-        c3[2] = bf16_bits(div20(popc(holeRows)));
-        c3[3] = bf16_bits(div100(rowTr));
-        c3[4] = bf16_bits(div20(deepest));
-        c3[5] = bf16_bits(div200(cells));
-        c3[6] = bf16_bits(div5(holeCols));
-        c3[7] = bf16_bits(div20(minH));
+        float const t[6] = {div20(popc(holeRows)), div100(rowTr), div20(deepest), div200(cells), div5(holeCols),
+                            div20(minH)}; // features 26..31
+        if constexpr(STRIDE == 1) {
+#pragma unroll
+            for(int k = 0; k < 6; ++k) g[10 + k] = t[k];
+        } else {
+            uint16_t* const c3 = &featTile[feat_index(lane, 3)]; // features 24..31
+#pragma unroll
+            for(int k = 0; k < 6; ++k) c3[2 + k] = bf16_bits(t[k]);
+        }
     }
 }
 
@@ -673,7 +682,6 @@ __device__ inline void store_features16(uint16_t* featTile, float const (&f)[16]
 // equals bfloat16(relu(x)) for every x that is not NaN (negative values and -0 have the sign bit set). A NaN
 // (only from non-finite or enormous weights) with the sign bit clear stays NaN.
 __device__ inline bf16x2 relu_bf16x2(float a, float b) {
-    typedef float f32x2 __attribute__((ext_vector_type(2)));
     bf16x2 const v = __builtin_convertvector(f32x2{a, b}, bf16x2);
     i16x2 const m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), i16x2{0, 0});
     return __builtin_bit_cast(bf16x2, m);
@@ -691,12 +699,38 @@ __device__ inline T lds_load(uint32_t addr) {
     return *reinterpret_cast<T const __attribute__((address_space(3)))*>(static_cast<uintptr_t>(addr));
 }
 
+// Lane l holds the 32 bfloat16 features of candidate l in F (word w = features 2 w, 2 w + 1). Returns in x[t] the
+// B operand of column tile t (candidates 16 t .. 16 t + 15): lane 16 q + n gets features 8 q .. 8 q + 7 of
+// candidate 16 t + n, the operand the feature tile in LDS gives. For each word d of an 8-feature chunk, moving word
+// group q of row t to row q of operand t is a 4 x 4 transpose of (register, 16-lane row) blocks:
+// v_permlane32_swap exchanges the off-diagonal 2 x 2 blocks of rows, v_permlane16_swap the single rows inside them.
+__device__ inline void feature_tiles(uint32_t const (&F)[16], bf16x8 (&x)[4]) {
+    u32x4 t[4];
+#pragma unroll
+    for(int d = 0; d < 4; ++d) {
+        uint32_t r0 = F[d], r1 = F[4 + d], r2 = F[8 + d], r3 = F[12 + d];
+        auto const s02 = __builtin_amdgcn_permlane32_swap(r0, r2, false, false); // rows 2, 3 of r0 <-> rows 0, 1 of r2
+        auto const s13 = __builtin_amdgcn_permlane32_swap(r1, r3, false, false);
+        auto const s01 = __builtin_amdgcn_permlane16_swap(s02[0], s13[0], false, false); // rows 1, 3 <-> rows 0, 2
+        auto const s23 = __builtin_amdgcn_permlane16_swap(s02[1], s13[1], false, false);
+        t[0][d] = s01[0];
+        t[1][d] = s01[1];
+        t[2][d] = s23[0];
+        t[3][d] = s23[1];
+    }
+#pragma unroll
+    for(int i = 0; i < 4; ++i) x[i] = __builtin_bit_cast(bf16x8, t[i]);
+}
+
 // LDSW: the model's weights were staged in LDS at byte address wlds (wb unused); else they are read from wb in
 // device memory. Fragment reads are lane-major (lane l reads 16 bytes at (fragment group) * 1024 + 16 l), so a
-// wave reads 1 KB in one piece from either.
-template<int CB, bool LDSW = false>
-__device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave, uint32_t wlds = 0) {
+// wave reads 1 KB in one piece from either. REGF (CB = 4): xin holds the four tiles' B operands, built in
+// registers by feature_tiles (featWave unused); else the operands are read from the wave's feature tile featWave.
+template<int CB, bool LDSW = false, bool REGF = false>
+__device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t const* featWave, uint32_t wlds = 0,
+                                   bf16x8 const* xin = nullptr) {
     static_assert(CB == 1 || CB == 2 || CB == 4);
+    static_assert(!REGF || CB == 4, "the register operands hold all four tiles");
     uint32_t const lane = lane_id(), r16 = lane & 15u, q = lane >> 4;
     bf16x8 const* w1f = reinterpret_cast<bf16x8 const*>(wb + f32w::W1F);
     bf16x8 const* w2f = reinterpret_cast<bf16x8 const*>(wb + f32w::W2F);
@@ -727,7 +761,8 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
         {
             bf16x8 x[CB];
 #pragma unroll
-            for(int c = 0; c < CB; ++c) x[c] = *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * (cb + c) + r16, q)]);
+            for(int c = 0; c < CB; ++c)
+                x[c] = REGF ? xin[c] : *reinterpret_cast<bf16x8 const*>(&featWave[feat_index(16 * (cb + c) + r16, q)]);
 #pragma unroll
             for(uint32_t t = 0; t < 8; ++t) {
                 bf16x8 const a = frag1(t);
@@ -921,11 +956,13 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
     uint64_t ms = 0;
+    constexpr bool REGF = KIND == MODEL_F32W && LDSW; // wide features stay in registers (no feature tile)
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per chunk
         uint32_t const f = base + lane;
         bool legal = false;
         uint32_t score = 0, s = 0, i = 0;
+        uint32_t fpk[REGF ? 16 : 1] = {}; // REGF: the lane's 32 features as bfloat16 pairs (zero for an illegal lane)
         if(f < total) {
             uint32_t const si = seg[f];
             s = si >> 6;
@@ -937,14 +974,16 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             int drop = 0;
             legal = fit_span(parent, e.cols, e.col0, drop);
             if(legal) {
-                if constexpr(KIND == MODEL_F32W && LDSW) {
-                    // the 4-wave plain kernel (256 VGPRs) keeps the candidate board in registers: no LDS round trip
+                if constexpr(REGF) {
+                    // the 4-wave plain kernel keeps the candidate board and its features in registers
                     uint32_t b[WIDTH];
                     int const cleared = place_regs(parent, e.cols, e.col0, drop, b);
                     uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
-                    float ft[16];
-                    leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
-                    store_features16(wide.feat, ft);
+                    float ft[32];
+                    leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, nullptr, ft + 16);
+#pragma unroll
+                    for(int k = 0; k < 16; ++k)
+                        fpk[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{ft[2 * k], ft[2 * k + 1]}), bf16x2));
                 } else {
                     int const cleared = place_into(parent, e.cols, e.col0, drop, col);
                     uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
@@ -958,7 +997,12 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
                 }
             }
         }
-        if constexpr(KIND == MODEL_F32W) { // the whole wave runs the network over the chunk's feature tile
+        if constexpr(REGF) { // the whole wave runs the network over the chunk's 64 candidates
+            bf16x8 x[4];
+            feature_tiles(fpk, x);
+            float const v = value_f32w<WIDE_CB, LDSW, true>(wide.wb, nullptr, wide.wlds, x);
+            if(legal) score = ordered_bits(v);
+        } else if constexpr(KIND == MODEL_F32W) { // the whole wave runs the network over the chunk's feature tile
             wave_sync();
             float const v = value_f32w<WIDE_CB, LDSW>(wide.wb, wide.feat, wide.wlds);
             if(legal) score = ordered_bits(v);
@@ -1331,6 +1375,8 @@ __device__ void emit_stage(EvalArgs const& a, GameLds const& L, PruneLds const* 
 
 // Re-applies the selected placement of rank j (lane j < count) to its parent and writes kept node j of
 // frontier buffer `dst` (seed_frontier / advance_frontier, beam.hpp:307-329, 379-403).
+// REGS: the board is built in registers (col unused), for kernels without the per-lane LDS board.
+template<bool REGS = false>
 __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, uint32_t ti, int layer, uint32_t src,
                                       uint32_t dst, bool root, uint32_t* col) {
     uint32_t const j = lane_id();
@@ -1356,10 +1402,18 @@ __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, 
     uint32_t const* parent = &L.boards[parentBoard * WIDTH];
     int drop = 0;
     fit_span(parent, e.cols, e.col0, drop); // a selected placement always fits
-    int const cleared = place_into(parent, e.cols, e.col0, drop, col);
     uint32_t const d = dst * 32 + j;
+    int cleared;
+    if constexpr(REGS) {
+        uint32_t b[WIDTH];
+        cleared = place_regs(parent, e.cols, e.col0, drop, b);
 #pragma unroll
-    for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = col[x * SCR_STRIDE];
+        for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = b[x];
+    } else {
+        cleared = place_into(parent, e.cols, e.col0, drop, col);
+#pragma unroll
+        for(int x = 0; x < WIDTH; ++x) L.boards[d * WIDTH + x] = col[x * SCR_STRIDE];
+    }
     L.clears[d] = hist_added(parentClears, cleared);
     L.score[d] = static_cast<uint32_t>(key >> 32);
     L.info[d] = node | (rank << 8);
@@ -1401,6 +1455,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                           uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
+    // the wide network's 4-wave plain kernel holds candidate boards and features in registers: no per-lane LDS board
+    constexpr bool WIDE_REGS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1414,7 +1470,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     group_sync<G>();
     uint32_t const ti = uniform(L.treeIdx);
     // the per-lane candidate-board scratch doubles as the pruner logits of a stage (2,176 floats)
-    float* const prs = reinterpret_cast<float*>(col - lane - wv * (WIDTH * SCR_STRIDE));
+    float* const prs = MODE == MODE_PRUNED ? reinterpret_cast<float*>(col - lane - wv * (WIDTH * SCR_STRIDE)) : nullptr;
 
     // stage 0: roots -> top K0
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
@@ -1429,7 +1485,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     if(count == 0) return false;
     if(wv == 0) {
         if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
-        write_frontier(L, top, count, ti, 0, 0, 0, true, col);
+        write_frontier<WIDE_REGS>(L, top, count, ti, 0, 0, 0, true, col);
     }
     group_sync<G>();
 
@@ -1482,7 +1538,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                 emit_stage<G>(a, L, P, total, static_cast<uint32_t>(level), wv);
             }
         }
-        if(wv == 0) write_frontier(L, top, next, ti, level, cur, cur ^ 1u, false, col);
+        if(wv == 0) write_frontier<WIDE_REGS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
         group_sync<G>();
         cur ^= 1u;
         count = next;
@@ -1571,10 +1627,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
-           : (KIND == MODEL_F32W ? (MODE == MODE_PLAIN && G == 4 ? 2 : 3)
+           : (KIND == MODEL_F32W ? 3
                                  : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
-// 4-wave plain kernel at 2, the limit of its LDS with the model's weights staged there)
+// 4-wave plain kernel, with the model's weights staged in LDS, fits 3 workgroups per CU in LDS)
 
 // One workgroup of G waves plays one (model, game) item at a time; in persistent mode it then pulls the
 // next item from the device work queue.
@@ -1582,17 +1638,21 @@ template<int KIND, int G, int MODE = MODE_PLAIN>
 __global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WAVES_PER_SIMD<KIND, MODE, G>, 8)))
 void beam_eval_kernel(EvalArgs a) {
     static_assert(MODE == MODE_PLAIN || G * WIDTH * SCR_STRIDE >= 64 * PRS_ROW, "pruner logits share the scratch");
+    // the wide network's weights, staged once per item (plain kernel): every chunk reads its fragments from LDS,
+    // and the candidate boards and features stay in registers (no per-lane board, no feature tile), which leaves
+    // ~52 KB of LDS per workgroup: 3 workgroups per CU
+    constexpr bool WIDE_LDS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
     __shared__ GameLds L;
-    __shared__ __attribute__((aligned(16))) uint32_t scratch[G * WIDTH * SCR_STRIDE]; // one candidate board per lane
+    // one candidate board per lane
+    __shared__ __attribute__((aligned(16))) uint32_t scratch[WIDE_LDS ? 1 : G * WIDTH * SCR_STRIDE];
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
-    __shared__ std::conditional_t<KIND == MODEL_F32W, FeatLds<G>, uint32_t> featStore; // wide network only
-    // the wide network's weights, staged once per item (plain kernel): every chunk reads its fragments from LDS
-    constexpr bool WIDE_LDS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
+    // wide network's feature tiles (kernels that keep the candidate boards in LDS)
+    __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
-    uint32_t* const col = scratch + wv * (WIDTH * SCR_STRIDE) + lane;
+    uint32_t* const col = WIDE_LDS ? nullptr : scratch + wv * (WIDTH * SCR_STRIDE) + lane;
     uint32_t item = a.persistent ? 0u : blockIdx.x;
     // second phase of a two-phase schedule: the work items are the parked games
     uint32_t const nItems = a.resumePhase ? uniform(*a.resumeCount) : a.items;
@@ -1609,7 +1669,7 @@ void beam_eval_kernel(EvalArgs a) {
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
         WideArgs wide{reinterpret_cast<uint8_t const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride), nullptr,
                       MODE == MODE_TRACE ? pruneLds->segBest : nullptr, false};
-        if constexpr(KIND == MODEL_F32W) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
+        if constexpr(KIND == MODEL_F32W && !WIDE_LDS) wide.feat = reinterpret_cast<FeatLds<G>*>(&featStore)->v[wv];
         if constexpr(WIDE_LDS) { // all waves copy the model's 42,512 bytes; the group sync below publishes them
             uint4 const* const src = reinterpret_cast<uint4 const*>(a.weights + static_cast<uint64_t>(m) * a.weightStride);
             uint4* const dst = reinterpret_cast<uint4*>(&wideWeights);
