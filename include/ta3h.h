@@ -75,7 +75,8 @@ int ta3h_set_garbage(ta3h_evaluator* ev, int enabled);
  * the default here; narrower beams give the moves of the reference's search built with those widths. */
 int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t k2);
 
-/* Launch shape: waves cooperating on one game (1, 2 or 4; a workgroup plays one game at a time),
+/* Launch shape: waves cooperating on one game (1, 2, 4, or 5 except for the wide network; a workgroup plays one
+ * game at a time),
  * persistent work-queue mode (1) or one workgroup per item (0), and the persistent grid in workgroups
  * (0 = fill the device at the occupancy limit). Results never depend on these knobs. */
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks);
@@ -99,10 +100,12 @@ int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uin
  * x ta3h_num_params(kind) floats in the host layout ta3h_eval takes; d_seeds: games seeds; d_fitness: num_models x
  * games floats, model-major; d_placements: the same count of engine.place calls (may be NULL). Launch shape and
  * two-phase budget are the defaults. The search's scratch (work-queue counters, the weights repacked into the
- * network's device layout, parked games, resume lists) lives in one static device array of
- * TA3H_NOALLOC_SCRATCH_BYTES per GPU: 256 B + models x the layout's bytes (V11 448, 16-32-32-1 6,592, wide 42,560,
- * each rounded up to 256 B per section) + 256 B per (model, game) + 8 B per (model, game) + 4 B per model; a
- * larger batch returns TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1: 4.0 MB; 256 x 100 wide: 17.7 MB). Calls
+ * network's device layout, parked games, the resume list and the second phase's work ring) lives in one static
+ * device array of TA3H_NOALLOC_SCRATCH_BYTES per GPU: 256 B + models x the layout's bytes (V11 448, 16-32-32-1
+ * 6,592, wide 42,560, each rounded up to 256 B per section) + 260 B per (model, game) + 4 B per model + 8 B + 4 B
+ * per (model, game) and turn of the second phase (1 + (max_moves - b) / b turns for the budget b = max_moves / 16
+ * clamped to [128, 512]; 15 at 2,000 moves, 16 at 5,000); a larger batch returns TA3H_E_SCRATCH (e.g. 200 x 50 at
+ * 16-32-32-1, 2,000 moves: 4.5 MB; 256 x 100 wide, 5,000 moves: 19.2 MB). Calls
  * that overlap on one GPU share that array and race: serialise them (one stream, or events). Returns TA3H_OK,
  * TA3H_E_ARG, TA3H_E_SCRATCH or TA3H_E_HIP (there is no evaluator to hold an error message). */
 int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_models, const uint64_t* d_seeds, uint32_t games,

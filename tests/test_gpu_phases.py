@@ -1,6 +1,6 @@
-"""GPU: the two-phase schedule (every game to a move budget, then only the parked survivors) changes nothing
-but the order of work: fitness, placements, work counters and trace records equal the one-phase launch's,
-which equal the oracle's."""
+"""GPU: the two-phase schedule (every game to a move budget, then only the parked survivors, in turns of that
+budget from a work ring) changes nothing but the order of work: fitness, placements, work counters and trace
+records equal the one-phase launch's, which equal the oracle's."""
 import json
 import os
 import sys
@@ -26,7 +26,7 @@ def run(ev, w, nm, seeds, cap):
     return fit, pl, ev.fetch_counters()
 
 
-@pytest.mark.parametrize("kind,wpg", [(O.MODEL_V11, 2), (O.MODEL_F16, 4), (O.MODEL_F32W, 4)])
+@pytest.mark.parametrize("kind,wpg", [(O.MODEL_V11, 2), (O.MODEL_F16, 4), (O.MODEL_F16, 5), (O.MODEL_F32W, 4)])
 def test_two_phases_equal_one_phase(kind, wpg):
     say(f"case: two phases kind={kind} wpg={wpg}")
     nm = 3
@@ -46,6 +46,26 @@ def test_two_phases_equal_one_phase(kind, wpg):
     if kind != O.MODEL_F32W:
         host, host_pl = O.eval_host_mt(kind, w, nm, seeds, 120)
         assert one[0].tobytes() == host.tobytes() and (one[1] == host_pl).all()
+
+
+@pytest.mark.parametrize("wpg,grid", [(4, 2), (5, 3), (2, 1)])
+def test_turns_on_few_workgroups(wpg, grid):
+    """more parked games than workgroups: the second phase hands the games round in turns (many per game at a
+    budget of 1-9 moves), re-parking each between turns; the results still equal the one-phase launch's"""
+    say(f"case: turns wpg={wpg} grid={grid}")
+    nm = 4
+    w = np.random.default_rng(53).normal(0, 0.5, nm * O.num_params(O.MODEL_F16)).astype(np.float32)
+    seeds = O.expand_seeds(54, 5)
+    ev = T.ModelEvaluator(model=O.MODEL_F16)
+    ev.set_launch(wpg, True, grid)
+    ev.set_phases(0)
+    one = run(ev, w, nm, seeds, 90)
+    for budget in (1, 4, 9):
+        ev.set_phases(budget)
+        two = run(ev, w, nm, seeds, 90)
+        assert one[0].tobytes() == two[0].tobytes(), budget
+        assert (one[1] == two[1]).all() and (one[2] == two[2]).all(), budget
+    ev.close()
 
 
 def test_two_phase_traces_and_graph():

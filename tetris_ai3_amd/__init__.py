@@ -174,9 +174,10 @@ class ModelEvaluator:
         self._check(lib().ta3h_set_beam_widths(self._h, k0, k1, k2))
         self._widths = (k0, k1, k2)
 
-    def set_launch(self, waves_per_game: int = 2, persistent: bool = True, grid_blocks: int = 0):
-        """waves cooperating on one game (1, 2 or 4), persistent work queue or one workgroup per item,
-        persistent grid size in workgroups (0 = occupancy limit). Results never depend on these."""
+    def set_launch(self, waves_per_game: int = 4, persistent: bool = True, grid_blocks: int = 0):
+        """waves cooperating on one game (1, 2, 4, or 5 except for the wide network), persistent work queue or
+        one workgroup per item, persistent grid size in workgroups (0 = occupancy limit). Results never depend on
+        these."""
         self._check(lib().ta3h_set_launch(self._h, waves_per_game, int(persistent), grid_blocks))
 
     def _args(self, weights, num_models, seeds):
@@ -527,7 +528,7 @@ def _load_cache() -> dict:
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
-             candidates=((1, 0), (2, 0), (4, 0)), phases=(None, 64, 256), repeats: int = 2, use_cache: bool = True):
+             candidates=None, phases=(None, 64, 256), repeats: int = 2, use_cache: bool = True):
     """Picks the launch shape (waves per game, grid) and first-phase budget for this batch shape and keeps it on
     `evaluator` (none of them changes a result). The choice is cached per (network, models x games, max_moves,
     garbage rows, kernel mode) in this process and, when TA3H_AUTOTUNE_CACHE names a JSON file, across processes;
@@ -544,6 +545,8 @@ def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_m
             evaluator.set_phases(phase)
             evaluator.stage(weights, num_models, seeds, max_moves)
             return {"key": key, "chosen": list(_TUNED[key]), "cached": True, "timings_ms": {}}
+    if candidates is None: # (waves per game, grid): 5 waves per game exist for all but the wide network
+        candidates = ((1, 0), (2, 0), (4, 0)) + (((5, 0),) if evaluator.model != MODEL_F32W else ())
     if evaluator._pruned or evaluator._traced or evaluator._logged:
         candidates = ((4, 0),)
     evaluator.stage(weights, num_models, seeds, max_moves)

@@ -197,6 +197,8 @@ constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
 static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 6 <= PARK_WORDS && ENGINE_WORDS <= 64,
               "a parked game fits its record and one wave copies it");
 
+constexpr int MAX_WAVES = 5; // waves cooperating on one game: 1, 2, 4 or 5
+
 struct alignas(16) GameLds {
     uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
     uint32_t clears[COMMITTED + 1];           // clear histogram along the path (model_defs.hpp:37-58)
@@ -210,10 +212,10 @@ struct alignas(16) GameLds {
     uint32_t treeIdx;
     uint32_t stageTotal;
     uint32_t item;
-    uint64_t partTop[4][32];                  // per-wave top-K lists of a stage
-    uint64_t partBest[4];                     // per-wave best leaf keys
-    uint64_t partMask[4];                     // per-wave kept-node / segment legality masks
-    uint32_t partLegal[4];                    // per-wave legal-candidate counts of the item
+    uint64_t partTop[MAX_WAVES][32];          // per-wave top-K lists of a stage
+    uint64_t partBest[MAX_WAVES];             // per-wave best leaf keys
+    uint64_t partMask[MAX_WAVES];             // per-wave kept-node / segment legality masks
+    uint32_t partLegal[MAX_WAVES];            // per-wave legal-candidate counts of the item
     EngineState eng;
     uint32_t moveItem; // the item being played (its move-log row; trace and pruned kernels)
 };
@@ -1047,6 +1049,7 @@ __device__ inline uint64_t merge_sorted(uint64_t a, uint64_t b) {
 // the per-wave exact top-K lists holds the global top-K, so the merged list is exact.
 template<int G, int K, bool TOPK>
 __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& mask, uint32_t wv) {
+    static_assert(G == 1 || G == 2 || G == 4 || G == MAX_WAVES);
     if constexpr(G == 1) {
         return part;
     } else {
@@ -1065,10 +1068,11 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
         mask = m;
         if constexpr(TOPK) {
             uint64_t r = merge_sorted<K>(lane < 32 ? L.partTop[0][lane] : 0, lane < 32 ? L.partTop[1][lane] : 0);
-            if constexpr(G == 4) {
+            if constexpr(G >= 4) {
                 uint64_t const r2 = merge_sorted<K>(lane < 32 ? L.partTop[2][lane] : 0, lane < 32 ? L.partTop[3][lane] : 0);
                 r = merge_sorted<K>(r, r2);
             }
+            if constexpr(G == 5) r = merge_sorted<K>(r, lane < 32 ? L.partTop[4][lane] : 0);
             return r;
         } else {
             uint64_t best = 0;
@@ -1446,6 +1450,36 @@ static_assert(f32w::BYTES % 16 == 0, "copied as 16-byte words");
 
 enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
 
+// Work ring of the second phase (EvalArgs::ring; thread 0 of a workgroup). Returns the next parked game, or
+// RING_EMPTY when no slot is left to take. A workgroup that finds the ring empty ends: every unfinished game is then
+// held by a running workgroup, which appends it and takes a slot again before it could end, so no game is left
+// behind. A taken slot's writer has reserved it and publishes it at once; the wait for it is bounded all the same.
+constexpr uint32_t RING_EMPTY = 0xFFFFFFFFu;
+__device__ inline uint32_t ring_pop(uint32_t* ring) {
+    for(uint32_t tries = 0; tries < (1u << 20); ++tries) {
+        uint32_t h = __hip_atomic_load(&ring[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t const t = __hip_atomic_load(&ring[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if(h >= t) return RING_EMPTY;
+        if(__hip_atomic_compare_exchange_strong(&ring[0], &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)) {
+            for(uint32_t w = 0; w < (1u << 26); ++w) {
+                uint32_t const v = __hip_atomic_load(&ring[2 + h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if(v != 0) return v - 1;
+                __builtin_amdgcn_s_sleep(2);
+            }
+            return RING_EMPTY;
+        }
+    }
+    return RING_EMPTY;
+}
+
+// Appends a parked game (thread 0, after its wave wrote the parked record): the release makes the record visible
+// to the workgroup that takes the slot, on any XCD.
+__device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
+    uint32_t const s = atomicAdd(&ring[1], 1u);
+    if(s < cap) __hip_atomic_store(&ring[2 + s], item + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One move of the beamed search plus its commit (search_move_beam, beam.hpp:483-570; the eval_host loop
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
 // ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
@@ -1621,14 +1655,15 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 // ---------------------------------------------------------------------------------------------------
 // Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (<= 102 VGPRs, no spills);
 // 16-32-32-1 at 5 with 4 waves per game (96 VGPRs; a few loop-invariant values spill to scratch and are
-// reloaded once per stage, 10% faster than 4 waves without spills) and at 4 with 2 waves per game. The trace
+// reloaded once per stage, 10% faster than 4 waves without spills; 5 waves per game likewise) and at 4 with 2
+// waves per game. The trace
 // and pruned kernels' LDS caps them at 4 workgroups of 4 waves per CU, so they get the 128 registers of 4
 // waves per SIMD; one wave per game needs more registers (3 waves per SIMD).
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
            : (KIND == MODEL_F32W ? 3
-                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
+                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G >= 4 ? 5 : 4)));
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
 // 4-wave plain kernel, with the model's weights staged in LDS, fits 3 workgroups per CU in LDS)
 
@@ -1654,16 +1689,14 @@ void beam_eval_kernel(EvalArgs a) {
     uint32_t const lane = lane_id();
     uint32_t* const col = WIDE_LDS ? nullptr : scratch + wv * (WIDTH * SCR_STRIDE) + lane;
     uint32_t item = a.persistent ? 0u : blockIdx.x;
-    // second phase of a two-phase schedule: the work items are the parked games
-    uint32_t const nItems = a.resumePhase ? uniform(*a.resumeCount) : a.items;
     for(;;) {
         if(a.persistent) {
-            if(threadIdx.x == 0) L.item = atomicAdd(a.queue, 1u);
+            // second phase of a two-phase schedule: the parked games, in turns, from the work ring
+            if(threadIdx.x == 0) L.item = a.resumePhase ? ring_pop(a.ring) : atomicAdd(a.queue, 1u);
             group_sync<G>();
             item = uniform(L.item);
         }
-        if(item >= nItems) break;
-        if(a.resumePhase) item = uniform(a.resumeOrder[item]);
+        if(item >= a.items) break;
         // integer division runs on the VALU; re-declare the results wave-uniform (SGPRs)
         uint32_t const m = uniform(item / a.games), g = uniform(item % a.games);
         wptr const w = (wptr)(a.weights + static_cast<uint64_t>(m) * a.weightStride);
@@ -1705,8 +1738,10 @@ void beam_eval_kernel(EvalArgs a) {
         // traced moves: index congruent to the game seed modulo traceEvery (spreads them over the games)
         uint32_t const tracePhase =
             MODE == MODE_TRACE && a.traceEvery != 0 ? uniform(static_cast<uint32_t>(a.seeds[g] % a.traceEvery)) : 0u;
-        uint32_t const limit =
-            a.phaseMoves != 0 && a.persistent && !a.resumePhase && a.phaseMoves < a.maxMoves ? a.phaseMoves : a.maxMoves;
+        // first phase: phaseMoves moves; second phase: a turn of phaseMoves more
+        uint32_t const limit = a.phaseMoves == 0 || !a.persistent ? a.maxMoves
+                               : !a.resumePhase                   ? (a.phaseMoves < a.maxMoves ? a.phaseMoves : a.maxMoves)
+                               : (a.maxMoves - mv > a.phaseMoves ? mv + a.phaseMoves : a.maxMoves);
         bool ended = false;
         for(; mv < limit; ++mv) {
             if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col,
@@ -1719,7 +1754,8 @@ void beam_eval_kernel(EvalArgs a) {
         }
         if(lane == 0) L.partLegal[wv] = nLegal;
         group_sync<G>();
-        if(!ended && limit < a.maxMoves) { // park the game for the second phase
+        if(!ended && limit < a.maxMoves) { // park the game for the second phase or its next turn
+            // (wave 0 writes the whole record, so the release of ring_push, in the same wave, covers it)
             uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
             if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
             if(threadIdx.x == 0) {
@@ -1731,7 +1767,8 @@ void beam_eval_kernel(EvalArgs a) {
                 out[ENGINE_WORDS + 3] = mv;
                 out[ENGINE_WORDS + 4] = static_cast<uint32_t>(nSlots);
                 out[ENGINE_WORDS + 5] = static_cast<uint32_t>(nSlots >> 32);
-                a.resumeList[atomicAdd(a.resumeCount, 1u)] = item;
+                if(a.resumePhase) ring_push(a.ring, a.ringCap, item);
+                else a.resumeList[atomicAdd(a.resumeCount, 1u)] = item;
             }
         } else if(threadIdx.x == 0) {
             a.fitness[item] = final_score(L.eng);
@@ -1784,15 +1821,26 @@ __global__ void reset_counter_kernel(uint32_t* q) {
     if(threadIdx.x == 0) q[0] = 0;
 }
 
+// Clears the work ring's slots past the parked games (before order_resume_kernel fills the first ones).
+__global__ void ring_clear_kernel(uint32_t* ring, uint32_t const* count, uint32_t cap) {
+    uint32_t const n = *count;
+    for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += gridDim.x * blockDim.x)
+        if(s >= n) ring[2 + s] = 0;
+}
+
 // Between the two phases (one workgroup): counts the parked games of every model, then writes the parked list
-// in descending order of that count (a counting sort on min(count, ORDER_BINS - 1); the order inside a bin is
-// whatever the atomics give, which changes no result, only the order of work).
+// into the work ring in descending order of that count (a counting sort on min(count, ORDER_BINS - 1); the order
+// inside a bin is whatever the atomics give, which changes no result, only the order of work).
 constexpr uint32_t ORDER_BINS = 1024;
 __global__ void __launch_bounds__(ORDER_BINS) order_resume_kernel(uint32_t const* list, uint32_t const* count,
-                                                                   uint32_t* order, uint32_t* modelParked,
+                                                                   uint32_t* ring, uint32_t* modelParked,
                                                                    uint32_t models, uint32_t games) {
     __shared__ uint32_t bins[ORDER_BINS];
     uint32_t const t = threadIdx.x, n = *count;
+    if(t == 0) {
+        ring[0] = 0; // head
+        ring[1] = n; // tail
+    }
     for(uint32_t m = t; m < models; m += ORDER_BINS) modelParked[m] = 0;
     bins[t] = 0;
     __syncthreads();
@@ -1816,7 +1864,7 @@ __global__ void __launch_bounds__(ORDER_BINS) order_resume_kernel(uint32_t const
     __syncthreads();
     for(uint32_t i = t; i < n; i += ORDER_BINS) {
         uint32_t const item = list[i];
-        order[atomicAdd(&bins[bin_of(item)], 1u)] = item;
+        ring[2 + atomicAdd(&bins[bin_of(item)], 1u)] = item + 1;
     }
 }
 
@@ -1865,6 +1913,10 @@ hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t 
         case 1: hipLaunchKernelGGL((beam_eval_kernel<KIND, 1>), dim3(blocks), dim3(64), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((beam_eval_kernel<KIND, 2>), dim3(blocks), dim3(128), 0, stream, a); break;
         case 4: hipLaunchKernelGGL((beam_eval_kernel<KIND, 4>), dim3(blocks), dim3(256), 0, stream, a); break;
+        case 5: // the 16-feature and V11 networks only (the wide network's LDS layout is for 4 waves)
+            if constexpr(KIND == MODEL_F32W) return hipErrorInvalidValue;
+            else hipLaunchKernelGGL((beam_eval_kernel<KIND, 5>), dim3(blocks), dim3(320), 0, stream, a);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1935,8 +1987,10 @@ hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wpb, uint32_t block
         if(e != hipSuccess) return e;
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.queue);
         if((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(ring_clear_kernel, dim3(256), dim3(256), 0, stream, a.ring, a.resumeCount, a.ringCap);
+        if((e = hipGetLastError()) != hipSuccess) return e;
         hipLaunchKernelGGL(order_resume_kernel, dim3(1), dim3(ORDER_BINS), 0, stream, a.resumeList, a.resumeCount,
-                           a.resumeOrder, a.modelParked, a.items / a.games, a.games);
+                           a.ring, a.modelParked, a.items / a.games, a.games);
         if((e = hipGetLastError()) != hipSuccess) return e;
         b.resumePhase = 1;
         return launch_model(kind, b, wpb, blocks, stream);
@@ -1962,6 +2016,9 @@ hipError_t occupancy_kind(int wpb, int* n, int mode) {
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 1>, 64, 0);
         case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 2>, 128, 0);
         case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 4>, 256, 0);
+        case 5:
+            if constexpr(KIND == MODEL_F32W) return hipErrorInvalidValue;
+            else return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 5>, 320, 0);
         default: return hipErrorInvalidValue;
     }
 }

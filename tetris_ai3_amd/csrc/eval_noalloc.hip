@@ -128,10 +128,14 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     if(items == 0) return TA3H_OK;
     if(d_weights == nullptr || d_seeds == nullptr || d_fitness == nullptr) return TA3H_E_ARG;
     if(items > 0xFFFFFFFFull / 2) return TA3H_E_ARG;
-    // scratch sections: counters, repacked weights, parked games, resume lists
+    // the two-phase schedule with the default budget: max_moves / 16 clamped to [128, 512]
+    uint32_t const budget = max_moves / 16 < 128 ? 128u : (max_moves / 16 > 512 ? 512u : max_moves / 16);
+    uint64_t const ringSlots = budget < max_moves ? ring_slots(items, max_moves, budget) : 0;
+    if(ringSlots > 0xFFFFFFFFull) return TA3H_E_SCRATCH;
+    // scratch sections: counters, repacked weights, parked games, resume list, per-model counts and work ring
     size_t const wBytes = round256(static_cast<size_t>(num_models) * stride_of(kind) * 4);
     size_t const parkBytes = round256(static_cast<size_t>(items) * PARK_WORDS * 4);
-    size_t const resumeBytes = round256((2 * static_cast<size_t>(items) + num_models) * 4);
+    size_t const resumeBytes = round256((static_cast<size_t>(items) + num_models + 2 + ringSlots) * 4);
     if(256 + wBytes + parkBytes + resumeBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH;
     int dev = 0;
     if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return TA3H_E_HIP;
@@ -165,17 +169,16 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     a.beamWidth[1] = 32;
     a.beamWidth[2] = 32;
     a.persistent = 1;
-    // the two-phase schedule with the default budget: max_moves / 16 clamped to [128, 512]
-    uint32_t const budget = max_moves / 16 < 128 ? 128u : (max_moves / 16 > 512 ? 512u : max_moves / 16);
     if(budget < max_moves) {
         a.phaseMoves = budget;
         a.parked = parked;
         a.resumeList = resume;
         a.resumeCount = counters + 1;
-        a.resumeOrder = resume + items;
-        a.modelParked = resume + 2 * items;
+        a.modelParked = resume + items;
+        a.ring = resume + items + num_models;
+        a.ringCap = static_cast<uint32_t>(ringSlots);
     }
-    int const wpg = kind == MODEL_V11 ? 2 : 4; // the networks' measured launch shapes
+    int const wpg = 4; // the networks' measured launch shape
     int perCU = 0;
     if(max_active_blocks(kind, wpg, 0, &perCU) != hipSuccess) return TA3H_E_HIP;
     uint64_t blocks = static_cast<uint64_t>(cus) * static_cast<uint32_t>(perCU > 0 ? perCU : 1);

@@ -24,6 +24,12 @@ inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 4198
 }
 
 inline constexpr uint32_t PARK_WORDS = 64; // parked game: engine state + counters + move index (256 B)
+
+// Work-ring slots of a two-phase launch: every parked game enters once after the first phase and once after
+// each further turn of `phase` moves it ends unfinished, at most (maxMoves - phase) / phase times.
+inline constexpr uint64_t ring_slots(uint64_t items, uint32_t maxMoves, uint32_t phase) {
+    return phase == 0 ? 0 : items * (1 + (maxMoves - (phase < maxMoves ? phase : maxMoves)) / phase);
+}
 inline constexpr int MAX_DEVICES = 64;      // per-device set-up tables of the library (fixed size)
 
 struct EvalArgs {
@@ -48,17 +54,20 @@ struct EvalArgs {
     uint32_t garbage; // start every game with 0-4 garbage rows (seeded from the game seed)
     // Two-phase schedule (persistent mode): the first launch plays every item for at most phaseMoves moves and
     // parks the unfinished ones (engine state, counters, move index) in parked[item], listing them in
-    // resumeList[0 .. *resumeCount); the second launch (resumePhase = 1) plays only the listed items on to
-    // maxMoves, so the long games all start together, in resumeOrder: the games of the models with the most
-    // parked games first (a model's games have similar lengths, and the longest work should start first).
-    // phaseMoves = 0: one phase.
+    // resumeList[0 .. *resumeCount); the second launch (resumePhase = 1) plays only the parked games, taking them
+    // from a work ring in turns of phaseMoves moves: a game not done after its turn is parked again and appended
+    // to the ring, so the unfinished games advance together and the longest ones end together, instead of the
+    // last ones to start running alone at the end. The ring starts with the parked games of the models with the
+    // most parked games first. phaseMoves = 0: one phase.
     uint32_t phaseMoves;
     uint32_t resumePhase;
     uint32_t* parked;      // [items][PARK_WORDS]
     uint32_t* resumeList;  // [items], in parking order
     uint32_t* resumeCount; // one counter
-    uint32_t* resumeOrder; // [items], resumeList sorted by the model's parked-game count, descending
     uint32_t* modelParked; // [items / games] parked games per model
+    uint32_t* ring;        // [0] head (next slot to take), [1] tail (slots taken by writers), [2 + s] item + 1 of
+                           // slot s, 0 until its writer publishes it
+    uint32_t ringCap;      // slots; ring_slots() bounds what a second launch can append
     // pruned beam: at plies in pruneMask (bit 1, 2 = middle levels, bit 3 = leaves) only the pruner's top pruneK
     // legal placements of every list are scored by the value network (null pruner = full beam)
     PrunerDev const* pruner;

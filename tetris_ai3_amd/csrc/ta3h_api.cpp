@@ -177,7 +177,7 @@ struct ta3h_evaluator {
     int kind = TA3H_MODEL_V11;
     int garbage = 0;
     uint32_t beamWidth[3] = {16, 32, 32}; // beam widths per level (variation_sequences.hpp:22)
-    int wavesPerBlock = 2; // waves cooperating on one game
+    int wavesPerBlock = 4; // waves cooperating on one game (the measured best of every network at the reference batch)
     int persistent = 1;
     uint32_t gridBlocks = 0; // 0: occupancy-derived
     uint32_t lastGrid = 0;
@@ -334,8 +334,9 @@ int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t 
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
     ev->drop_graph(); // captured settings and buffers change
-    if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4)
-        return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4");
+    if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4
+       && !(waves_per_game == 5 && ev->kind != TA3H_MODEL_F32W))
+        return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4 (or 5, except for the wide network)");
     ev->wavesPerBlock = waves_per_game;
     ev->persistent = persistent ? 1 : 0;
     ev->gridBlocks = grid_blocks;
@@ -401,8 +402,10 @@ uint32_t phase_budget(ta3h_evaluator const* ev) {
     return phase != 0 && ev->persistent && phase < ev->maxMoves ? phase : 0u;
 }
 
-size_t resume_words(ta3h_evaluator const* ev) { // list [items], counter, order [items], per-model counts
-    return 2 * static_cast<size_t>(ev->items) + 1 + ev->items / ev->games;
+// resume list [items], its counter, per-model parked counts, then the second phase's work ring (head, tail, slots)
+size_t resume_words(ta3h_evaluator const* ev) {
+    return static_cast<size_t>(ev->items) + 1 + ev->items / ev->games + 2
+           + ta3h::ring_slots(ev->items, ev->maxMoves, phase_budget(ev));
 }
 
 // Device buffers the launch of the staged batch needs beyond the staged ones (the parked games of the two-phase
@@ -413,6 +416,10 @@ int prepare_launch(ta3h_evaluator* ev) {
        && (r = ev->grow(ev->dMoves, ev->movesCap, static_cast<size_t>(ev->items) * ev->maxMoves, "malloc move log")))
         return r;
     if(phase_budget(ev) == 0) return TA3H_OK;
+    if(ta3h::ring_slots(ev->items, ev->maxMoves, phase_budget(ev)) > 0xFFFFFFF0ull) {
+        ev->err = "launch: too many second-phase turns (raise the first-phase budget)";
+        return TA3H_E_ARG;
+    }
     if((r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked")))
         return r;
     return ev->grow(ev->dResume, ev->resumeCap, resume_words(ev), "malloc resume list");
@@ -454,8 +461,9 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
         a.parked = ev->dParked;
         a.resumeList = ev->dResume;
         a.resumeCount = ev->dResume + ev->items;
-        a.resumeOrder = ev->dResume + ev->items + 1;
-        a.modelParked = ev->dResume + 2 * static_cast<size_t>(ev->items) + 1;
+        a.modelParked = ev->dResume + ev->items + 1;
+        a.ring = a.modelParked + ev->items / ev->games;
+        a.ringCap = static_cast<uint32_t>(ta3h::ring_slots(ev->items, ev->maxMoves, phase));
     }
     int mode = 0, wpb = ev->wavesPerBlock;
     if(ev->prunerOn) {
