@@ -221,7 +221,7 @@ def main():
                     help="copy 1 GiB device-to-device first: a kernel of known traffic to scale FETCH_SIZE / "
                          "WRITE_SIZE by in the same profiled run")
     ap.add_argument("--waves-per-game", type=int, default=0,
-                    help="fixed waves per game (1, 2, 4, 5) instead of the autotuned launch shape")
+                    help="fixed waves per game (1, 2, 4) instead of the autotuned launch shape")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prune-steps", type=int, default=3, help="timed launches per pruned-leg setting")
     args = ap.parse_args()

@@ -75,8 +75,7 @@ int ta3h_set_garbage(ta3h_evaluator* ev, int enabled);
  * the default here; narrower beams give the moves of the reference's search built with those widths. */
 int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t k2);
 
-/* Launch shape: waves cooperating on one game (1, 2, 4, or 5 except for the wide network; a workgroup plays one
- * game at a time),
+/* Launch shape: waves cooperating on one game (1, 2 or 4; a workgroup plays one game at a time),
  * persistent work-queue mode (1) or one workgroup per item (0), and the persistent grid in workgroups
  * (0 = fill the device at the occupancy limit). Results never depend on these knobs. */
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks);

@@ -26,7 +26,7 @@ def run(ev, w, nm, seeds, cap):
     return fit, pl, ev.fetch_counters()
 
 
-@pytest.mark.parametrize("kind,wpg", [(O.MODEL_V11, 2), (O.MODEL_F16, 4), (O.MODEL_F16, 5), (O.MODEL_F32W, 4)])
+@pytest.mark.parametrize("kind,wpg", [(O.MODEL_V11, 2), (O.MODEL_V11, 4), (O.MODEL_F16, 4), (O.MODEL_F32W, 4)])
 def test_two_phases_equal_one_phase(kind, wpg):
     say(f"case: two phases kind={kind} wpg={wpg}")
     nm = 3
@@ -48,7 +48,7 @@ def test_two_phases_equal_one_phase(kind, wpg):
         assert one[0].tobytes() == host.tobytes() and (one[1] == host_pl).all()
 
 
-@pytest.mark.parametrize("wpg,grid", [(4, 2), (5, 3), (2, 1)])
+@pytest.mark.parametrize("wpg,grid", [(4, 2), (4, 3), (2, 1)])
 def test_turns_on_few_workgroups(wpg, grid):
     """more parked games than workgroups: the second phase hands the games round in turns (many per game at a
     budget of 1-9 moves), re-parking each between turns; the results still equal the one-phase launch's"""

@@ -55,8 +55,6 @@ def test_argument_errors_without_gpu():
     assert ev.eval(np.zeros(97, dtype=np.float32), 1, np.zeros(0, dtype=np.uint64), 10).size == 0
     with pytest.raises(T.HipError, match="waves_per_game"):
         ev.set_launch(3)
-    with pytest.raises(T.HipError, match="waves_per_game"):
-        T.ModelEvaluator(model=T.MODEL_F32W).set_launch(5) # 5 waves per game: not for the wide network
     with pytest.raises(T.HipError, match="unknown model"):
         T.ModelEvaluator(model=7)
     # pruner arguments are checked before any device work

@@ -175,9 +175,8 @@ class ModelEvaluator:
         self._widths = (k0, k1, k2)
 
     def set_launch(self, waves_per_game: int = 4, persistent: bool = True, grid_blocks: int = 0):
-        """waves cooperating on one game (1, 2, 4, or 5 except for the wide network), persistent work queue or
-        one workgroup per item, persistent grid size in workgroups (0 = occupancy limit). Results never depend on
-        these."""
+        """waves cooperating on one game (1, 2 or 4), persistent work queue or one workgroup per item, persistent
+        grid size in workgroups (0 = occupancy limit). Results never depend on these."""
         self._check(lib().ta3h_set_launch(self._h, waves_per_game, int(persistent), grid_blocks))
 
     def _args(self, weights, num_models, seeds):
@@ -545,8 +544,8 @@ def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_m
             evaluator.set_phases(phase)
             evaluator.stage(weights, num_models, seeds, max_moves)
             return {"key": key, "chosen": list(_TUNED[key]), "cached": True, "timings_ms": {}}
-    if candidates is None: # (waves per game, grid): 5 waves per game exist for all but the wide network
-        candidates = ((1, 0), (2, 0), (4, 0)) + (((5, 0),) if evaluator.model != MODEL_F32W else ())
+    if candidates is None: # (waves per game, grid)
+        candidates = ((1, 0), (2, 0), (4, 0))
     if evaluator._pruned or evaluator._traced or evaluator._logged:
         candidates = ((4, 0),)
     evaluator.stage(weights, num_models, seeds, max_moves)

@@ -197,7 +197,7 @@ constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
 static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 6 <= PARK_WORDS && ENGINE_WORDS <= 64,
               "a parked game fits its record and one wave copies it");
 
-constexpr int MAX_WAVES = 5; // waves cooperating on one game: 1, 2, 4 or 5
+constexpr int MAX_WAVES = 4; // waves cooperating on one game: 1, 2 or 4
 
 struct alignas(16) GameLds {
     uint32_t boards[(COMMITTED + 1) * WIDTH]; // frontier ping-pong: [buf*32 + j], committed at 64
@@ -1049,7 +1049,7 @@ __device__ inline uint64_t merge_sorted(uint64_t a, uint64_t b) {
 // the per-wave exact top-K lists holds the global top-K, so the merged list is exact.
 template<int G, int K, bool TOPK>
 __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& mask, uint32_t wv) {
-    static_assert(G == 1 || G == 2 || G == 4 || G == MAX_WAVES);
+    static_assert(G == 1 || G == 2 || G == 4);
     if constexpr(G == 1) {
         return part;
     } else {
@@ -1067,12 +1067,11 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
         for(int g = 0; g < G; ++g) m |= L.partMask[g];
         mask = m;
         if constexpr(TOPK) {
-            uint64_t r = merge_sorted<K>(lane < 32 ? L.partTop[0][lane] : 0, lane < 32 ? L.partTop[1][lane] : 0);
-            if constexpr(G >= 4) {
-                uint64_t const r2 = merge_sorted<K>(lane < 32 ? L.partTop[2][lane] : 0, lane < 32 ? L.partTop[3][lane] : 0);
-                r = merge_sorted<K>(r, r2);
-            }
-            if constexpr(G == 5) r = merge_sorted<K>(r, lane < 32 ? L.partTop[4][lane] : 0);
+            auto part_of = [&](int g) { return lane < 32 ? L.partTop[g][lane] : uint64_t{0}; };
+            uint64_t r = merge_sorted<K>(part_of(0), part_of(1));
+#pragma unroll
+            for(int g = 2; g + 1 < G; g += 2) r = merge_sorted<K>(r, merge_sorted<K>(part_of(g), part_of(g + 1)));
+            if constexpr(G % 2 == 1) r = merge_sorted<K>(r, part_of(G - 1));
             return r;
         } else {
             uint64_t best = 0;
@@ -1655,15 +1654,14 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 // ---------------------------------------------------------------------------------------------------
 // Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (<= 102 VGPRs, no spills);
 // 16-32-32-1 at 5 with 4 waves per game (96 VGPRs; a few loop-invariant values spill to scratch and are
-// reloaded once per stage, 10% faster than 4 waves without spills; 5 waves per game likewise) and at 4 with 2
-// waves per game. The trace
+// reloaded once per stage, 10% faster than 4 waves without spills) and at 4 with 2 waves per game. The trace
 // and pruned kernels' LDS caps them at 4 workgroups of 4 waves per CU, so they get the 128 registers of 4
 // waves per SIMD; one wave per game needs more registers (3 waves per SIMD).
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
            : (KIND == MODEL_F32W ? 3
-                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G >= 4 ? 5 : 4)));
+                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
 // 4-wave plain kernel, with the model's weights staged in LDS, fits 3 workgroups per CU in LDS)
 
@@ -1913,10 +1911,6 @@ hipError_t launch_kind(EvalArgs const& a, int wpb, uint32_t blocks, hipStream_t 
         case 1: hipLaunchKernelGGL((beam_eval_kernel<KIND, 1>), dim3(blocks), dim3(64), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((beam_eval_kernel<KIND, 2>), dim3(blocks), dim3(128), 0, stream, a); break;
         case 4: hipLaunchKernelGGL((beam_eval_kernel<KIND, 4>), dim3(blocks), dim3(256), 0, stream, a); break;
-        case 5: // the 16-feature and V11 networks only (the wide network's LDS layout is for 4 waves)
-            if constexpr(KIND == MODEL_F32W) return hipErrorInvalidValue;
-            else hipLaunchKernelGGL((beam_eval_kernel<KIND, 5>), dim3(blocks), dim3(320), 0, stream, a);
-            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2016,9 +2010,6 @@ hipError_t occupancy_kind(int wpb, int* n, int mode) {
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 1>, 64, 0);
         case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 2>, 128, 0);
         case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 4>, 256, 0);
-        case 5:
-            if constexpr(KIND == MODEL_F32W) return hipErrorInvalidValue;
-            else return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_eval_kernel<KIND, 5>, 320, 0);
         default: return hipErrorInvalidValue;
     }
 }

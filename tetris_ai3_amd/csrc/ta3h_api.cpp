@@ -334,9 +334,8 @@ int ta3h_set_beam_widths(ta3h_evaluator* ev, uint32_t k0, uint32_t k1, uint32_t 
 int ta3h_set_launch(ta3h_evaluator* ev, int waves_per_game, int persistent, uint32_t grid_blocks) {
     if(!ev) return TA3H_E_ARG;
     ev->drop_graph(); // captured settings and buffers change
-    if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4
-       && !(waves_per_game == 5 && ev->kind != TA3H_MODEL_F32W))
-        return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4 (or 5, except for the wide network)");
+    if(waves_per_game != 1 && waves_per_game != 2 && waves_per_game != 4)
+        return ev->fail_arg("set_launch: waves_per_game must be 1, 2 or 4");
     ev->wavesPerBlock = waves_per_game;
     ev->persistent = persistent ? 1 : 0;
     ev->gridBlocks = grid_blocks;
