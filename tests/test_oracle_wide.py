@@ -113,6 +113,24 @@ def garbage_rows(seed):
     return [nxt() % 10 for _ in range(n)]
 
 
+def bf16_bits(v):
+    u = np.asarray(v, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+
+
+def test_reciprocal_multiply_equals_quotient_after_bf16():
+    """the device normalises the wide network's integer features (0..2,600: heights, holes, hole depths, wells,
+    transitions, cells) by one multiply with the float32 reciprocal instead of the IEEE quotient the oracle takes;
+    the two differ in the last float32 bit for about a fifth of the numerators, and never after rounding to
+    bfloat16, the precision the network reads them at"""
+    x = np.arange(0, 2601, dtype=f32)
+    for c in (5, 20, 40, 100, 200):
+        quotient = (x / f32(c)).astype(f32)
+        product = (x * f32(1.0 / c)).astype(f32)
+        assert (quotient != product).sum() > 0, c # the shortcut is not exact in float32 ...
+        assert (bf16_bits(quotient) == bf16_bits(product)).all(), c # ... but is after the rounding
+
+
 def test_garbage_rows_start_positions():
     counts = set()
     for seed in O.expand_seeds(5, 40):

@@ -535,6 +535,34 @@ __device__ inline float div100(int x) { return div_const(static_cast<float>(x), 
 // set-bit index planes for 23 rows: plane b has bit y set when bit b of y is set
 constexpr uint32_t PLANE0 = 0x2AAAAAu, PLANE1 = 0x4CCCCCu, PLANE2 = 0x70F0F0u, PLANE3 = 0x00FF00u, PLANE4 = 0x7F0000u;
 
+// Sum over a board's filled cells of their row index (bit index y of the column words), for a board in registers:
+// a carry-save count of the ten columns gives every row's filled-cell count as four bit planes (count =
+// p0 + 2 p1 + 4 p2 + 8 p3 per row), and each plane's row-index sum is its five masked population counts.
+__device__ inline int row_index_sum(uint32_t const* b) {
+    auto fa = [](uint32_t x, uint32_t y, uint32_t z, uint32_t& carry) { // full adder of three one-bit planes
+        carry = (x & y) | (z & (x ^ y));
+        return x ^ y ^ z;
+    };
+    uint32_t ka, kb, kc, kd, kf, kg;
+    uint32_t const sa = fa(b[0], b[1], b[2], ka), sb = fa(b[3], b[4], b[5], kb), sc = fa(b[6], b[7], b[8], kc);
+    uint32_t const s1 = fa(sa, sb, sc, kd);
+    uint32_t const p0 = s1 ^ b[9], ke = s1 & b[9];                // weight 1
+    uint32_t const s2 = fa(ka, kb, kc, kf);                        // ka..ke: weight 2
+    uint32_t const p1 = fa(s2, kd, ke, kg);
+    uint32_t const p2 = kf ^ kg, p3 = kf & kg;                     // kf, kg: weight 4
+    uint32_t const planes[4] = {p0, p1, p2, p3};
+    constexpr uint32_t IDX[5] = {PLANE0, PLANE1, PLANE2, PLANE3, PLANE4};
+    int byWeight[8] = {}; // population counts of weight 2^(k + j)
+#pragma unroll
+    for(int k = 0; k < 4; ++k)
+#pragma unroll
+        for(int j = 0; j < 5; ++j) byWeight[k + j] += popc(planes[k] & IDX[j]);
+    int sum = 0;
+#pragma unroll
+    for(int e = 7; e >= 0; --e) sum = 2 * sum + byWeight[e];
+    return sum;
+}
+
 // Candidate boards live in LDS, one per lane: column x of the lane's board is col[x * SCR_STRIDE], so the
 // 64 lanes of a wave touch 64 consecutive words (no bank conflicts) and the column loops stay rolled,
 // which keeps the register footprint small.
@@ -557,6 +585,13 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
                                      uint16_t* featTile, float* g = nullptr) {
     constexpr bool WIDE = KIND != MODEL_V11;
     constexpr bool F32 = KIND == MODEL_F32W;
+    // x / c: the IEEE quotient; the wide network rounds its features to bfloat16, and there one multiply by the
+    // rounded reciprocal gives the same value as the quotient for every numerator 0..2,600 (the features' range;
+    // checked exhaustively on the host)
+    auto nd = [](int x, float c, float r) {
+        if constexpr(F32) return __fmul_rn(static_cast<float>(x), r);
+        else return div_const(static_cast<float>(x), c, r);
+    };
     int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
     int holeCols = 0, cells = 0, rowTr = 0, deepest = 0;
     uint32_t holeRows = 0, cPrev = 0;
@@ -570,9 +605,13 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         int const n = popc(c);
         holes += hx - n;
         maxH = hx > maxH ? hx : maxH;
-        int const s = popc(c & PLANE0) + 2 * popc(c & PLANE1) + 4 * popc(c & PLANE2)
-            + 8 * popc(c & PLANE3) + 16 * popc(c & PLANE4);
-        hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - (s + n * n);
+        if constexpr(STRIDE == 1) { // the row indices' sum comes from row_index_sum after the loop
+            hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - n * n;
+        } else {
+            int const s = popc(c & PLANE0) + 2 * popc(c & PLANE1) + 4 * popc(c & PLANE2)
+                + 8 * popc(c & PLANE3) + 16 * popc(c & PLANE4);
+            hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - (s + n * n);
+        }
         if(x > 0) {
             int const d = hx - hPrev;
             sv += d * d;
@@ -589,10 +628,10 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         }
         if(F32) {
             if constexpr(STRIDE == 1) {
-                g[x] = div20(hx);
+                g[x] = nd(hx, 20.0f, 1.0f / 20.0f);
             } else {
                 uint32_t const k = 16 + static_cast<uint32_t>(x);
-                featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(div20(hx));
+                featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(nd(hx, 20.0f, 1.0f / 20.0f));
             }
             uint32_t const hole = FIELD & ~((1u << top) - 1u) & ~c; // empty cells under the column's top
             holeRows |= hole;
@@ -610,23 +649,24 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         hPrev = hx;
     }
     wells += hPrev2 >= hPrev + 3 ? 1 : 0; // column 9 against its single neighbour
+    if constexpr(STRIDE == 1) hd -= row_index_sum(col);
 #pragma unroll
     for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
     f[4] = c_svNorm[sv];
-    f[5] = div20(holes);
-    f[6] = div200(hd);
+    f[5] = nd(holes, 20.0f, 1.0f / 20.0f);
+    f[6] = nd(hd, 200.0f, 1.0f / 200.0f);
     f[7] = static_cast<float>(maxH);
     f[8] = heldIsI ? 1.0f : 0.0f;
-    f[9] = div5(wells);
+    f[9] = nd(wells, 5.0f, 1.0f / 5.0f);
     if constexpr(WIDE) {
         uint32_t const c0 = clears & 0xFFu, c1 = (clears >> 8) & 0xFFu, c2 = (clears >> 16) & 0xFFu,
                        c3 = clears >> 24;
         int const lines = static_cast<int>(c0 + 2 * c1 + 3 * c2 + 4 * c3);
         int const cscore = -static_cast<int>(c0) - 2 * static_cast<int>(c1) - 3 * static_cast<int>(c2)
             + 10 * static_cast<int>(c3);
-        f[10] = div200(agg);
-        f[11] = div40(ytr);
-        f[12] = div100(bump);
+        f[10] = nd(agg, 200.0f, 1.0f / 200.0f);
+        f[11] = nd(ytr, 40.0f, 1.0f / 40.0f);
+        f[12] = nd(bump, 100.0f, 1.0f / 100.0f);
         f[13] = __fmul_rn(static_cast<float>(lines), 0.1f);
         f[14] = __fmul_rn(static_cast<float>(cscore), 0.1f);
         f[15] = static_cast<float>(maxH - minH);
@@ -635,8 +675,9 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         int const d9 = hPrev2 - hPrev; // column 9 against column 8
         deepest = d9 > deepest ? d9 : deepest;
         rowTr += popc(~cPrev & FIELD); // the right wall
-        float const t[6] = {div20(popc(holeRows)), div100(rowTr), div20(deepest), div200(cells), div5(holeCols),
-                            div20(minH)}; // features 26..31
+        float const t[6] = {nd(popc(holeRows), 20.0f, 1.0f / 20.0f), nd(rowTr, 100.0f, 1.0f / 100.0f),
+                            nd(deepest, 20.0f, 1.0f / 20.0f),    nd(cells, 200.0f, 1.0f / 200.0f),
+                            nd(holeCols, 5.0f, 1.0f / 5.0f),     nd(minH, 20.0f, 1.0f / 20.0f)}; // features 26..31
         if constexpr(STRIDE == 1) {
 #pragma unroll
             for(int k = 0; k < 6; ++k) g[10 + k] = t[k];
