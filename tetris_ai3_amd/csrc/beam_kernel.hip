@@ -592,38 +592,40 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         if constexpr(F32) return __fmul_rn(static_cast<float>(x), r);
         else return div_const(static_cast<float>(x), c, r);
     };
-    int holes = 0, maxH = 0, minH = HEIGHT, sv = 0, bump = 0, hd = 0, wells = 0, agg = 0, ytr = 0;
-    int holeCols = 0, cells = 0, rowTr = 0, deepest = 0;
+    // column sums: heights, filled cells and their squares, row indices of the filled cells
+    int agg = 0, cells = 0, sq = 0, rows = 0;
+    int maxH = 0, minH = HEIGHT, sv = 0, bump = 0, wells = 0, ytr = 0;
+    int holeCols = 0, rowTr = 0, deepest = 0;
     uint32_t holeRows = 0, cPrev = 0;
     int hPrev2 = 0, hPrev = 0;
     uint32_t const lane = lane_id();
 #pragma unroll(STRIDE == 1 ? WIDTH : 2)
     for(int x = 0; x < WIDTH; ++x) {
         uint32_t const c = col[x * STRIDE];
-        int const top = ctz_nz(c | FLOOR);
+        uint32_t const yc = c | FLOOR;
+        int const top = ctz_nz(yc);
         int const hx = HEIGHT - top;
         int const n = popc(c);
-        holes += hx - n;
+        agg += hx;
+        cells += n;
+        sq += n * n;
         maxH = hx > maxH ? hx : maxH;
-        if constexpr(STRIDE == 1) { // the row indices' sum comes from row_index_sum after the loop
-            hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - n * n;
-        } else {
-            int const s = popc(c & PLANE0) + 2 * popc(c & PLANE1) + 4 * popc(c & PLANE2)
-                + 8 * popc(c & PLANE3) + 16 * popc(c & PLANE4);
-            hd += (HEIGHT - 1) * n + ((n * (n + 1)) >> 1) - (s + n * n);
-        }
+        if constexpr(STRIDE != 1) // (a board in registers gets the sum from row_index_sum after the loop)
+            rows += popc(c & PLANE0) + 2 * popc(c & PLANE1) + 4 * popc(c & PLANE2) + 8 * popc(c & PLANE3)
+                + 16 * popc(c & PLANE4);
         if(x > 0) {
             int const d = hx - hPrev;
             sv += d * d;
             if(WIDE) bump += d < 0 ? -d : d;
-            int const lim = hPrev + 3; // wells_t (v_board_metrics.hpp:204-225) for column x-1
-            bool const leftOk = x == 1 ? true : hPrev2 >= lim;
-            wells += (leftOk && hx >= lim) ? 1 : 0;
+            // column x-1 against its lower neighbour (only the right one at the wall): a well of depth >= 3
+            // counts (wells_t, v_board_metrics.hpp:204-225); the wide set also keeps the deepest
+            int const lo = x == 1 ? hx : (hPrev2 < hx ? hPrev2 : hx);
+            int const dp = lo - hPrev;
+            wells += dp >= 3 ? 1 : 0;
+            if(F32) deepest = dp > deepest ? dp : deepest;
         }
         if(WIDE) {
             minH = hx < minH ? hx : minH;
-            agg += hx;
-            uint32_t const yc = c | FLOOR;
             ytr += popc((yc ^ (yc >> 1)) & FIELD);
         }
         if(F32) {
@@ -633,23 +635,24 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
                 uint32_t const k = 16 + static_cast<uint32_t>(x);
                 featTile[feat_index(lane, k >> 3) + (k & 7u)] = bf16_bits(nd(hx, 20.0f, 1.0f / 20.0f));
             }
-            uint32_t const hole = FIELD & ~((1u << top) - 1u) & ~c; // empty cells under the column's top
+            // empty cells under the column's top: c - 1 fills the rows above its top filled cell (all rows of an
+            // empty column)
+            uint32_t const hole = ~(c | (c - 1u)) & FIELD;
             holeRows |= hole;
             holeCols += hole != 0 ? 1 : 0;
-            cells += n;
             rowTr += popc((x > 0 ? (cPrev ^ c) : ~c) & FIELD); // the left wall counts as filled
-            if(x > 0) { // well depth of column x-1: its lower neighbour (only the right one at the wall)
-                int const lo = x == 1 ? hx : (hPrev2 < hx ? hPrev2 : hx);
-                int const dp = lo - hPrev;
-                deepest = dp > deepest ? dp : deepest;
-            }
             cPrev = c;
         }
         hPrev2 = hPrev;
         hPrev = hx;
     }
-    wells += hPrev2 >= hPrev + 3 ? 1 : 0; // column 9 against its single neighbour
-    if constexpr(STRIDE == 1) hd -= row_index_sum(col);
+    int const d9 = hPrev2 - hPrev; // column 9 against its single neighbour, column 8
+    wells += d9 >= 3 ? 1 : 0;
+    if(F32) deepest = d9 > deepest ? d9 : deepest;
+    if constexpr(STRIDE == 1) rows = row_index_sum(col);
+    int const holes = agg - cells; // empty cells under the tops
+    // hole depths, sum over the columns of (HEIGHT - 1) n + n (n + 1) / 2 - (n^2 + the filled rows' indices)
+    int const hd = (HEIGHT - 1) * cells + (cells - sq) / 2 - rows;
 #pragma unroll
     for(int k = 0; k < 4; ++k) f[k] = __fmul_rn(static_cast<float>((clears >> (8 * k)) & 0xFFu), 0.25f);
     f[4] = c_svNorm[sv];
@@ -672,8 +675,6 @@ __device__ inline void leaf_features(uint32_t const* col, uint32_t clears, bool 
         f[15] = static_cast<float>(maxH - minH);
     }
     if constexpr(F32) {
-        int const d9 = hPrev2 - hPrev; // column 9 against column 8
-        deepest = d9 > deepest ? d9 : deepest;
         rowTr += popc(~cPrev & FIELD); // the right wall
         float const t[6] = {nd(popc(holeRows), 20.0f, 1.0f / 20.0f), nd(rowTr, 100.0f, 1.0f / 100.0f),
                             nd(deepest, 20.0f, 1.0f / 20.0f),    nd(cells, 200.0f, 1.0f / 200.0f),
