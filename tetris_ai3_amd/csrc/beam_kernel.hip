@@ -563,9 +563,10 @@ __device__ inline int row_index_sum(uint32_t const* b) {
     return sum;
 }
 
-// Candidate boards live in LDS, one per lane: column x of the lane's board is col[x * SCR_STRIDE], so the
-// 64 lanes of a wave touch 64 consecutive words (no bank conflicts) and the column loops stay rolled,
-// which keeps the register footprint small.
+// Candidate boards in LDS (the wide network's kernels with a feature tile, and the leaf-value kernel), one per
+// lane: column x of the lane's board is col[x * SCR_STRIDE], so the 64 lanes of a wave touch 64 consecutive words
+// (no bank conflicts) and the column loops stay rolled, which keeps the register footprint small. The search's
+// other kernels keep the board in ten registers (place_regs, leaf_features<KIND, 1>): their loops unroll fully.
 constexpr int SCR_STRIDE = 64;
 
 // Features of the 64 candidates of a chunk for the wide network, bfloat16, one 64-byte row per lane; the 16-byte
@@ -1018,7 +1019,15 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             int drop = 0;
             legal = fit_span(parent, e.cols, e.col0, drop);
             if(legal) {
-                if constexpr(REGF) {
+                if constexpr(KIND != MODEL_F32W) { // the candidate board in registers through place, clear, features
+                    uint32_t b[WIDTH];
+                    int const cleared = place_regs(parent, e.cols, e.col0, drop, b);
+                    uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
+                    float ft[16];
+                    leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, nullptr);
+                    if constexpr(KIND == MODEL_F16) score = ordered_bits(value_f16(wl, ft, wide.finite));
+                    else score = ordered_bits(value_v11(wl, ft));
+                } else if constexpr(REGF) {
                     // the 4-wave plain kernel keeps the candidate board and its features in registers
                     uint32_t b[WIDTH];
                     int const cleared = place_regs(parent, e.cols, e.col0, drop, b);
@@ -1028,16 +1037,12 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
 #pragma unroll
                     for(int k = 0; k < 16; ++k)
                         fpk[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{ft[2 * k], ft[2 * k + 1]}), bf16x2));
-                } else {
+                } else { // the wide network's kernels with a feature tile: the lane's LDS board
                     int const cleared = place_into(parent, e.cols, e.col0, drop, col);
                     uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
-                    if constexpr(KIND == MODEL_F32W) {
-                        float ft[16];
-                        leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
-                        store_features16(wide.feat, ft);
-                    } else {
-                        score = ordered_bits(leaf_value<KIND>(col, clears, ((info >> 11) & 1u) != 0, wl, wide.finite));
-                    }
+                    float ft[16];
+                    leaf_features<KIND>(col, clears, ((info >> 11) & 1u) != 0, ft, wide.feat);
+                    store_features16(wide.feat, ft);
                 }
             }
         }
@@ -1530,8 +1535,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                           uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
-    // the wide network's 4-wave plain kernel holds candidate boards and features in registers: no per-lane LDS board
-    constexpr bool WIDE_REGS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
+    // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
+    constexpr bool REG_BOARDS = KIND != MODEL_F32W || (MODE == MODE_PLAIN && G == 4);
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1560,7 +1565,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     if(count == 0) return false;
     if(wv == 0) {
         if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
-        write_frontier<WIDE_REGS>(L, top, count, ti, 0, 0, 0, true, col);
+        write_frontier<REG_BOARDS>(L, top, count, ti, 0, 0, 0, true, col);
     }
     group_sync<G>();
 
@@ -1613,7 +1618,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                 emit_stage<G>(a, L, P, total, static_cast<uint32_t>(level), wv);
             }
         }
-        if(wv == 0) write_frontier<WIDE_REGS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
+        if(wv == 0) write_frontier<REG_BOARDS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
         group_sync<G>();
         cur ^= 1u;
         count = next;
@@ -1718,8 +1723,9 @@ void beam_eval_kernel(EvalArgs a) {
     // ~52 KB of LDS per workgroup: 3 workgroups per CU
     constexpr bool WIDE_LDS = KIND == MODEL_F32W && MODE == MODE_PLAIN && G == 4;
     __shared__ GameLds L;
-    // one candidate board per lane
-    __shared__ __attribute__((aligned(16))) uint32_t scratch[WIDE_LDS ? 1 : G * WIDTH * SCR_STRIDE];
+    // one candidate board per lane (the wide network's kernels with a feature tile) or the pruner's logits
+    constexpr bool SCRATCH = (KIND == MODEL_F32W && !WIDE_LDS) || MODE == MODE_PRUNED;
+    __shared__ __attribute__((aligned(16))) uint32_t scratch[SCRATCH ? G * WIDTH * SCR_STRIDE : 1];
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
@@ -1727,7 +1733,7 @@ void beam_eval_kernel(EvalArgs a) {
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
-    uint32_t* const col = WIDE_LDS ? nullptr : scratch + wv * (WIDTH * SCR_STRIDE) + lane;
+    uint32_t* const col = SCRATCH ? scratch + wv * (WIDTH * SCR_STRIDE) + lane : nullptr;
     uint32_t item = a.persistent ? 0u : blockIdx.x;
     for(;;) {
         if(a.persistent) {
