@@ -1699,16 +1699,16 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 // ---------------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------------
-// Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (<= 102 VGPRs, no spills);
-// 16-32-32-1 at 5 with 4 waves per game (96 VGPRs; a few loop-invariant values spill to scratch and are
-// reloaded once per stage, 10% faster than 4 waves without spills) and at 4 with 2 waves per game. The trace
+// Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (96 VGPRs; with 4 waves per game
+// a few values spill, still 7% faster than 4 waves per SIMD without spills); 16-32-32-1 at 4 (128 VGPRs, no
+// spills; with the candidate board in registers 5% faster than 5 waves per SIMD with spills). The trace
 // and pruned kernels' LDS caps them at 4 workgroups of 4 waves per CU, so they get the 128 registers of 4
 // waves per SIMD; one wave per game needs more registers (3 waves per SIMD).
 template<int KIND, int MODE, int G>
 constexpr int WAVES_PER_SIMD =
     G == 1 ? 3
            : (KIND == MODEL_F32W ? 3
-                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 || G == 4 ? 5 : 4)));
+                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 ? 5 : 4)));
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
 // 4-wave plain kernel, with the model's weights staged in LDS, fits 3 workgroups per CU in LDS)
 
