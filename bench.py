@@ -139,8 +139,10 @@ def champion_leg(T, steps):
     from tetris_ai3_amd import learn
     w = T.champ_read(CHAMP_PATH).astype(np.float32)
     ev = T.ModelEvaluator(model=T.MODEL_F16)
+    # the trace buffer holds every record the 200 games produce (~22.5 M), so the training set, and with it every
+    # figure, does not depend on the order the device plays the games in
     pw, act, info = learn.fit_pruner(ev, w, 1, T.expand_seeds(CHAMP_TRAIN_SEED, 200), CAP, every=64,
-                                     cap_records=16_000_000, records=4_000_000, epochs=10)
+                                     cap_records=24_000_000, records=4_000_000, epochs=10)
     rows = learn.compare_settings(ev, w, 1, T.expand_seeds(CHAMP_HELDOUT_SEED, CHAMP_GAMES), CAP, PRUNE_SPECS, pw,
                                   act, steps)
     ev.close()
