@@ -174,6 +174,9 @@ def _main(argv=None) -> int:
     ap.add_argument("--trace-games", type=int, default=200)
     ap.add_argument("--every", type=int, default=64)
     ap.add_argument("--records", type=int, default=4_000_000)
+    ap.add_argument("--trace-cap", type=int, default=24_000_000,
+                    help="trace buffer records; when the games emit more, which ones are kept depends on the order "
+                         "of work (200 games at cap 2,000, 1 move in 64, emit ~22.5 M)")
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--garbage", action="store_true")
     ap.add_argument("--settings", default="p8:8,p8:12,p4:12,n8-16-16,n4-8-8,n8-16-16+p8:8",
@@ -181,6 +184,7 @@ def _main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--save", default="", help="write the trained pruner (npy: weights then act_range)")
+    ap.add_argument("--load", default="", help="use a pruner written by --save instead of training one")
     args = ap.parse_args(argv)
 
     kind = kinds[args.kind]
@@ -189,9 +193,14 @@ def _main(argv=None) -> int:
     if args.garbage:
         ev.set_garbage(True)
     nm = w.size // ev.num_params
-    pw, act, info = fit_pruner(ev, w, nm, expand_seeds(args.seed, args.trace_games), args.max_moves, args.every,
-                               args.records * 4, args.records, args.epochs)
-    print(json.dumps(dict(info, act_range=act)), flush=True)
+    if args.load:
+        saved = np.load(args.load)
+        pw, act = saved[:-1].astype(np.float32), float(saved[-1])
+        print(json.dumps({"loaded": args.load, "act_range": act}), flush=True)
+    else:
+        pw, act, info = fit_pruner(ev, w, nm, expand_seeds(args.seed, args.trace_games), args.max_moves,
+                                   args.every, args.trace_cap, args.records, args.epochs)
+        print(json.dumps(dict(info, act_range=act)), flush=True)
     if args.save:
         np.save(args.save, np.r_[pw, np.float32(act)])
     rows = compare_settings(ev, w, nm, expand_seeds(args.seed + 1, args.games), args.max_moves,
