@@ -44,6 +44,8 @@ class Counters(C.Structure):
         ("dead_leaf", C.c_uint64),
         ("fallback_mid_best", C.c_uint64),
         ("fallback_leaf_best", C.c_uint64),
+        ("repeat_applies", C.c_uint64),  # of applies / legal: in repeated lists of a middle level or the leaves
+        ("repeat_legal", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <thread>
 #include <memory>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -797,7 +798,10 @@ struct SearchResult { bool some; bool hold; int o, x; float value; };
 struct Counters {
     uint64_t moves = 0, applies = 0, legal = 0, reapplies = 0, placements = 0;
     uint64_t deadMid = 0, deadLeaf = 0, fallbackMidBest = 0, fallbackLeafBest = 0; // dead ends, and moves they decided
+    uint64_t repeatApplies = 0, repeatLegal = 0; // of applies / legal: in repeated lists (ListSeen)
     void add(Counters const& c) {
+        repeatApplies += c.repeatApplies;
+        repeatLegal += c.repeatLegal;
         moves += c.moves;
         applies += c.applies;
         legal += c.legal;
@@ -1022,10 +1026,38 @@ double pruner_loss_grad(float const* w, uint32_t const* samples, int n, double p
     return loss;
 }
 
+// The lists of one level seen so far, for counting repeats: a list is a repeat when an earlier list of its level
+// has the same parent board, parent clear histogram, piece and held-is-I. Its candidates' legality and values
+// equal those of the earlier list (they are functions of these and the placement); the search still scores them
+// (the reference does), the counters record them (the device search scores each such content once).
+struct ListSeen {
+    struct Key {
+        uint32_t w[WIDTH + 2]; // board columns, clear histogram, piece | heldIsI << 3
+        bool operator==(Key const& o) const { return std::memcmp(w, o.w, sizeof(w)) == 0; }
+    };
+    struct Hash {
+        size_t operator()(Key const& k) const {
+            uint64_t h = 0x9E3779B97F4A7C15ull;
+            for(uint32_t v : k.w) h = (h ^ v) * 0xBF58476D1CE4E5B9ull;
+            return static_cast<size_t>(h ^ (h >> 31));
+        }
+    };
+    std::unordered_set<Key, Hash> keys;
+    void clear() { keys.clear(); }
+    bool repeat(Board const& b, uint32_t clears, uint32_t piece, bool heldIsI) {
+        Key k;
+        std::memcpy(k.w, b.c, sizeof(b.c));
+        k.w[WIDTH] = clears;
+        k.w[WIDTH + 1] = piece | (heldIsI ? 8u : 0u);
+        return !keys.insert(k).second;
+    }
+};
+
 struct BeamWorkspace {
     std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
     std::vector<uint8_t> hasChild;
     Frontier kept[2];
+    ListSeen seen;
 };
 
 // search_move_beam (:483-570) at runtime depth = ks.size() + 1
@@ -1110,6 +1142,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         ws.hasChild.assign(kMax, 0);
         bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> level) & 1u) != 0;
         std::vector<uint8_t> legalMid(midSlots, 0);
+        ws.seen.clear();
         for(uint32_t k = 0; k < keptCount; ++k) {
             Node const& pn = tree.layers[level - 1][F.node[k]];
             for(int ci = pn.childBegin; ci < pn.childEnd; ++ci) {
@@ -1118,11 +1151,13 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 bool const hI = heldIsI(cn);
                 size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
                 int const n = LUT.count(piece);
+                bool const rep = ws.seen.repeat(F.board[k], F.clears[k], piece, hI);
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
                 int dropTo[MAX_PLACEMENTS];
                 for(int i = 0; i < n; ++i) {
                     Fit const f = fit(F.board[k], LUT.nth(piece, i).shape);
                     ++cnt.applies;
+                    cnt.repeatApplies += rep;
                     legal[i] = f.legal ? 1 : 0;
                     dropTo[i] = f.drop;
                     keep[i] = legal[i];
@@ -1134,6 +1169,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                     ws.hasChild[k] = 1;
                     if(!keep[i]) continue;
                     ++cnt.legal;
+                    cnt.repeatLegal += rep;
                     Step const st = apply(F.board[k], F.clears[k], LUT.nth(piece, i));
                     (void)dropTo;
                     ws.midScores[base + i] = ordered_bits(model.value(st.clears, st.board, hI));
@@ -1186,6 +1222,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         Frontier const& F = ws.kept[cur];
         int const leafPly = depth - 1;
         bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> leafPly) & 1u) != 0;
+        ws.seen.clear();
         for(uint32_t j = 0; j < keptCount; ++j) {
             Node const& ln = tree.layers[depth - 2][F.node[j]];
             for(int ci = ln.childBegin; ci < ln.childEnd; ++ci) {
@@ -1194,11 +1231,13 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 bool const hI = heldIsI(cn);
                 uint32_t const base = (j * maxLast + (ci - ln.childBegin)) * MP;
                 int const n = LUT.count(piece);
+                bool const rep = ws.seen.repeat(F.board[j], F.clears[j], piece, hI);
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
                 for(int i = 0; i < n; ++i) {
                     legal[i] = fit(F.board[j], LUT.nth(piece, i).shape).legal ? 1 : 0;
                     keep[i] = legal[i];
                     ++cnt.applies;
+                    cnt.repeatApplies += rep;
                 }
                 if(pruneHere) prune_list(pr, F.board[j], static_cast<int>(piece), leafPly, legal, n, keep);
                 bool expanded = false;
@@ -1209,6 +1248,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                     expanded = true;
                     if(!keep[i]) continue;
                     ++cnt.legal;
+                    cnt.repeatLegal += rep;
                     Step const s = apply(F.board[j], F.clears[j], LUT.nth(piece, i));
                     uint32_t const sc = ordered_bits(model.value(s.clears, s.board, hI));
                     consider(sc, base + i);
@@ -1420,6 +1460,8 @@ void to_counters(Counters const& c, ta3o_counters* out) {
     out->dead_leaf += c.deadLeaf;
     out->fallback_mid_best += c.fallbackMidBest;
     out->fallback_leaf_best += c.fallbackLeafBest;
+    out->repeat_applies += c.repeatApplies;
+    out->repeat_legal += c.repeatLegal;
 }
 
 Board board_from(uint32_t const cols[10]) {

@@ -48,6 +48,10 @@ typedef struct {
     uint64_t dead_leaf;   /* leaf lists with no legal placement (beam.hpp:439-440) */
     uint64_t fallback_mid_best;  /* moves whose best key is a middle-level dead-end fallback */
     uint64_t fallback_leaf_best; /* moves whose best key is a leaf-list dead-end fallback */
+    /* of applies / legal: those in repeated lists of a middle level or the leaves (an earlier list of the level
+       has the same parent board, clear histogram, piece and held-is-I); counted, still scored here */
+    uint64_t repeat_applies;
+    uint64_t repeat_legal;
 } ta3o_counters;
 
 int ta3o_num_params(int kind);

@@ -935,9 +935,55 @@ __device__ inline bool held_is_i(GameLds const& L, uint8_t heldSlot) {
 // ---------------------------------------------------------------------------------------------------
 enum StageKind { STAGE_ROOT = 0, STAGE_MID = 1, STAGE_LEAF = 2 };
 
+// segInfo fields: parent board (bits 0-6) | piece << 8 | heldIsI << 11 | tree child << 12 | origin << 20 |
+// SEG_EXISTS | SEG_REPEAT
+constexpr uint32_t SEG_EXISTS = 1u << 30; // the list has placements (every piece has at least 9)
+constexpr uint32_t SEG_REPEAT = 1u << 31; // an earlier list of the stage has the same content: not scored
+__device__ inline uint32_t seg_origin(uint32_t info) { return (info >> 20) & 63u; }
+
+// Hash of kept node q's content (board, clear histogram), read from LDS, for finding equal nodes.
+__device__ inline uint32_t node_hash(GameLds const& L, uint32_t q) {
+    uint32_t h = L.clears[q] * 0x9E3779B1u;
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) {
+        h = (h ^ L.boards[q * WIDTH + x]) * 0x85EBCA6Bu;
+        h ^= h >> 13;
+    }
+    return h ^ (h >> 16);
+}
+
+// Lanes d of `valid` whose BITS-bit key theirs(d) equals this lane's `mine`: one ballot per key bit.
+template<int BITS>
+__device__ inline uint64_t match_keys(uint32_t mine, uint32_t theirs, uint64_t valid) {
+    uint64_t m = valid;
+#pragma unroll
+    for(int b = 0; b < BITS; ++b) {
+        uint64_t const set = __ballot(((theirs >> b) & 1u) != 0);
+        m &= ((mine >> b) & 1u) != 0 ? set : ~set;
+    }
+    return m;
+}
+
+// Kept nodes q and r (frontier indices) hold the same board and clear histogram.
+__device__ inline bool same_nodes(GameLds const& L, uint32_t q, uint32_t r) {
+    bool same = L.clears[q] == L.clears[r];
+#pragma unroll
+    for(int x = 0; x < WIDTH; ++x) same = same && L.boards[q * WIDTH + x] == L.boards[r * WIDTH + x];
+    return same;
+}
+
+// Repeated lists. Two lists of a stage with equal parent board, parent clear histogram, piece and held-is-I have
+// the same legal placements with the same values (a candidate's board, clear histogram and value are functions of
+// these and the placement), which the reference computes twice (beam.hpp:337-372, :407-443): the hold plans reach
+// equal positions by different orders, and equal candidates of one level are both kept and expanded again. With
+// DEDUP a list is marked a repeat of the earliest list with its content (its origin, which is scored): kept nodes
+// are matched by a 16-bit hash of board and clears and checked exactly (a collision falls back to a scan), then
+// lists by (earliest equal node, piece, held-is-I).
+
 // Builds the stage's segment table in LDS. Segment s = 2k + c: child c of kept node k of buffer `cur`
-// (roots: s = root group, parent = the committed board). Returns the candidate count.
-template<int STAGE>
+// (roots: s = root group, parent = the committed board). With DEDUP (middle levels and leaves), repeated lists
+// get no candidates. Returns the candidate count.
+template<int STAGE, bool DEDUP = false>
 __device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur) {
     uint32_t const s = lane_id();
     uint32_t n = 0, info = 0;
@@ -962,6 +1008,32 @@ __device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, ui
             }
         }
     }
+    if(n != 0) info |= SEG_EXISTS | (s << 20);
+    if constexpr(DEDUP && STAGE != STAGE_ROOT) {
+        // lane k < keptCount: the earliest kept node with the same board and clear histogram
+        bool const node = s < keptCount;
+        uint32_t const q = cur * 32 + (s & 31u);
+        uint32_t const h = node ? node_hash(L, q) & 0xFFFFu : 0u;
+        uint64_t const eq = match_keys<16>(h, h, __ballot(node));
+        uint32_t canon = node ? static_cast<uint32_t>(__builtin_ctzll(eq)) : s; // eq holds the lane itself
+        if(canon != s && !same_nodes(L, cur * 32 + canon, q)) {
+            canon = s;
+            for(uint32_t t = 0; t < s; ++t)
+                if(same_nodes(L, cur * 32 + t, q)) {
+                    canon = t;
+                    break;
+                }
+        }
+        // lane s: the earliest list with the same (node content, piece, held-is-I)
+        uint32_t const key = static_cast<uint32_t>(__shfl(static_cast<int>(canon), static_cast<int>(s >> 1)))
+                             | (((info >> 8) & 15u) << 5);
+        uint64_t const same = match_keys<9>(key, key, __ballot(n != 0));
+        uint32_t const origin = n != 0 ? static_cast<uint32_t>(__builtin_ctzll(same)) : s;
+        if(origin != s) {
+            info = (info & ~(63u << 20)) | (origin << 20) | SEG_REPEAT;
+            n = 0;
+        }
+    }
     uint32_t const off = wave_excl_scan(n);
     L.segOff[s] = off;
     L.segInfo[s] = info;
@@ -981,8 +1053,9 @@ __device__ inline void group_sync() {
 
 // Scores this wave's share of a stage's candidates (chunks wv, wv + G, ...).
 //  ROOT / MID: returns the exact top-K keys (score << 32 | ~slot) of the legal candidates with a
-//    nonzero score (select_top, beam.hpp:244-265); MID also returns in *mask the kept nodes that
-//    have at least one legal continuation (score_mid_level, beam.hpp:337-372).
+//    nonzero score (select_top, beam.hpp:244-265); MID also returns in *mask the segments with at least one
+//    legal placement (a kept node has a legal continuation when one of its two does, score_mid_level,
+//    beam.hpp:337-372).
 //  LEAF: returns the best (score, order) key over all legal leaves and, in *mask, the segments with at
 //    least one legal placement (expand_leaves, beam.hpp:407-443).
 // per-game data of the wide network (its weights and the wave's feature tile) and, in trace launches, the
@@ -1067,7 +1140,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             }
             ms |= legal ? (1ull << s) : 0ull;
         } else {
-            if(STAGE == STAGE_MID) ms |= legal ? (1ull << (s >> 1)) : 0ull;
+            if(STAGE == STAGE_MID) ms |= legal ? (1ull << s) : 0ull;
             uint64_t const key = (legal && score != 0) ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0;
             acc = topk_merge<K>(acc, key);
         }
@@ -1129,6 +1202,46 @@ __device__ inline uint64_t combine_parts(GameLds& L, uint64_t part, uint64_t& ma
     }
 }
 
+
+// Kept nodes (lanes 0..31) with a legal continuation, from the legality of their two segments.
+__device__ inline uint64_t node_mask(uint64_t segMask) {
+    uint32_t const lane = lane_id();
+    return __ballot(lane < 32 && ((segMask >> (2 * (lane & 31u))) & 3ull) != 0);
+}
+
+// Legality of the stage's repeated lists, which were not scored: a repeat's placements fit where its origin's do.
+__device__ inline uint64_t repeat_legality(GameLds const& L, uint64_t segMask) {
+    uint32_t const info = L.segInfo[lane_id()];
+    return segMask | __ballot((info & SEG_REPEAT) != 0 && ((segMask >> seg_origin(info)) & 1ull) != 0);
+}
+
+// Adds the candidates of a middle level's repeated lists to its exact top-K of the scored candidates. A repeat's
+// candidate has the score of its origin's candidate at the same placement and a higher slot, so it ranks in the
+// top K of all candidates only if that candidate ranks in the top K of the scored ones: merging, for every kept
+// key, the keys of the same placement in the repeats of its list gives the top K of all candidates, as
+// select_top over the whole level would (beam.hpp:244-265).
+template<int K>
+__device__ inline uint64_t add_repeats(GameLds const& L, uint64_t top) {
+    uint32_t const info = L.segInfo[lane_id()];
+    uint64_t const reps = __ballot((info & SEG_REPEAT) != 0);
+    if(reps == 0) return top;
+    // lane j < K holds kept key j: its list, placement and score; `mine` = the repeats of its list
+    uint32_t const slot = 0xFFFFFFFFu - static_cast<uint32_t>(top);
+    uint32_t const seg = slot / 34, i = slot - seg * 34;
+    uint64_t mine = match_keys<6>(seg & 63u, seg_origin(info), reps);
+    mine = top != 0 ? mine : 0ull;
+    uint64_t const score = top & 0xFFFFFFFF00000000ull;
+    while(__ballot(mine != 0) != 0) { // one repeat per kept key and round
+        uint64_t key = 0;
+        if(mine != 0) {
+            uint32_t const d = static_cast<uint32_t>(__builtin_ctzll(mine));
+            mine &= mine - 1;
+            key = score | (0xFFFFFFFFu - (d * 34 + i));
+        }
+        top = topk_merge<K>(top, key);
+    }
+    return top;
+}
 
 // ---------------------------------------------------------------------------------------------------
 // pruned stages: placement-proposal network on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate)
@@ -1470,11 +1583,11 @@ __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, 
 }
 
 // wave 0 builds the stage's segment table; every wave gets the candidate count
-template<int STAGE, int G>
+template<int STAGE, int G, bool DEDUP = false>
 __device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur,
                                           uint32_t wv) {
     if(wv == 0) {
-        uint32_t const total = build_segments<STAGE>(L, ti, layer, keptCount, cur);
+        uint32_t const total = build_segments<STAGE, DEDUP>(L, ti, layer, keptCount, cur);
         if(lane_id() == 0) L.stageTotal = total;
     }
     group_sync<G>();
@@ -1537,6 +1650,9 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
     constexpr bool REG_BOARDS = KIND != MODEL_F32W || (MODE == MODE_PLAIN && G == 4);
+    // repeated lists of the middle levels and the leaves are not scored (their results are their origins'; a
+    // repeat's pruner logits are its origin's too); trace launches record every candidate of every list
+    constexpr bool DEDUP = MODE != MODE_TRACE;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1574,7 +1690,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     // middle levels 1 and 2 -> top K1, K2
 #pragma unroll
     for(int level = 1; level <= 2; ++level) {
-        total = stage_segments<STAGE_MID, G>(L, ti, level, count, cur, wv);
+        total = stage_segments<STAGE_MID, G, DEDUP>(L, ti, level, count, cur, wv);
         nApply += total;
         uint16_t const* seg = L.candSeg;
         uint32_t evalTotal = total;
@@ -1594,7 +1710,12 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                          : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
+        if constexpr(DEDUP) {
+            mask = repeat_legality(L, mask);
+            top = add_repeats<K2>(L, top);
+        }
         top = lane < a.beamWidth[level] ? top : 0; // narrower beam: the first beamWidth[level] keys
+        mask = node_mask(mask);
         // kept nodes without a legal continuation become leaves at this depth, keyed by root rank
         uint64_t fb = 0;
         if(lane < count && ((mask >> lane) & 1ull) == 0) {
@@ -1625,7 +1746,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     }
 
     // leaves
-    total = stage_segments<STAGE_LEAF, G>(L, ti, 3, count, cur, wv);
+    total = stage_segments<STAGE_LEAF, G, DEDUP>(L, ti, 3, count, cur, wv);
     nApply += total;
     uint16_t const* seg = L.candSeg;
     uint32_t evalTotal = total;
@@ -1655,15 +1776,15 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     }
     nSlots += (evalTotal + 63u) & ~63u;
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
+    if constexpr(DEDUP) mask = repeat_legality(L, mask); // a repeat's leaves never beat its origin's (higher orders)
     if constexpr(MODE == MODE_TRACE) {
         if(traceMove) emit_stage<G>(a, L, P, total, 3u, wv);
     }
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
         uint32_t const s = lane;
-        uint32_t const n = (s < 63 ? L.segOff[s + 1] : total) - L.segOff[s];
         uint64_t fb = 0;
-        if(n != 0 && ((mask >> s) & 1ull) == 0) {
+        if((L.segInfo[s] & SEG_EXISTS) != 0 && ((mask >> s) & 1ull) == 0) {
             uint32_t const k = s >> 1;
             fb = (static_cast<uint64_t>(L.score[cur * 32 + k]) << 32) | (SPAN - s * 34);
         }
