@@ -14,10 +14,14 @@ non-learned way to spend fewer value evaluations, narrower beams, and a narrower
 holds the fastest learned setting within the <=1% mean-score bar (else the one with the smallest loss),
 `pruned.narrow_best` the same for narrower beams, `pruned.champion` the same study on a trained 16-32-32-1
 champion (its own pruner from its own games on training seeds, judged on 1,000 held-out games). `roofline` prices
-the dominant kernel's algorithmic integer/FP32 VALU work (exact per-item candidate counts from the kernel x a
-fixed per-candidate op model) against the MI355X VALU peak; `traffic` is the profiled HBM bytes per launch when a
-PMC summary for the workload is committed under profiles/. `cpu_baseline` times the CPU oracle on a fixed sample
-of the same workload (items spread over the generation) on all host cores, and a smaller sample on one thread.
+the dominant kernel's algorithmic integer/FP32 VALU work (exact per-item counts of the candidates the kernel fits
+and scores x a fixed per-candidate op model) against the MI355X VALU peak. The kernel scores each repeated list
+once (a list whose parent board, clear histogram, piece and held-is-I equal an earlier list's at the same level:
+the reference scores it again), so these counts are the reference's less its repeats; the CPU leg reports the
+repeat share on its sample. `traffic` is the profiled HBM bytes per launch when a PMC summary for the workload is
+committed under profiles/. `cpu_baseline` times the CPU oracle on a fixed sample of the same workload (items spread
+over the generation) on all host cores, and a smaller sample on one thread; the sampled items' fitness,
+placements and counters must equal the GPU's.
 
 `--workload wide` measures the scaled configuration instead: 256 models x 100 games, cap 5,000 pieces,
 32-128-128-1 network on BF16 MFMA, garbage rows, with its own pruned leg (pruner trained on that generation's
@@ -167,11 +171,12 @@ def cpu_info():
     return model, cores
 
 
-def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_pl):
+def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_pl, gpu_cnt):
     """The oracle (this build's scalar C++ restatement of eval_host, model_eval_host.cpp:26-54) on CPU_SAMPLE items
     spread evenly over the generation, one item per task on every host core, each played to completion; then
     the first of those items (about 2,500 placements by the GPU's count) on one thread, as eval_host runs. The
-    sampled items' fitness must equal the GPU's bit for bit."""
+    sampled items' fitness must equal the GPU's bit for bit, and their work counters the GPU's plus the repeated
+    lists the GPU scores once."""
     import oracle as O
     from concurrent.futures import ThreadPoolExecutor
     P, G = O.num_params(model_kind), len(seeds)
@@ -180,15 +185,16 @@ def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_p
 
     def one(i):
         m, g = divmod(i, G)
-        f, p = O.eval_host_mt(model_kind, w[m * P:(m + 1) * P], 1, seeds[g:g + 1], cap, threads=1)
-        return f[0], int(p[0])
+        c = O.Counters()
+        f, p = O.eval_host_mt(model_kind, w[m * P:(m + 1) * P], 1, seeds[g:g + 1], cap, threads=1, counters=c)
+        return f[0], int(p[0]), c
 
     model, cores = cpu_info()
     t0 = time.perf_counter()
     with ThreadPoolExecutor(cores) as pool:
         res = list(pool.map(one, sample))
     dt = time.perf_counter() - t0
-    placements = sum(p for _, p in res)
+    placements = sum(p for _, p, _ in res)
     single, acc = [], 0
     for i in sample:
         if single and acc + int(gpu_pl[i]) > 2500:
@@ -198,9 +204,16 @@ def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_p
     t1 = time.perf_counter()
     res1 = [one(i) for i in single]
     dt1 = time.perf_counter() - t1
-    p1 = sum(p for _, p in res1)
-    fit = np.array([f for f, _ in res], dtype=np.float32)
-    ok = fit.tobytes() == gpu_fit[sample].tobytes() and [p for _, p in res] == [int(gpu_pl[i]) for i in sample]
+    p1 = sum(p for _, p, _ in res1)
+    fit = np.array([f for f, _, _ in res], dtype=np.float32)
+    ok = fit.tobytes() == gpu_fit[sample].tobytes() and [p for _, p, _ in res] == [int(gpu_pl[i]) for i in sample]
+    ok = ok and all(int(gpu_cnt[i, 0]) == c.applies - c.repeat_applies and int(gpu_cnt[i, 1]) == c.legal - c.repeat_legal
+                    for i, (_, _, c) in zip(sample, res))
+    legal = sum(c.legal for _, _, c in res)
+    repeats = {"scored_by_the_reference_search": legal,
+               "in_repeated_lists": sum(c.repeat_legal for _, _, c in res),
+               "share": sum(c.repeat_legal for _, _, c in res) / max(legal, 1),
+               "sample": f"the {CPU_SAMPLE} sampled items, counted by the oracle"}
     return {"value": placements / dt, "unit": "placements/s", "cores": cores, "kind": "port",
             "sample": f"{CPU_SAMPLE} (model, game) items evenly spaced over the generation (item k * {items} / "
                       f"{CPU_SAMPLE}), each played to completion, one per task on {cores} threads "
@@ -208,7 +221,7 @@ def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_p
             "one_thread": {"value": p1 / dt1, "unit": "placements/s",
                            "sample": f"the first {len(single)} of those items on 1 thread ({p1} placements, {dt1:.1f} s)"},
             "nproc": os.cpu_count(), "cpu_model": model,
-            "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}, len(sample), ok
+            "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}, len(sample), ok, repeats
 
 
 def main():
@@ -341,10 +354,11 @@ def main():
             if champ is not None:
                 out["pruned"]["champion"] = champ
     if not args.no_cpu and not wide:
-        base, n, ok = cpu_baseline(kind, w, M, seeds, cap, fit, pl)
+        base, n, ok, repeats = cpu_baseline(kind, w, M, seeds, cap, fit, pl, cnt)
         out["cpu_baseline"] = base
         out["cpu_parity_items"] = n
         out["cpu_parity_ok"] = bool(ok)
+        out["config"]["value_evaluations_in_repeated_lists"] = repeats
     print(json.dumps(out), flush=True)
 
 
