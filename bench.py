@@ -207,7 +207,8 @@ def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_p
     p1 = sum(p for _, p, _ in res1)
     fit = np.array([f for f, _, _ in res], dtype=np.float32)
     ok = fit.tobytes() == gpu_fit[sample].tobytes() and [p for _, p, _ in res] == [int(gpu_pl[i]) for i in sample]
-    ok = ok and all(int(gpu_cnt[i, 0]) == c.applies - c.repeat_applies and int(gpu_cnt[i, 1]) == c.legal - c.repeat_legal
+    ok = ok and all(int(gpu_cnt[i, 0]) == c.applies - c.repeat_applies - c.reuse_applies
+                    and int(gpu_cnt[i, 1]) == c.legal - c.repeat_legal - c.reuse_legal
                     for i, (_, _, c) in zip(sample, res))
     legal = sum(c.legal for _, _, c in res)
     repeats = {"scored_by_the_reference_search": legal,

@@ -46,6 +46,8 @@ class Counters(C.Structure):
         ("fallback_leaf_best", C.c_uint64),
         ("repeat_applies", C.c_uint64),  # of applies / legal: in repeated lists of a middle level or the leaves
         ("repeat_legal", C.c_uint64),
+        ("reuse_applies", C.c_uint64),  # of applies / legal: roots the previous move scored (no line cleared)
+        ("reuse_legal", C.c_uint64),
     ]
 
     def as_dict(self):

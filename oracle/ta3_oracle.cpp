@@ -799,7 +799,10 @@ struct Counters {
     uint64_t moves = 0, applies = 0, legal = 0, reapplies = 0, placements = 0;
     uint64_t deadMid = 0, deadLeaf = 0, fallbackMidBest = 0, fallbackLeafBest = 0; // dead ends, and moves they decided
     uint64_t repeatApplies = 0, repeatLegal = 0; // of applies / legal: in repeated lists (ListSeen)
+    uint64_t reuseApplies = 0, reuseLegal = 0;   // of applies / legal: roots scored by the previous move (RootReuse)
     void add(Counters const& c) {
+        reuseApplies += c.reuseApplies;
+        reuseLegal += c.reuseLegal;
         repeatApplies += c.repeatApplies;
         repeatLegal += c.repeatLegal;
         moves += c.moves;
@@ -1053,11 +1056,23 @@ struct ListSeen {
     }
 };
 
+// Roots scored by the previous move. When the committed placement cleared no line, the next move's root lists
+// (the committed board, an empty clear histogram, a piece and held-is-I) are level-1 lists of the previous search's
+// chosen root node whenever piece and held-is-I agree: the same candidates with the same values. The device search
+// takes those values from the previous move when every root list has such a match; the search here still scores
+// them (the reference does) and the counters record them.
+struct RootReuse {
+    bool valid = false;     // the last committed placement cleared no line
+    uint32_t keys[2] = {};  // the chosen root's level-1 lists: piece | heldIsI << 3 | 16 (0 = no list)
+    uint32_t level1[64] = {}; // this search's level-1 lists of kept root k: [2 k + child]
+};
+
 struct BeamWorkspace {
     std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
     std::vector<uint8_t> hasChild;
     Frontier kept[2];
     ListSeen seen;
+    RootReuse reuse;
 };
 
 // search_move_beam (:483-570) at runtime depth = ks.size() + 1
@@ -1088,6 +1103,13 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     Board const& board = g.board;
 
     // stage 0: score_roots (:283-301)
+    bool reuseRoots = ws.reuse.valid; // RootReuse: every root list matches a level-1 list of the last chosen root
+    for(size_t gi = 0; gi < tree.layers[0].size(); ++gi) {
+        Node const& node = tree.layers[0][gi];
+        uint32_t const key = slot[node.slot] | (heldIsI(node) ? 8u : 0u) | 16u;
+        reuseRoots = reuseRoots && (ws.reuse.keys[0] == key || ws.reuse.keys[1] == key);
+    }
+    std::fill(std::begin(ws.reuse.level1), std::end(ws.reuse.level1), 0u);
     ws.rootScores.assign(static_cast<size_t>(TREES.maxRootGroups[depth]) * MP, 0u);
     for(size_t gi = 0; gi < tree.layers[0].size(); ++gi) {
         Node const& node = tree.layers[0][gi];
@@ -1097,6 +1119,8 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
             Step const s = apply(board, 0, LUT.nth(p0, i));
             ++cnt.applies;
             if(s.legal) ++cnt.legal;
+            cnt.reuseApplies += reuseRoots;
+            cnt.reuseLegal += reuseRoots && s.legal;
             ws.rootScores[gi * MP + i] = s.legal ? ordered_bits(model.value(s.clears, s.board, hI)) : 0u;
         }
     }
@@ -1152,6 +1176,8 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
                 int const n = LUT.count(piece);
                 bool const rep = ws.seen.repeat(F.board[k], F.clears[k], piece, hI);
+                if(level == 1 && k < 32 && ci - pn.childBegin < 2)
+                    ws.reuse.level1[2 * k + (ci - pn.childBegin)] = piece | (hI ? 8u : 0u) | 16u;
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
                 int dropTo[MAX_PLACEMENTS];
                 for(int i = 0; i < n; ++i) {
@@ -1281,6 +1307,9 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     ++cnt.moves;
     if(bestFallback == 1) ++cnt.fallbackMidBest;
     if(bestFallback == 2) ++cnt.fallbackLeafBest;
+    // the chosen root's level-1 lists, for the next move's RootReuse (rank = kept root index at level 1)
+    ws.reuse.keys[0] = rootRank < 32 ? ws.reuse.level1[2 * rootRank] : 0u;
+    ws.reuse.keys[1] = rootRank < 32 ? ws.reuse.level1[2 * rootRank + 1] : 0u;
     return {true, rn.rootHold, pl0.o, pl0.x, ordered_bits_inv(static_cast<uint32_t>(bestKey >> 32))};
 }
 
@@ -1402,6 +1431,7 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
     std::vector<uint32_t> const widths(g_widths, g_widths + 3);
     Stats stats;
     int rec = 0;
+    ws.reuse.valid = false; // the first move has no previous search
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
         TraceOut tr;
         tr.out = traceOut;
@@ -1411,6 +1441,7 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
         if(!r.some) break;
         if(r.hold) engine_hold(g);
         int const cleared = engine_place(g, r.o, r.x);
+        ws.reuse.valid = cleared == 0;
         ++cnt.placements;
         if(moves) {
             moves[rec * 4 + 0] = r.hold ? 1 : 0;
@@ -1462,6 +1493,8 @@ void to_counters(Counters const& c, ta3o_counters* out) {
     out->fallback_leaf_best += c.fallbackLeafBest;
     out->repeat_applies += c.repeatApplies;
     out->repeat_legal += c.repeatLegal;
+    out->reuse_applies += c.reuseApplies;
+    out->reuse_legal += c.reuseLegal;
 }
 
 Board board_from(uint32_t const cols[10]) {

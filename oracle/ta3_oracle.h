@@ -52,6 +52,11 @@ typedef struct {
        has the same parent board, clear histogram, piece and held-is-I); counted, still scored here */
     uint64_t repeat_applies;
     uint64_t repeat_legal;
+    /* of applies / legal: root candidates equal to candidates the previous move's search scored (the committed
+       placement cleared no line and every root list is a level-1 list of the chosen root node: same board, empty
+       clear histogram, piece and held-is-I); counted, still scored here */
+    uint64_t reuse_applies;
+    uint64_t reuse_legal;
 } ta3o_counters;
 
 int ta3o_num_params(int kind);

@@ -85,7 +85,8 @@ def test_config1_generation_bit_exact_sample_and_properties():
         m, g = divmod(i, G)
         c = O.Counters()
         f, p = O.eval_host_mt(kind, w[m * P:(m + 1) * P], 1, seeds[g:g + 1], cap, threads=1, counters=c)
-        return f[0], p[0], c.applies - c.repeat_applies, c.legal - c.repeat_legal  # repeated lists scored once
+        # repeated lists are scored once, the roots the previous move scored are taken from it
+        return f[0], p[0], c.applies - c.repeat_applies - c.reuse_applies, c.legal - c.repeat_legal - c.reuse_legal
 
     say(f"case: config1 oracle on {len(items)} items ({int(pl[items].sum())} placements)")
     with ThreadPoolExecutor(16) as pool:

@@ -112,8 +112,9 @@ def test_zero_skipping_matches_oracle(kind):
 
 def test_work_counters():
     """Per-item counters: candidates fitted and legal equal the oracle's less its repeated lists (an earlier list
-    of the level with the same parent board, clear histogram, piece and held-is-I), which the device scores once;
-    candidate lanes issued are whole
+    of the level with the same parent board, clear histogram, piece and held-is-I), which the device scores once,
+    and less the roots the previous move scored (the committed placement cleared no line); candidate lanes issued
+    are whole
     64-lane chunks covering every fitted candidate of a plain beam, with less than one chunk of padding per
     stage (four stages per move)"""
     kind, nm = T.MODEL_F16, 3
@@ -130,9 +131,9 @@ def test_work_counters():
     for m in range(nm):
         c = O.Counters()
         O.eval_host(kind, w[m * P:(m + 1) * P], seeds, 100, c)
-        assert c.repeat_legal > 0
-        assert int(cnt[m * 4:(m + 1) * 4, 0].sum()) == c.applies - c.repeat_applies
-        assert int(cnt[m * 4:(m + 1) * 4, 1].sum()) == c.legal - c.repeat_legal
+        assert c.repeat_legal > 0 and c.reuse_legal > 0
+        assert int(cnt[m * 4:(m + 1) * 4, 0].sum()) == c.applies - c.repeat_applies - c.reuse_applies
+        assert int(cnt[m * 4:(m + 1) * 4, 1].sum()) == c.legal - c.repeat_legal - c.reuse_legal
     applies, legal, lanes = cnt[:, 0], cnt[:, 1], cnt[:, 2]
     assert (lanes % 64 == 0).all() and (legal <= applies).all() and (applies <= lanes).all()
     assert (lanes - applies < 4 * 64 * (pl.astype(np.int64) + 1)).all()

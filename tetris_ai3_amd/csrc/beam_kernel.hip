@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <mutex>
 #include <type_traits>
@@ -194,7 +195,9 @@ struct EngineState {
 };
 
 constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
-static_assert(sizeof(EngineState) % 4 == 0 && ENGINE_WORDS + 6 <= PARK_WORDS && ENGINE_WORDS <= 64,
+// root scores carried to the next move (RootCacheLds): valid, two list keys, two lists of 34 scores
+constexpr uint32_t PARK_RC = ENGINE_WORDS + 6, RC_WORDS = 3 + 2 * 34;
+static_assert(sizeof(EngineState) % 4 == 0 && PARK_RC + RC_WORDS <= PARK_WORDS && ENGINE_WORDS <= 64,
               "a parked game fits its record and one wave copies it");
 
 constexpr int MAX_WAVES = 4; // waves cooperating on one game: 1, 2 or 4
@@ -230,6 +233,22 @@ struct alignas(16) PruneLds {
     uint32_t keptCount;     // pruned stages: candidates kept for the value network
 };
 
+
+// Root lists scored by the previous move (plain V11 / 16-32-32-1 kernels). When the committed placement cleared no
+// line, the next move's root lists (the committed board, an empty clear histogram, a piece, held-is-I) are level-1
+// lists of the chosen root node whenever piece and held-is-I agree, which the previous search scored: the same
+// candidates with the same values. Level 1 keeps every score here; the commit copies the chosen root's two lists
+// (a repeat's from its origin) to `next`; a root stage whose lists all match takes its keys from there. A parked
+// game carries `valid`, `nextKey` and `next` in its record, so the choice depends only on the game.
+struct alignas(16) RootCacheLds {
+    uint32_t level1[32 * 34]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
+    uint32_t info1[32];       // segInfo of the level-1 lists
+    uint32_t valid;           // the last committed placement cleared no line (the lists below apply)
+    uint32_t nextKey[2];      // piece | heldIsI << 3 | 16 of the chosen root's level-1 lists (0: none)
+    uint32_t next[2 * 34];    // their scores
+};
+static_assert(offsetof(RootCacheLds, next) == offsetof(RootCacheLds, valid) + 12 && RC_WORDS == 3 + 2 * 34,
+              "valid, nextKey and next are one block of RC_WORDS words (a parked game's copy)");
 
 __device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:33-45
     uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
@@ -1070,7 +1089,7 @@ struct WideArgs {
 
 template<int KIND, int STAGE, int K, int G, bool TRACE = false, int WIDE_CB = 4, bool LDSW = false>
 __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
-                                     uint64_t* mask, uint32_t& nLegal, WideArgs const& wide) {
+                                     uint64_t* mask, uint32_t& nLegal, WideArgs const& wide, uint32_t* keep = nullptr) {
     uint32_t const lane = lane_id();
     uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
     uint64_t ms = 0;
@@ -1132,6 +1151,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
         uint32_t const slot = s * 34 + i;
+        if(keep != nullptr && f < total) keep[slot] = legal ? score : 0u; // level-1 scores for the next move's roots
         if constexpr(STAGE == STAGE_LEAF) {
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
             acc = key > acc ? key : acc;
@@ -1643,9 +1663,37 @@ __device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
 // ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
 // pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
+// The root stage from the previous move's scores (RootCacheLds): when every root list matches one of the chosen
+// root's level-1 lists, each wave builds the exact top-K of the root keys from them (no scoring) and returns true.
+template<int K>
+__device__ inline bool root_reuse(GameLds const& L, RootCacheLds const& RC, uint32_t total, uint64_t& top) {
+    if(uniform(RC.valid) == 0) return false;
+    uint32_t const lane = lane_id();
+    uint32_t const info = L.segInfo[lane & 1u]; // lanes 0, 1: the root lists
+    uint32_t const key = ((info >> 8) & 15u) | 16u;
+    uint32_t const c = RC.nextKey[0] == key ? 0u : (RC.nextKey[1] == key ? 1u : 2u);
+    if(__ballot(lane < 2 && (info & SEG_EXISTS) != 0 && c == 2u) != 0) return false;
+    uint32_t const c0 = __builtin_amdgcn_readlane(c, 0), c1 = __builtin_amdgcn_readlane(c, 1);
+    uint64_t acc = 0;
+    for(uint32_t base = 0; base < total; base += 64) {
+        uint32_t const f = base + lane;
+        uint64_t k = 0;
+        if(f < total) {
+            uint32_t const si = L.candSeg[f];
+            uint32_t const s = si >> 6, i = si & 63u;
+            uint32_t const score = RC.next[(s == 0 ? c0 : c1) * 34 + i];
+            k = score != 0 ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - (s * 34 + i))) : 0ull;
+        }
+        acc = topk_merge<K>(acc, k);
+    }
+    top = acc;
+    return true;
+}
+
 template<int KIND, int G, int MODE>
-__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, WideArgs const& wide, uint32_t wv,
-                          uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
+__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheLds* RC, wptr w, WideArgs const& wide,
+                          uint32_t wv, uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal,
+                          uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
@@ -1653,6 +1701,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     // repeated lists of the middle levels and the leaves are not scored (their results are their origins'; a
     // repeat's pruner logits are its origin's too); trace launches record every candidate of every list
     constexpr bool DEDUP = MODE != MODE_TRACE;
+    // roots from the previous move's level-1 scores (RootCacheLds; the wide kernel's LDS has no room for them)
+    constexpr bool ROOT_CACHE = MODE == MODE_PLAIN && KIND != MODEL_F32W;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1670,11 +1720,16 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 
     // stage 0: roots -> top K0
     uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
-    nApply += total;
     uint64_t mask = 0;
-    uint64_t top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
-    nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
-    top = combine_parts<G, K0, true>(L, top, mask, wv);
+    uint64_t top = 0;
+    bool reused = false;
+    if constexpr(ROOT_CACHE) reused = root_reuse<K0>(L, *RC, total, top);
+    if(!reused) {
+        nApply += total;
+        top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
+        nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
+        top = combine_parts<G, K0, true>(L, top, mask, wv);
+    }
     // a narrower beam keeps the first beamWidth[0] keys of the exact, descending top-K0 list
     top = lane < a.beamWidth[0] ? top : 0;
     uint32_t count = __popcll(__ballot(top != 0));
@@ -1692,6 +1747,13 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
     for(int level = 1; level <= 2; ++level) {
         total = stage_segments<STAGE_MID, G, DEDUP>(L, ti, level, count, cur, wv);
         nApply += total;
+        uint32_t* keep = nullptr;
+        if constexpr(ROOT_CACHE) {
+            if(level == 1) {
+                if(wv == 0 && lane < 32) RC->info1[lane] = L.segInfo[lane];
+                keep = RC->level1;
+            }
+        }
         uint16_t const* seg = L.candSeg;
         uint32_t evalTotal = total;
         if constexpr(MODE == MODE_PRUNED) {
@@ -1706,7 +1768,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
             }
         }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide)
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep)
                          : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
@@ -1795,6 +1857,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
 
     // decode_move (beam.hpp:450-470) and commit on one lane
     uint32_t const order = SPAN - static_cast<uint32_t>(bestKey);
+    int cleared = -1;
     if(wv == 0 && lane == 0) {
         uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
         uint32_t const rootSlot = L.rootSlot[rank];
@@ -1808,10 +1871,28 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, wptr w, Wi
                 a.moves[static_cast<uint64_t>(L.moveItem) * a.maxMoves + eng.placements] =
                     static_cast<uint8_t>((rn.rootHold ? 0x80u : 0u) | (e.ox & 3u) << 5 | (e.ox >> 2));
         if(rn.rootHold) engine_hold(eng);
-        int const cleared = engine_place(eng, o, x);
+        cleared = engine_place(eng, o, x);
         ++eng.placements;
         if(cleared >= 0) stats_advance(eng, cleared);
         else eng.gameOver = 1;
+    }
+    if constexpr(ROOT_CACHE) { // the chosen root's level-1 lists become the next move's root lists (RootCacheLds)
+        if(wv == 0) {
+            uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
+            uint32_t keys[2];
+#pragma unroll
+            for(uint32_t c = 0; c < 2; ++c) {
+                uint32_t const li = RC->info1[2 * rank + c];
+                keys[c] = (li & SEG_EXISTS) != 0 ? (((li >> 8) & 15u) | 16u) : 0u;
+                uint32_t const src = (li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c;
+                if(lane < 34) RC->next[c * 34 + lane] = RC->level1[src * 34 + lane];
+            }
+            if(lane == 0) {
+                RC->nextKey[0] = keys[0];
+                RC->nextKey[1] = keys[1];
+                RC->valid = cleared == 0 ? 1u : 0u;
+            }
+        }
     }
     group_sync<G>();
     return uniform(L.eng.gameOver) == 0;
@@ -1852,6 +1933,9 @@ void beam_eval_kernel(EvalArgs a) {
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
+    constexpr bool ROOT_CACHE = MODE == MODE_PLAIN && KIND != MODEL_F32W;
+    __shared__ std::conditional_t<ROOT_CACHE, RootCacheLds, uint32_t> rcStore;
+    RootCacheLds* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheLds*>(&rcStore) : nullptr;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = SCRATCH ? scratch + wv * (WIDTH * SCR_STRIDE) + lane : nullptr;
@@ -1896,10 +1980,17 @@ void beam_eval_kernel(EvalArgs a) {
             nLegal = wv == 0 ? uniform(park[ENGINE_WORDS + 2]) : 0u;
             mv = uniform(park[ENGINE_WORDS + 3]);
             nSlots = uniform(static_cast<uint64_t>(park[ENGINE_WORDS + 4]) | static_cast<uint64_t>(park[ENGINE_WORDS + 5]) << 32);
+            if constexpr(ROOT_CACHE) {
+                if(wv == 0) {
+                    uint32_t* const rc = &RC->valid; // valid, nextKey[2], next[68]: contiguous
+                    for(uint32_t k = lane; k < RC_WORDS; k += 64) rc[k] = park[PARK_RC + k];
+                }
+            }
         } else if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
             engine_reset(L.eng, a.seeds[g]);
             if(a.garbage) add_garbage_rows(L.eng, a.seeds[g]);
+            if constexpr(ROOT_CACHE) RC->valid = 0; // the first move has no previous search
         }
         group_sync<G>();
         // traced moves: index congruent to the game seed modulo traceEvery (spreads them over the games)
@@ -1911,7 +2002,7 @@ void beam_eval_kernel(EvalArgs a) {
                                : (a.maxMoves - mv > a.phaseMoves ? mv + a.phaseMoves : a.maxMoves);
         bool ended = false;
         for(; mv < limit; ++mv) {
-            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, w, wide, wv, col,
+            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, RC, w, wide, wv, col,
                                                                          MODE == MODE_TRACE && a.traceEvery != 0
                                                                              && mv % a.traceEvery == tracePhase,
                                                                          nApply, nLegal, nSlots)) {
@@ -1925,6 +2016,12 @@ void beam_eval_kernel(EvalArgs a) {
             // (wave 0 writes the whole record, so the release of ring_push, in the same wave, covers it)
             uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
             if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
+            if constexpr(ROOT_CACHE) {
+                if(wv == 0) {
+                    uint32_t const* const rc = &RC->valid;
+                    for(uint32_t k = lane; k < RC_WORDS; k += 64) out[PARK_RC + k] = rc[k];
+                }
+            }
             if(threadIdx.x == 0) {
                 uint32_t legal = 0;
                 for(int k = 0; k < G; ++k) legal += L.partLegal[k];

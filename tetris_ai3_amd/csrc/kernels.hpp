@@ -23,7 +23,7 @@ namespace f32w {
 inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 41984, B3 = 42496, BYTES = 42512;
 }
 
-inline constexpr uint32_t PARK_WORDS = 64; // parked game: engine state + counters + move index (256 B)
+inline constexpr uint32_t PARK_WORDS = 128; // parked game: engine state, counters, move index, root scores (512 B)
 
 // Work-ring slots of a two-phase launch: every parked game enters once after the first phase and once after
 // each further turn of `phase` moves it ends unfinished, at most (maxMoves - phase) / phase times.
