@@ -1901,16 +1901,14 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
 // ---------------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------------
-// Waves per SIMD the register allocation is held to (measured fastest): V11 at 5 (96 VGPRs; with 4 waves per game
-// a few values spill, still 7% faster than 4 waves per SIMD without spills); 16-32-32-1 at 4 (128 VGPRs, no
-// spills; with the candidate board in registers 5% faster than 5 waves per SIMD with spills). The trace
-// and pruned kernels' LDS caps them at 4 workgroups of 4 waves per CU, so they get the 128 registers of 4
-// waves per SIMD; one wave per game needs more registers (3 waves per SIMD).
+// Waves per SIMD the register allocation is held to (measured fastest): V11 and 16-32-32-1 at 4 (128 VGPRs, no
+// spills; V11 at 5 waves per SIMD, 96 VGPRs, spilled 92 B per lane once the repeated lists and the root scores
+// of the previous move were added, and was 15% slower; 16-32-32-1 with the candidate board in registers was 5%
+// faster at 4 than at 5 with spills). The trace and pruned kernels' LDS caps them at 4 workgroups of 4 waves per
+// CU, so they get the 128 registers of 4 waves per SIMD; one wave per game needs more registers (3 waves per
+// SIMD).
 template<int KIND, int MODE, int G>
-constexpr int WAVES_PER_SIMD =
-    G == 1 ? 3
-           : (KIND == MODEL_F32W ? 3
-                                 : (MODE != MODE_PLAIN ? 4 : (KIND == MODEL_V11 ? 5 : 4)));
+constexpr int WAVES_PER_SIMD = G == 1 ? 3 : (KIND == MODEL_F32W ? 3 : 4);
 // (the wide network at 3: four column tiles per weight load need ~168 VGPRs, and beat 4 waves/SIMD with two; its
 // 4-wave plain kernel, with the model's weights staged in LDS, fits 3 workgroups per CU in LDS)
 
