@@ -1307,9 +1307,11 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     ++cnt.moves;
     if(bestFallback == 1) ++cnt.fallbackMidBest;
     if(bestFallback == 2) ++cnt.fallbackLeafBest;
-    // the chosen root's level-1 lists, for the next move's RootReuse (rank = kept root index at level 1)
-    ws.reuse.keys[0] = rootRank < 32 ? ws.reuse.level1[2 * rootRank] : 0u;
-    ws.reuse.keys[1] = rootRank < 32 ? ws.reuse.level1[2 * rootRank + 1] : 0u;
+    // the chosen root's level-1 lists, for the next move's RootReuse (rank = kept root index at level 1); a pruned
+    // level 1 scores only some of them, and the device search then takes no roots from it
+    bool const level1Pruned = pr.q != nullptr && ((pr.plyMask >> 1) & 1u) != 0;
+    ws.reuse.keys[0] = rootRank < 32 && !level1Pruned ? ws.reuse.level1[2 * rootRank] : 0u;
+    ws.reuse.keys[1] = rootRank < 32 && !level1Pruned ? ws.reuse.level1[2 * rootRank + 1] : 0u;
     return {true, rn.rootHold, pl0.o, pl0.x, ordered_bits_inv(static_cast<uint32_t>(bestKey >> 32))};
 }
 

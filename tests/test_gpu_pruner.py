@@ -178,9 +178,12 @@ def test_pruned_beam_matches_oracle(kind, k, mask):
     assert fit.tobytes() == want.tobytes()
     assert (pl == want_pl).all()
     # repeated lists (same parent board, clear histogram, piece and held-is-I as an earlier list of the level) are
-    # scored once on the device, pruner included; the oracle counts them
-    assert int(cnt[:, 0].sum()) == want_cnt.applies - want_cnt.repeat_applies
-    assert int(cnt[:, 1].sum()) == want_cnt.legal - want_cnt.repeat_legal
+    # scored once on the device, pruner included, and the roots the previous move scored are taken from it (V11 and
+    # 16-32-32-1, when level 1 is not pruned); the oracle counts both
+    reuse_a, reuse_l = want_cnt.reuse_applies, want_cnt.reuse_legal
+    assert (reuse_l > 0) == ((mask & 0b10) == 0)
+    assert int(cnt[:, 0].sum()) == want_cnt.applies - want_cnt.repeat_applies - reuse_a
+    assert int(cnt[:, 1].sum()) == want_cnt.legal - want_cnt.repeat_legal - reuse_l
 
 
 def test_training_learns_the_labels(trace_set):

@@ -1701,8 +1701,9 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
     // repeated lists of the middle levels and the leaves are not scored (their results are their origins'; a
     // repeat's pruner logits are its origin's too); trace launches record every candidate of every list
     constexpr bool DEDUP = MODE != MODE_TRACE;
-    // roots from the previous move's level-1 scores (RootCacheLds; the wide kernel's LDS has no room for them)
-    constexpr bool ROOT_CACHE = MODE == MODE_PLAIN && KIND != MODEL_F32W;
+    // roots from the previous move's level-1 scores (RootCacheLds; the wide kernels' LDS has no room for them; a
+    // pruned search whose level 1 is pruned keeps only some of its scores and takes none)
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && KIND != MODEL_F32W;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1890,7 +1891,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
             if(lane == 0) {
                 RC->nextKey[0] = keys[0];
                 RC->nextKey[1] = keys[1];
-                RC->valid = cleared == 0 ? 1u : 0u;
+                RC->valid = cleared == 0 && (MODE != MODE_PRUNED || ((a.pruneMask >> 1) & 1u) == 0) ? 1u : 0u;
             }
         }
     }
@@ -1931,7 +1932,7 @@ void beam_eval_kernel(EvalArgs a) {
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
-    constexpr bool ROOT_CACHE = MODE == MODE_PLAIN && KIND != MODEL_F32W;
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && KIND != MODEL_F32W;
     __shared__ std::conditional_t<ROOT_CACHE, RootCacheLds, uint32_t> rcStore;
     RootCacheLds* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheLds*>(&rcStore) : nullptr;
     uint32_t const wv = uniform(threadIdx.x >> 6);
