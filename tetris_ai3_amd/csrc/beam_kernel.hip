@@ -195,7 +195,7 @@ struct EngineState {
 };
 
 constexpr uint32_t ENGINE_WORDS = sizeof(EngineState) / 4;
-// root scores carried to the next move (RootCacheLds): valid, two list keys, two lists of 34 scores
+// root scores carried to the next move (RootCacheMeta): valid, two list keys, two lists of 34 scores
 constexpr uint32_t PARK_RC = ENGINE_WORDS + 6, RC_WORDS = 3 + 2 * 34;
 static_assert(sizeof(EngineState) % 4 == 0 && PARK_RC + RC_WORDS <= PARK_WORDS && ENGINE_WORDS <= 64,
               "a parked game fits its record and one wave copies it");
@@ -234,21 +234,23 @@ struct alignas(16) PruneLds {
 };
 
 
-// Root lists scored by the previous move (plain V11 / 16-32-32-1 kernels). When the committed placement cleared no
-// line, the next move's root lists (the committed board, an empty clear histogram, a piece, held-is-I) are level-1
-// lists of the chosen root node whenever piece and held-is-I agree, which the previous search scored: the same
-// candidates with the same values. Level 1 keeps every score here; the commit copies the chosen root's two lists
-// (a repeat's from its origin) to `next`; a root stage whose lists all match takes its keys from there. A parked
-// game carries `valid`, `nextKey` and `next` in its record, so the choice depends only on the game.
-struct alignas(16) RootCacheLds {
-    uint32_t level1[32 * 34]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
-    uint32_t info1[32];       // segInfo of the level-1 lists
-    uint32_t valid;           // the last committed placement cleared no line (the lists below apply)
-    uint32_t nextKey[2];      // piece | heldIsI << 3 | 16 of the chosen root's level-1 lists (0: none)
-    uint32_t next[2 * 34];    // their scores
+// Root lists scored by the previous move. When the committed placement cleared no line, the next move's root lists
+// (the committed board, an empty clear histogram, a piece, held-is-I) are level-1 lists of the chosen root node
+// whenever piece and held-is-I agree, which the previous search scored: the same candidates with the same values.
+// Level 1 keeps every score (`level1`: LDS, or a global block per workgroup for the wide network); the commit
+// records where the chosen root's two lists are (a repeat's at its origin); the next root stage, which runs before
+// the next level 1 rewrites them, takes its keys from there when every root list matches. A parked game carries
+// `valid`, `nextKey` and the two lists' scores in its record, so the choice depends only on the game.
+struct RootCacheMeta {
+    uint32_t valid;      // the last committed placement cleared no line (nextKey / src apply)
+    uint32_t nextKey[2]; // piece | heldIsI << 3 | 16 of the chosen root's level-1 lists (0: none)
+    uint32_t src[2];     // their list indices in level1 (scores of list l at level1[34 l ..])
+    uint32_t info1[32];  // segInfo of the level-1 lists
 };
-static_assert(offsetof(RootCacheLds, next) == offsetof(RootCacheLds, valid) + 12 && RC_WORDS == 3 + 2 * 34,
-              "valid, nextKey and next are one block of RC_WORDS words (a parked game's copy)");
+struct alignas(16) RootScoresLds {
+    uint32_t s[ROOT_SCORE_WORDS]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
+};
+static_assert(RC_WORDS == 3 + 2 * 34, "a parked game's root scores: valid, nextKey, two lists");
 
 __device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:33-45
     uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
@@ -1663,17 +1665,19 @@ __device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
 // ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
 // pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
-// The root stage from the previous move's scores (RootCacheLds): when every root list matches one of the chosen
+// The root stage from the previous move's scores (RootCacheMeta): when every root list matches one of the chosen
 // root's level-1 lists, each wave builds the exact top-K of the root keys from them (no scoring) and returns true.
 template<int K>
-__device__ inline bool root_reuse(GameLds const& L, RootCacheLds const& RC, uint32_t total, uint64_t& top) {
+__device__ inline bool root_reuse(GameLds const& L, RootCacheMeta const& RC, uint32_t const* level1, uint32_t total,
+                                  uint64_t& top) {
     if(uniform(RC.valid) == 0) return false;
     uint32_t const lane = lane_id();
     uint32_t const info = L.segInfo[lane & 1u]; // lanes 0, 1: the root lists
     uint32_t const key = ((info >> 8) & 15u) | 16u;
     uint32_t const c = RC.nextKey[0] == key ? 0u : (RC.nextKey[1] == key ? 1u : 2u);
     if(__ballot(lane < 2 && (info & SEG_EXISTS) != 0 && c == 2u) != 0) return false;
-    uint32_t const c0 = __builtin_amdgcn_readlane(c, 0), c1 = __builtin_amdgcn_readlane(c, 1);
+    uint32_t const src0 = uniform(RC.src[__builtin_amdgcn_readlane(c, 0) & 1u]);
+    uint32_t const src1 = uniform(RC.src[__builtin_amdgcn_readlane(c, 1) & 1u]);
     uint64_t acc = 0;
     for(uint32_t base = 0; base < total; base += 64) {
         uint32_t const f = base + lane;
@@ -1681,7 +1685,7 @@ __device__ inline bool root_reuse(GameLds const& L, RootCacheLds const& RC, uint
         if(f < total) {
             uint32_t const si = L.candSeg[f];
             uint32_t const s = si >> 6, i = si & 63u;
-            uint32_t const score = RC.next[(s == 0 ? c0 : c1) * 34 + i];
+            uint32_t const score = level1[(s == 0 ? src0 : src1) * 34 + i];
             k = score != 0 ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - (s * 34 + i))) : 0ull;
         }
         acc = topk_merge<K>(acc, k);
@@ -1691,9 +1695,9 @@ __device__ inline bool root_reuse(GameLds const& L, RootCacheLds const& RC, uint
 }
 
 template<int KIND, int G, int MODE>
-__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheLds* RC, wptr w, WideArgs const& wide,
-                          uint32_t wv, uint32_t* col, bool traceMove, uint64_t& nApply, uint32_t& nLegal,
-                          uint64_t& nSlots) {
+__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheMeta* RC, uint32_t* level1, wptr w,
+                          WideArgs const& wide, uint32_t wv, uint32_t* col, bool traceMove, uint64_t& nApply,
+                          uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
@@ -1701,9 +1705,9 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
     // repeated lists of the middle levels and the leaves are not scored (their results are their origins'; a
     // repeat's pruner logits are its origin's too); trace launches record every candidate of every list
     constexpr bool DEDUP = MODE != MODE_TRACE;
-    // roots from the previous move's level-1 scores (RootCacheLds; the wide kernels' LDS has no room for them; a
-    // pruned search whose level 1 is pruned keeps only some of its scores and takes none)
-    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && KIND != MODEL_F32W;
+    // roots from the previous move's level-1 scores (RootCacheMeta; level1 null: none kept; a pruned search whose
+    // level 1 is pruned keeps only some of its scores and takes none)
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1724,7 +1728,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
     uint64_t mask = 0;
     uint64_t top = 0;
     bool reused = false;
-    if constexpr(ROOT_CACHE) reused = root_reuse<K0>(L, *RC, total, top);
+    if constexpr(ROOT_CACHE) reused = level1 != nullptr && root_reuse<K0>(L, *RC, level1, total, top);
     if(!reused) {
         nApply += total;
         top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
@@ -1752,7 +1756,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
         if constexpr(ROOT_CACHE) {
             if(level == 1) {
                 if(wv == 0 && lane < 32) RC->info1[lane] = L.segInfo[lane];
-                keep = RC->level1;
+                keep = level1;
             }
         }
         uint16_t const* seg = L.candSeg;
@@ -1877,22 +1881,17 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheL
         if(cleared >= 0) stats_advance(eng, cleared);
         else eng.gameOver = 1;
     }
-    if constexpr(ROOT_CACHE) { // the chosen root's level-1 lists become the next move's root lists (RootCacheLds)
-        if(wv == 0) {
+    if constexpr(ROOT_CACHE) { // the chosen root's level-1 lists become the next move's root lists (RootCacheMeta)
+        if(wv == 0 && lane == 0) {
             uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
-            uint32_t keys[2];
 #pragma unroll
             for(uint32_t c = 0; c < 2; ++c) {
                 uint32_t const li = RC->info1[2 * rank + c];
-                keys[c] = (li & SEG_EXISTS) != 0 ? (((li >> 8) & 15u) | 16u) : 0u;
-                uint32_t const src = (li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c;
-                if(lane < 34) RC->next[c * 34 + lane] = RC->level1[src * 34 + lane];
+                RC->nextKey[c] = (li & SEG_EXISTS) != 0 ? (((li >> 8) & 15u) | 16u) : 0u;
+                RC->src[c] = (li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c;
             }
-            if(lane == 0) {
-                RC->nextKey[0] = keys[0];
-                RC->nextKey[1] = keys[1];
-                RC->valid = cleared == 0 && (MODE != MODE_PRUNED || ((a.pruneMask >> 1) & 1u) == 0) ? 1u : 0u;
-            }
+            RC->valid = level1 != nullptr && cleared == 0 && (MODE != MODE_PRUNED || ((a.pruneMask >> 1) & 1u) == 0)
+                            ? 1u : 0u;
         }
     }
     group_sync<G>();
@@ -1932,9 +1931,17 @@ void beam_eval_kernel(EvalArgs a) {
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
-    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && KIND != MODEL_F32W;
-    __shared__ std::conditional_t<ROOT_CACHE, RootCacheLds, uint32_t> rcStore;
-    RootCacheLds* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheLds*>(&rcStore) : nullptr;
+    // the previous move's root scores (RootCacheMeta): level-1 scores in LDS, or for the wide network's plain kernel
+    // in a global block per workgroup
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
+    constexpr bool RC_LDS = ROOT_CACHE && KIND != MODEL_F32W;
+    __shared__ std::conditional_t<ROOT_CACHE, RootCacheMeta, uint32_t> rcMeta;
+    __shared__ std::conditional_t<RC_LDS, RootScoresLds, uint32_t> rcScores;
+    RootCacheMeta* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheMeta*>(&rcMeta) : nullptr;
+    uint32_t* const level1 = !ROOT_CACHE ? nullptr
+                             : RC_LDS    ? reinterpret_cast<RootScoresLds*>(&rcScores)->s
+                             : (a.rootScores != nullptr ? a.rootScores + static_cast<uint64_t>(blockIdx.x) * ROOT_SCORE_WORDS
+                                                        : nullptr);
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = SCRATCH ? scratch + wv * (WIDTH * SCR_STRIDE) + lane : nullptr;
@@ -1979,10 +1986,13 @@ void beam_eval_kernel(EvalArgs a) {
             nLegal = wv == 0 ? uniform(park[ENGINE_WORDS + 2]) : 0u;
             mv = uniform(park[ENGINE_WORDS + 3]);
             nSlots = uniform(static_cast<uint64_t>(park[ENGINE_WORDS + 4]) | static_cast<uint64_t>(park[ENGINE_WORDS + 5]) << 32);
-            if constexpr(ROOT_CACHE) {
-                if(wv == 0) {
-                    uint32_t* const rc = &RC->valid; // valid, nextKey[2], next[68]: contiguous
-                    for(uint32_t k = lane; k < RC_WORDS; k += 64) rc[k] = park[PARK_RC + k];
+            if constexpr(ROOT_CACHE) { // the root scores carried in the record: lists 0 and 1 of level1
+                if(level1 != nullptr && wv == 0) {
+                    if(lane < 3) (&RC->valid)[lane] = park[PARK_RC + lane]; // valid, nextKey[2]
+                    if(lane < 2) RC->src[lane] = lane;
+                    for(uint32_t k = lane; k < 2 * 34; k += 64) level1[k] = park[PARK_RC + 3 + k];
+                } else if(threadIdx.x == 0) {
+                    RC->valid = 0;
                 }
             }
         } else if(threadIdx.x == 0) {
@@ -2001,7 +2011,7 @@ void beam_eval_kernel(EvalArgs a) {
                                : (a.maxMoves - mv > a.phaseMoves ? mv + a.phaseMoves : a.maxMoves);
         bool ended = false;
         for(; mv < limit; ++mv) {
-            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, RC, w, wide, wv, col,
+            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, RC, level1, w, wide, wv, col,
                                                                          MODE == MODE_TRACE && a.traceEvery != 0
                                                                              && mv % a.traceEvery == tracePhase,
                                                                          nApply, nLegal, nSlots)) {
@@ -2016,9 +2026,10 @@ void beam_eval_kernel(EvalArgs a) {
             uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
             if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
             if constexpr(ROOT_CACHE) {
-                if(wv == 0) {
-                    uint32_t const* const rc = &RC->valid;
-                    for(uint32_t k = lane; k < RC_WORDS; k += 64) out[PARK_RC + k] = rc[k];
+                if(level1 != nullptr && wv == 0) {
+                    if(lane < 3) out[PARK_RC + lane] = (&RC->valid)[lane];
+                    for(uint32_t k = lane; k < 2 * 34; k += 64)
+                        out[PARK_RC + 3 + k] = level1[RC->src[k / 34] * 34 + k % 34];
                 }
             }
             if(threadIdx.x == 0) {

@@ -124,6 +124,8 @@ struct ta3h_evaluator {
     size_t parkedCap = 0;
     uint32_t* dResume = nullptr; // parked list [items], its counter, resume order [items], per-model counts
     size_t resumeCap = 0;
+    uint32_t* dRootScores = nullptr; // level-1 scores per workgroup (the wide network's plain kernel)
+    size_t rootScoresCap = 0;
 
     // pruned beam
     ta3h::PrunerDev* dPruner = nullptr;
@@ -271,7 +273,8 @@ struct ta3h_evaluator {
         if(dTimeline) static_cast<void>(hipFree(dTimeline));
         if(dMoves) static_cast<void>(hipFree(dMoves));
         if(dQueue) static_cast<void>(hipFree(dQueue));
-        for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
+        for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dRootScores),
+                       static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
                        static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
                        static_cast<void*>(dOrder), static_cast<void*>(dAct), static_cast<void*>(dDelta),
                        static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut),
@@ -407,10 +410,40 @@ size_t resume_words(ta3h_evaluator const* ev) {
            + ta3h::ring_slots(ev->items, ev->maxMoves, phase_budget(ev));
 }
 
+// Kernel mode (0 full beam, 1 traces or move log, 2 pruned), waves per game and workgroups of the staged batch's
+// launch: one workgroup (of wpb cooperating waves) per game in flight.
+int launch_shape(ta3h_evaluator* ev, int* mode, int* wpb, uint32_t* blocks) {
+    *mode = ev->prunerOn ? 2 : (ev->traceEvery != 0 || ev->movesOn ? 1 : 0);
+    *wpb = *mode != 0 ? 4 : ev->wavesPerBlock; // the pruned and trace kernels run four waves per game
+    uint32_t const needed = ev->items;
+    uint32_t b = needed;
+    if(ev->persistent) {
+        b = ev->gridBlocks;
+        if(b == 0) {
+            // fill every CU up to the kernel's occupancy limit; workgroups beyond it would only queue
+            int perCU = 0;
+            hipError_t const e0 = ta3h::max_active_blocks(ev->kind, *wpb, *mode, &perCU);
+            if(e0 != hipSuccess) return ev->fail("occupancy", e0);
+            b = ev->numCUs * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
+        }
+        if(b > needed) b = needed;
+    }
+    *blocks = b;
+    return TA3H_OK;
+}
+
 // Device buffers the launch of the staged batch needs beyond the staged ones (the parked games of the two-phase
-// schedule). Called before enqueue_eval and never while the stream is being captured.
+// schedule, the wide kernel's level-1 scores). Called before enqueue_eval and never while the stream is being
+// captured.
 int prepare_launch(ta3h_evaluator* ev) {
     int r;
+    int mode = 0, wpb = 0;
+    uint32_t blocks = 0;
+    if((r = launch_shape(ev, &mode, &wpb, &blocks))) return r;
+    if(ta3h::root_scores_global(ev->kind, mode)
+       && (r = ev->grow(ev->dRootScores, ev->rootScoresCap, static_cast<size_t>(blocks) * ta3h::ROOT_SCORE_WORDS,
+                        "malloc root scores")))
+        return r;
     if(ev->movesOn
        && (r = ev->grow(ev->dMoves, ev->movesCap, static_cast<size_t>(ev->items) * ev->maxMoves, "malloc move log")))
         return r;
@@ -464,39 +497,27 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
         a.ring = a.modelParked + ev->items / ev->games;
         a.ringCap = static_cast<uint32_t>(ta3h::ring_slots(ev->items, ev->maxMoves, phase));
     }
-    int mode = 0, wpb = ev->wavesPerBlock;
+    int mode = 0, wpb = 0;
+    uint32_t blocks = 0;
+    int const r = launch_shape(ev, &mode, &wpb, &blocks);
+    if(r) return r;
     if(ev->prunerOn) {
         a.pruner = ev->dPruner;
         a.pruneK = ev->pruneK;
         a.pruneMask = ev->pruneMask;
-        mode = 2;
-        wpb = 4; // the pruned and trace kernels run four waves per game
     } else if(ev->traceEvery != 0) {
         a.trace = ev->dTrace;
         a.traceCount = ev->dTraceCount;
         a.traceCap = static_cast<uint32_t>(ev->traceCap);
         a.traceEvery = ev->traceEvery;
-        mode = 1;
-        wpb = 4;
-    } else if(ev->movesOn) { // the trace kernel (recording no traces) writes the move log of a full beam
-        mode = 1;
-        wpb = 4;
-    }
-    // one workgroup (of wpb cooperating waves) per game in flight
-    uint32_t const needed = ev->items;
-    uint32_t blocks;
-    if(ev->persistent) {
-        blocks = ev->gridBlocks;
-        if(blocks == 0) {
-            // fill every CU up to the kernel's occupancy limit; workgroups beyond it would only queue
-            int perCU = 0;
-            hipError_t const e0 = ta3h::max_active_blocks(ev->kind, wpb, mode, &perCU);
-            if(e0 != hipSuccess) return ev->fail("occupancy", e0);
-            blocks = ev->numCUs * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
+    } // (a move log alone: the trace kernel, recording no traces, writes the move log of a full beam)
+    if(ta3h::root_scores_global(ev->kind, mode)) {
+        if(ev->rootScoresCap < static_cast<size_t>(blocks) * ta3h::ROOT_SCORE_WORDS) {
+            ev->err = "launch: root score buffer not prepared";
+            return TA3H_E_STATE;
         }
-        if(blocks > needed) blocks = needed;
-    } else
-        blocks = needed;
+        a.rootScores = ev->dRootScores;
+    }
     ev->lastGrid = blocks;
     hipError_t e;
     if(events && (e = hipEventRecord(ev->evStart, ev->stream)) != hipSuccess) return ev->fail("event record", e);
