@@ -237,18 +237,24 @@ struct alignas(16) PruneLds {
 // Root lists scored by the previous move. When the committed placement cleared no line, the next move's root lists
 // (the committed board, an empty clear histogram, a piece, held-is-I) are level-1 lists of the chosen root node
 // whenever piece and held-is-I agree, which the previous search scored: the same candidates with the same values.
-// Level 1 keeps every score (`level1`: LDS, or a global block per workgroup for the wide network); the commit
-// records where the chosen root's two lists are (a repeat's at its origin); the next root stage, which runs before
-// the next level 1 rewrites them, takes its keys from there when every root list matches. A parked game carries
-// `valid`, `nextKey` and the two lists' scores in its record, so the choice depends only on the game.
+// Level 1 keeps the scores of the lists of the best RC_ROOTS kept roots in LDS (`level1`; all 16 but for the wide
+// network, whose LDS leaves room for the best root's: the chosen root is the best in ~46% of moves; 6 roots, ~86%,
+// made its workgroups too large for three per CU and the generation 30% slower); the commit records where
+// the chosen root's two lists are (a repeat's at its origin, an earlier list, so of a better root); the next root
+// stage, which runs before the next level 1 rewrites them, takes its keys from there when every root list
+// matches. A parked game carries `valid`, `nextKey` and the two lists' scores in its record, so the choice
+// depends only on the game.
 struct RootCacheMeta {
     uint32_t valid;      // the last committed placement cleared no line (nextKey / src apply)
     uint32_t nextKey[2]; // piece | heldIsI << 3 | 16 of the chosen root's level-1 lists (0: none)
     uint32_t src[2];     // their list indices in level1 (scores of list l at level1[34 l ..])
     uint32_t info1[32];  // segInfo of the level-1 lists
 };
+template<int KIND>
+constexpr uint32_t RC_ROOTS = KIND == MODEL_F32W ? 1 : 16;
+template<int KIND>
 struct alignas(16) RootScoresLds {
-    uint32_t s[ROOT_SCORE_WORDS]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
+    uint32_t s[RC_ROOTS<KIND> * 2 * 34]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
 };
 static_assert(RC_WORDS == 3 + 2 * 34, "a parked game's root scores: valid, nextKey, two lists");
 
@@ -1153,7 +1159,8 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
         uint32_t const slot = s * 34 + i;
-        if(keep != nullptr && f < total) keep[slot] = legal ? score : 0u; // level-1 scores for the next move's roots
+        if(keep != nullptr && f < total && slot < RC_ROOTS<KIND> * 2 * 34) // level-1 scores for the next move's roots
+            keep[slot] = legal ? score : 0u;
         if constexpr(STAGE == STAGE_LEAF) {
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
             acc = key > acc ? key : acc;
@@ -1705,8 +1712,8 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     // repeated lists of the middle levels and the leaves are not scored (their results are their origins'; a
     // repeat's pruner logits are its origin's too); trace launches record every candidate of every list
     constexpr bool DEDUP = MODE != MODE_TRACE;
-    // roots from the previous move's level-1 scores (RootCacheMeta; level1 null: none kept; a pruned search whose
-    // level 1 is pruned keeps only some of its scores and takes none)
+    // roots from the previous move's level-1 scores (RootCacheMeta; a pruned search whose level 1 is pruned keeps
+    // only some of its scores and takes none)
     constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
@@ -1728,7 +1735,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     uint64_t mask = 0;
     uint64_t top = 0;
     bool reused = false;
-    if constexpr(ROOT_CACHE) reused = level1 != nullptr && root_reuse<K0>(L, *RC, level1, total, top);
+    if constexpr(ROOT_CACHE) reused = root_reuse<K0>(L, *RC, level1, total, top);
     if(!reused) {
         nApply += total;
         top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
@@ -1884,14 +1891,16 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     if constexpr(ROOT_CACHE) { // the chosen root's level-1 lists become the next move's root lists (RootCacheMeta)
         if(wv == 0 && lane == 0) {
             uint32_t const rank = order >= LEAF_ORDERS ? order - LEAF_ORDERS : (L.info[cur * 32 + order / 68] >> 8);
+            bool const valid = rank < RC_ROOTS<KIND> && cleared == 0
+                               && (MODE != MODE_PRUNED || ((a.pruneMask >> 1) & 1u) == 0);
 #pragma unroll
             for(uint32_t c = 0; c < 2; ++c) {
                 uint32_t const li = RC->info1[2 * rank + c];
                 RC->nextKey[c] = (li & SEG_EXISTS) != 0 ? (((li >> 8) & 15u) | 16u) : 0u;
-                RC->src[c] = (li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c;
+                // a repeat's origin is an earlier list, of a better root, so also kept when valid
+                RC->src[c] = !valid ? c : ((li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c);
             }
-            RC->valid = level1 != nullptr && cleared == 0 && (MODE != MODE_PRUNED || ((a.pruneMask >> 1) & 1u) == 0)
-                            ? 1u : 0u;
+            RC->valid = valid ? 1u : 0u;
         }
     }
     group_sync<G>();
@@ -1931,17 +1940,12 @@ void beam_eval_kernel(EvalArgs a) {
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
-    // the previous move's root scores (RootCacheMeta): level-1 scores in LDS, or for the wide network's plain kernel
-    // in a global block per workgroup
+    // the previous move's root scores (RootCacheMeta, level-1 scores in LDS)
     constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
-    constexpr bool RC_LDS = ROOT_CACHE && KIND != MODEL_F32W;
     __shared__ std::conditional_t<ROOT_CACHE, RootCacheMeta, uint32_t> rcMeta;
-    __shared__ std::conditional_t<RC_LDS, RootScoresLds, uint32_t> rcScores;
+    __shared__ std::conditional_t<ROOT_CACHE, RootScoresLds<KIND>, uint32_t> rcScores;
     RootCacheMeta* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheMeta*>(&rcMeta) : nullptr;
-    uint32_t* const level1 = !ROOT_CACHE ? nullptr
-                             : RC_LDS    ? reinterpret_cast<RootScoresLds*>(&rcScores)->s
-                             : (a.rootScores != nullptr ? a.rootScores + static_cast<uint64_t>(blockIdx.x) * ROOT_SCORE_WORDS
-                                                        : nullptr);
+    uint32_t* const level1 = ROOT_CACHE ? reinterpret_cast<RootScoresLds<KIND>*>(&rcScores)->s : nullptr;
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = SCRATCH ? scratch + wv * (WIDTH * SCR_STRIDE) + lane : nullptr;
@@ -1987,12 +1991,10 @@ void beam_eval_kernel(EvalArgs a) {
             mv = uniform(park[ENGINE_WORDS + 3]);
             nSlots = uniform(static_cast<uint64_t>(park[ENGINE_WORDS + 4]) | static_cast<uint64_t>(park[ENGINE_WORDS + 5]) << 32);
             if constexpr(ROOT_CACHE) { // the root scores carried in the record: lists 0 and 1 of level1
-                if(level1 != nullptr && wv == 0) {
+                if(wv == 0) {
                     if(lane < 3) (&RC->valid)[lane] = park[PARK_RC + lane]; // valid, nextKey[2]
                     if(lane < 2) RC->src[lane] = lane;
                     for(uint32_t k = lane; k < 2 * 34; k += 64) level1[k] = park[PARK_RC + 3 + k];
-                } else if(threadIdx.x == 0) {
-                    RC->valid = 0;
                 }
             }
         } else if(threadIdx.x == 0) {
@@ -2026,7 +2028,7 @@ void beam_eval_kernel(EvalArgs a) {
             uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
             if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
             if constexpr(ROOT_CACHE) {
-                if(level1 != nullptr && wv == 0) {
+                if(wv == 0) {
                     if(lane < 3) out[PARK_RC + lane] = (&RC->valid)[lane];
                     for(uint32_t k = lane; k < 2 * 34; k += 64)
                         out[PARK_RC + 3 + k] = level1[RC->src[k / 34] * 34 + k % 34];

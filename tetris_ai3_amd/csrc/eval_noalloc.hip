@@ -6,7 +6,6 @@
 //   weights             the models' weights repacked into the network's device layout (a kernel, not the host)
 //   parked games        PARK_WORDS words per item (two-phase schedule)
 //   resume lists        parked list [items], resume order [items], per-model parked counts [models]
-//   root scores         ROOT_SCORE_WORDS words per workgroup (the wide network: level-1 scores for the next move)
 // Nothing here calls hipMalloc, hipFree or any other allocating routine. Calls that overlap on one GPU share the
 // array and race; every counter the search reads is cleared by a kernel at the start of the call.
 #include <hip/hip_runtime.h>
@@ -148,15 +147,13 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     if(max_active_blocks(kind, wpg, 0, &perCU) != hipSuccess) return TA3H_E_HIP;
     uint64_t blocks = static_cast<uint64_t>(cus) * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
     if(blocks > items) blocks = items;
-    size_t const rootBytes = root_scores_global(kind, 0) ? round256(blocks * ROOT_SCORE_WORDS * 4) : 0;
-    if(256 + wBytes + parkBytes + resumeBytes + rootBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH;
+    if(256 + wBytes + parkBytes + resumeBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH;
     auto* const s = static_cast<hipStream_t>(stream);
     uint8_t* const p = static_cast<uint8_t*>(base);
     uint32_t* const counters = reinterpret_cast<uint32_t*>(p);
     float* const weights = reinterpret_cast<float*>(p + 256);
     uint32_t* const parked = reinterpret_cast<uint32_t*>(p + 256 + wBytes);
     uint32_t* const resume = reinterpret_cast<uint32_t*>(p + 256 + wBytes + parkBytes);
-    uint32_t* const rootScores = reinterpret_cast<uint32_t*>(p + 256 + wBytes + parkBytes + resumeBytes);
 
     uint64_t const words = static_cast<uint64_t>(num_models) * stride_of(kind);
     hipLaunchKernelGGL(repack_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, s, kind,
@@ -186,7 +183,6 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
         a.ring = resume + items + num_models;
         a.ringCap = static_cast<uint32_t>(ringSlots);
     }
-    if(rootBytes != 0) a.rootScores = rootScores;
     if(launch_beam_eval(kind, a, wpg, static_cast<uint32_t>(blocks), s) != hipSuccess) return TA3H_E_HIP;
     return TA3H_OK;
 }

@@ -79,15 +79,7 @@ struct EvalArgs {
     uint32_t* traceCount; // records produced (may exceed traceCap; the excess is dropped)
     uint32_t traceCap;
     uint32_t traceEvery;
-    // level-1 scores kept for the next move's roots, ROOT_SCORE_WORDS per workgroup of the launch (the wide
-    // network's plain kernel, whose LDS has no room for them; the other kernels keep them in LDS). Null: the wide
-    // kernel scores every root stage.
-    uint32_t* rootScores;
 };
-
-inline constexpr uint32_t ROOT_SCORE_WORDS = 32 * 34; // level-1 slots: 16 kept roots x 2 lists x 34 placements
-// the launch keeps level-1 scores in global memory (EvalArgs::rootScores): the wide network's plain kernel
-inline constexpr bool root_scores_global(int kind, int mode) { return kind == 3 && mode == 0; }
 
 hipError_t launch_beam_eval(int kind, EvalArgs const& a, int wavesPerBlock, uint32_t blocks, hipStream_t stream);
 // uploads the constant tables computed on the host before the first launch on the current device (once per

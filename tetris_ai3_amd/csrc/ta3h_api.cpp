@@ -124,8 +124,6 @@ struct ta3h_evaluator {
     size_t parkedCap = 0;
     uint32_t* dResume = nullptr; // parked list [items], its counter, resume order [items], per-model counts
     size_t resumeCap = 0;
-    uint32_t* dRootScores = nullptr; // level-1 scores per workgroup (the wide network's plain kernel)
-    size_t rootScoresCap = 0;
 
     // pruned beam
     ta3h::PrunerDev* dPruner = nullptr;
@@ -273,8 +271,8 @@ struct ta3h_evaluator {
         if(dTimeline) static_cast<void>(hipFree(dTimeline));
         if(dMoves) static_cast<void>(hipFree(dMoves));
         if(dQueue) static_cast<void>(hipFree(dQueue));
-        for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dRootScores),
-                       static_cast<void*>(dPruner), static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
+        for(void* p : {static_cast<void*>(dParked), static_cast<void*>(dResume), static_cast<void*>(dPruner),
+                       static_cast<void*>(dTrace), static_cast<void*>(dTraceCount),
                        static_cast<void*>(dPw), static_cast<void*>(dAdam), static_cast<void*>(dSamples),
                        static_cast<void*>(dOrder), static_cast<void*>(dAct), static_cast<void*>(dDelta),
                        static_cast<void*>(dScore), static_cast<void*>(dLoss), static_cast<void*>(dOut),
@@ -433,17 +431,9 @@ int launch_shape(ta3h_evaluator* ev, int* mode, int* wpb, uint32_t* blocks) {
 }
 
 // Device buffers the launch of the staged batch needs beyond the staged ones (the parked games of the two-phase
-// schedule, the wide kernel's level-1 scores). Called before enqueue_eval and never while the stream is being
-// captured.
+// schedule). Called before enqueue_eval and never while the stream is being captured.
 int prepare_launch(ta3h_evaluator* ev) {
     int r;
-    int mode = 0, wpb = 0;
-    uint32_t blocks = 0;
-    if((r = launch_shape(ev, &mode, &wpb, &blocks))) return r;
-    if(ta3h::root_scores_global(ev->kind, mode)
-       && (r = ev->grow(ev->dRootScores, ev->rootScoresCap, static_cast<size_t>(blocks) * ta3h::ROOT_SCORE_WORDS,
-                        "malloc root scores")))
-        return r;
     if(ev->movesOn
        && (r = ev->grow(ev->dMoves, ev->movesCap, static_cast<size_t>(ev->items) * ev->maxMoves, "malloc move log")))
         return r;
@@ -511,13 +501,6 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
         a.traceCap = static_cast<uint32_t>(ev->traceCap);
         a.traceEvery = ev->traceEvery;
     } // (a move log alone: the trace kernel, recording no traces, writes the move log of a full beam)
-    if(ta3h::root_scores_global(ev->kind, mode)) {
-        if(ev->rootScoresCap < static_cast<size_t>(blocks) * ta3h::ROOT_SCORE_WORDS) {
-            ev->err = "launch: root score buffer not prepared";
-            return TA3H_E_STATE;
-        }
-        a.rootScores = ev->dRootScores;
-    }
     ev->lastGrid = blocks;
     hipError_t e;
     if(events && (e = hipEventRecord(ev->evStart, ev->stream)) != hipSuccess) return ev->fail("event record", e);
