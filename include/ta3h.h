@@ -101,10 +101,10 @@ int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uin
  * two-phase budget are the defaults. The search's scratch (work-queue counters, the weights repacked into the
  * network's device layout, parked games, the resume list and the second phase's work ring) lives in one static
  * device array of TA3H_NOALLOC_SCRATCH_BYTES per GPU: 256 B + models x the layout's bytes (V11 448, 16-32-32-1
- * 6,592, wide 42,560, each rounded up to 256 B per section) + 260 B per (model, game) + 4 B per model + 8 B + 4 B
+ * 6,592, wide 42,560, each rounded up to 256 B per section) + 516 B per (model, game) + 4 B per model + 8 B + 4 B
  * per (model, game) and turn of the second phase (1 + (max_moves - b) / b turns for the budget b = max_moves / 16
  * clamped to [128, 512]; 15 at 2,000 moves, 16 at 5,000); a larger batch returns TA3H_E_SCRATCH (e.g. 200 x 50 at
- * 16-32-32-1, 2,000 moves: 4.5 MB; 256 x 100 wide, 5,000 moves: 19.2 MB). Calls
+ * 16-32-32-1, 2,000 moves: 7.1 MB; 256 x 100 wide, 5,000 moves: 25.7 MB). Calls
  * that overlap on one GPU share that array and race: serialise them (one stream, or events). Returns TA3H_OK,
  * TA3H_E_ARG, TA3H_E_SCRATCH or TA3H_E_HIP (there is no evaluator to hold an error message). */
 int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_models, const uint64_t* d_seeds, uint32_t games,
@@ -121,8 +121,13 @@ int ta3h_sync(ta3h_evaluator* ev);
 int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements);
 
 /* Per-item work counters of the last launch (no reference counterpart): out[3*item] = candidate placements
- * fitted, out[3*item+1] = candidates that fit and were scored (the algorithmic work of the search),
- * out[3*item+2] = lanes the scoring chunks occupied (64 per chunk; legal / lanes = lane utilisation). */
+ * fitted, out[3*item+1] = candidates that fit and were scored (the work the search executed),
+ * out[3*item+2] = lanes the scoring chunks occupied (64 per chunk; legal / lanes = lane utilisation). The search
+ * scores a repeated list once (a list of a level whose parent board, clear histogram, piece and held-is-I equal
+ * an earlier list's: the same candidates with the same values) and, after a placement that cleared no line,
+ * takes the root lists from the previous move's level-1 scores when they match (V11 and 16-32-32-1: any kept
+ * root; the wide network: the best one; not in pruned searches that prune level 1, nor in trace and move-log
+ * launches, which score every candidate): the reference's search fits and scores ~21% more candidates. */
 int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 
 /* Optional per-item trace (no reference counterpart): when enabled before ta3h_stage, each launch records
