@@ -72,8 +72,21 @@ def test_config1_generation_bit_exact_sample_and_properties():
     cnt = ev.fetch_counters()
     ev.launch()
     fit2, pl2 = ev.fetch()
+    # the move-log (trace) kernel scores every list and every root stage, as the reference does; the plain
+    # kernel skips repeated lists and reuses roots: every game of the generation must come out the same (the
+    # agreement reports compare the move-log kernel with the oracle move by move over all 10,000 games)
+    say("case: config1 generation, plain kernel vs the kernel that scores every candidate")
+    ev.set_move_log(True)
+    ev.stage(w, M, seeds, cap)
+    ev.launch()
+    fit3, pl3 = ev.fetch()
+    cnt3 = ev.fetch_counters()
     ev.close()
     assert fit.tobytes() == fit2.tobytes() and np.array_equal(pl, pl2)  # a relaunch repeats bit for bit
+    assert fit.tobytes() == fit3.tobytes() and np.array_equal(pl, pl3)
+    skipped = 1.0 - cnt[:, 1].sum() / cnt3[:, 1].sum()
+    say(f"value evaluations skipped by the plain kernel: {skipped:.4f}")
+    assert 0.15 < skipped < 0.35
     util = check_batch_properties(fit, pl, cnt, cap)
     assert (pl == cap).sum() >= 100  # good models reach the cap (uneven lengths)
     say(f"placements {int(pl.sum())}, p50 {np.percentile(pl, 50)}, at cap {(pl == cap).sum()}, lane util {util:.4f}")
