@@ -1714,7 +1714,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     constexpr bool DEDUP = MODE != MODE_TRACE;
     // roots from the previous move's level-1 scores (RootCacheMeta; a pruned search whose level 1 is pruned keeps
     // only some of its scores and takes none)
-    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1941,7 +1941,7 @@ void beam_eval_kernel(EvalArgs a) {
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
     __shared__ std::conditional_t<WIDE_LDS, WideWeightsLds, uint32_t> wideWeights;
     // the previous move's root scores (RootCacheMeta, level-1 scores in LDS)
-    constexpr bool ROOT_CACHE = MODE != MODE_TRACE && (KIND != MODEL_F32W || MODE == MODE_PLAIN);
+    constexpr bool ROOT_CACHE = MODE != MODE_TRACE;
     __shared__ std::conditional_t<ROOT_CACHE, RootCacheMeta, uint32_t> rcMeta;
     __shared__ std::conditional_t<ROOT_CACHE, RootScoresLds<KIND>, uint32_t> rcScores;
     RootCacheMeta* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheMeta*>(&rcMeta) : nullptr;
