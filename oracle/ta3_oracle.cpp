@@ -1310,9 +1310,9 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     // the chosen root's level-1 lists, for the next move's RootReuse (rank = kept root index at level 1); a pruned
     // level 1 scores only some of them, and the device search then takes no roots from it
     bool const level1Pruned = pr.q != nullptr && ((pr.plyMask >> 1) & 1u) != 0;
-    // the device keeps the level-1 scores of the best 16 kept roots, of the best one for the wide network (its LDS
+    // the device keeps the level-1 scores of the best 16 kept roots, of the best 6 for the wide network (its LDS
     // budget)
-    uint32_t const keptRoots = model.kind == TA3O_MODEL_F32W ? 1u : 16u;
+    uint32_t const keptRoots = model.kind == TA3O_MODEL_F32W ? 6u : 16u;
     bool const keep = rootRank < keptRoots && !level1Pruned;
     ws.reuse.keys[0] = keep ? ws.reuse.level1[2 * rootRank] : 0u;
     ws.reuse.keys[1] = keep ? ws.reuse.level1[2 * rootRank + 1] : 0u;

@@ -209,7 +209,12 @@ struct alignas(16) GameLds {
     uint32_t info[COMMITTED + 1];             // tree node | root rank << 8
     uint32_t segOff[64];                      // exclusive prefix offsets of the stage's segments
     uint32_t segInfo[64];                     // parent board | piece << 8 | heldIsI << 11 | child << 12
-    uint16_t candSeg[32 * 2 * 34];            // flat candidate index -> segment << 6 | placement index
+    // a stage's candidates c = 64 k + l (chunk k, lane l) without a per-candidate table (the wide plain kernel): the
+    // scored lists in order (listOf[j] = segment), the list holding chunk k's first candidate (chunkList[k]) and
+    // the lanes of chunk k where another list starts (chunkStarts[k]); see cand_of
+    uint64_t chunkStarts[36];
+    uint8_t listOf[64];
+    uint8_t chunkList[36];
     uint32_t rootSlot[32];
     uint32_t slotPiece[8];                    // pieces per abstract window slot (beam.hpp:226-237)
     uint32_t treeIdx;
@@ -221,6 +226,13 @@ struct alignas(16) GameLds {
     uint32_t partLegal[MAX_WAVES];            // per-wave legal-candidate counts of the item
     EngineState eng;
     uint32_t moveItem; // the item being played (its move-log row; trace and pruned kernels)
+};
+
+// A stage's flat candidate index -> segment << 6 | placement index (every kernel but the wide network's plain one,
+// whose LDS has no room for it and which uses the chunk headers of GameLds instead; with the table the 16-32-32-1
+// generation runs 1.5% faster, V11's alike).
+struct alignas(16) CandLds {
+    uint16_t seg[32 * 2 * 34];
 };
 
 // Extra LDS of the trace / pruned kernels: the pruner's board projection and kept-candidate list (the pruner
@@ -238,10 +250,9 @@ struct alignas(16) PruneLds {
 // (the committed board, an empty clear histogram, a piece, held-is-I) are level-1 lists of the chosen root node
 // whenever piece and held-is-I agree, which the previous search scored: the same candidates with the same values.
 // Level 1 keeps the scores of the lists of the best RC_ROOTS kept roots in LDS (`level1`; all 16 but for the wide
-// network, whose LDS leaves room for the best root's: the chosen root is the best in ~46% of moves; 6 roots, ~86%,
-// made its workgroups too large for three per CU and the generation 30% slower); the commit records where
-// the chosen root's two lists are (a repeat's at its origin, an earlier list, so of a better root); the next root
-// stage, which runs before the next level 1 rewrites them, takes its keys from there when every root list
+// network, whose LDS leaves room for the best 6: the chosen root is among them in ~86% of moves); the commit records
+// where the chosen root's two lists are (a repeat's at its origin, an earlier list, so of a better root); the next
+// root stage, which runs before the next level 1 rewrites them, takes its keys from there when every root list
 // matches. A parked game carries `valid`, `nextKey` and the two lists' scores in its record, so the choice
 // depends only on the game.
 struct RootCacheMeta {
@@ -251,7 +262,7 @@ struct RootCacheMeta {
     uint32_t info1[32];  // segInfo of the level-1 lists
 };
 template<int KIND>
-constexpr uint32_t RC_ROOTS = KIND == MODEL_F32W ? 1 : 16;
+constexpr uint32_t RC_ROOTS = KIND == MODEL_F32W ? 6 : 16;
 template<int KIND>
 struct alignas(16) RootScoresLds {
     uint32_t s[RC_ROOTS<KIND> * 2 * 34]; // ordered-bits score of level-1 candidate slot s * 34 + i (0: illegal)
@@ -1007,11 +1018,26 @@ __device__ inline bool same_nodes(GameLds const& L, uint32_t q, uint32_t r) {
 // are matched by a 16-bit hash of board and clears and checked exactly (a collision falls back to a scan), then
 // lists by (earliest equal node, piece, held-is-I).
 
+// Segment << 6 | placement index of candidate f of the stage: from the candidate table when the kernel keeps one
+// (seg), else from the chunk headers: lane l of chunk k is in scored list chunkList[k] + (lists starting at lanes
+// 1..l of the chunk), at placement f - segOff of that list. The calling lane must be lane f % 64 (chunks are
+// processed one candidate per lane).
+__device__ inline uint32_t cand_of(GameLds const& L, uint16_t const* seg, uint32_t f) {
+    if(seg != nullptr) return seg[f];
+    uint32_t const k = f >> 6, l = f & 63u;
+    uint64_t const m = L.chunkStarts[k];
+    uint32_t const below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    uint32_t const s = L.listOf[L.chunkList[k] + below + static_cast<uint32_t>((m >> l) & 1ull)];
+    return (s << 6) | (f - L.segOff[s]);
+}
+
 // Builds the stage's segment table in LDS. Segment s = 2k + c: child c of kept node k of buffer `cur`
 // (roots: s = root group, parent = the committed board). With DEDUP (middle levels and leaves), repeated lists
 // get no candidates. Returns the candidate count.
 template<int STAGE, bool DEDUP = false>
-__device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur) {
+__device__ inline uint32_t build_segments(GameLds& L, uint16_t* cand, uint32_t ti, int layer, uint32_t keptCount,
+                                          uint32_t cur) {
     uint32_t const s = lane_id();
     uint32_t n = 0, info = 0;
     if(STAGE == STAGE_ROOT) {
@@ -1064,8 +1090,24 @@ __device__ inline uint32_t build_segments(GameLds& L, uint32_t ti, int layer, ui
     uint32_t const off = wave_excl_scan(n);
     L.segOff[s] = off;
     L.segInfo[s] = info;
-    for(uint32_t i = 0; i < n; ++i) L.candSeg[off + i] = static_cast<uint16_t>((s << 6) | i);
     uint32_t const total = __builtin_amdgcn_readlane(off + n, 63);
+    if(cand != nullptr) {
+        for(uint32_t i = 0; i < n; ++i) cand[off + i] = static_cast<uint16_t>((s << 6) | i);
+    } else { // chunk headers (cand_of); every list has at most 34 < 64 placements
+        uint32_t const j = wave_excl_scan(n != 0 ? 1u : 0u);
+        if(n != 0) L.listOf[j] = static_cast<uint8_t>(s);
+        if(s < (total + 63u) / 64u) L.chunkStarts[s] = 0;
+        wave_sync();
+        if(n != 0) {
+            uint32_t const k = off >> 6;
+            if((off & 63u) == 0) {
+                L.chunkList[k] = static_cast<uint8_t>(j); // the list starts chunk k
+            } else {
+                atomicOr(reinterpret_cast<unsigned long long*>(&L.chunkStarts[k]), 1ull << (off & 63u));
+                if(64u * (k + 1) < off + n) L.chunkList[k + 1] = static_cast<uint8_t>(j); // it runs into chunk k + 1
+            }
+        }
+    }
     wave_sync();
     return total;
 }
@@ -1109,7 +1151,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         uint32_t score = 0, s = 0, i = 0;
         uint32_t fpk[REGF ? 16 : 1] = {}; // REGF: the lane's 32 features as bfloat16 pairs (zero for an illegal lane)
         if(f < total) {
-            uint32_t const si = seg[f];
+            uint32_t const si = cand_of(L, seg, f);
             s = si >> 6;
             i = si & 63u;
             uint32_t const info = L.segInfo[s];
@@ -1399,8 +1441,8 @@ constexpr uint32_t PRS_ROW = 36;
 // Legality of a stage's candidates, one per lane: prs[36 s + i] = -inf for illegal placement i of list s; the
 // legal ones are listed (any order, as list << 6 | placement index) in `legalList`. Returns their count.
 template<int G>
-__device__ uint32_t pruner_legal(GameLds& L, uint32_t& count, uint32_t total, float* prs, uint16_t* legalList,
-                                 uint32_t wv) {
+__device__ uint32_t pruner_legal(GameLds& L, uint16_t const* cand, uint32_t& count, uint32_t total, float* prs,
+                                 uint16_t* legalList, uint32_t wv) {
     uint32_t const lane = lane_id();
     if(wv == 0 && lane == 0) count = 0;
     group_sync<G>();
@@ -1408,7 +1450,7 @@ __device__ uint32_t pruner_legal(GameLds& L, uint32_t& count, uint32_t total, fl
         uint32_t const f = base + lane;
         bool legal = false;
         if(f < total) {
-            uint32_t const si = L.candSeg[f];
+            uint32_t const si = cand[f];
             uint32_t const info = L.segInfo[si >> 6];
             uint32_t const piece = (info >> 8) & 7u;
             LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + (si & 63u)];
@@ -1423,7 +1465,7 @@ __device__ uint32_t pruner_legal(GameLds& L, uint32_t& count, uint32_t total, fl
         b = uniform(b);
         if(legal)
             legalList[b + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1)))] =
-                static_cast<uint16_t>(L.candSeg[f]);
+                static_cast<uint16_t>(cand[f]);
     }
     group_sync<G>();
     return uniform(count);
@@ -1475,8 +1517,8 @@ __device__ void pruner_scores(GameLds const& L, PrunerDev const* __restrict__ pr
 // returns how many were kept and lists them in `kept` (list << 6 | placement index, any order). A lane reads its
 // whole list's logits with nine 16-byte loads (lanes of one list read the same addresses).
 template<int G>
-__device__ uint32_t pruner_select(GameLds& L, uint32_t& count, uint32_t total, float const* prs, uint16_t* kept,
-                                  uint32_t K, uint32_t wv) {
+__device__ uint32_t pruner_select(GameLds& L, uint16_t const* cand, uint32_t& count, uint32_t total, float const* prs,
+                                  uint16_t* kept, uint32_t K, uint32_t wv) {
     uint32_t const lane = lane_id();
     if(wv == 0 && lane == 0) count = 0;
     group_sync<G>();
@@ -1485,7 +1527,7 @@ __device__ uint32_t pruner_select(GameLds& L, uint32_t& count, uint32_t total, f
         bool keep = false;
         uint32_t si = 0;
         if(f < total) {
-            si = L.candSeg[f];
+            si = cand[f];
             uint32_t const s = si >> 6, i = si & 63u;
             f32x4 const* row = reinterpret_cast<f32x4 const*>(prs + s * PRS_ROW);
             float const pf = prs[s * PRS_ROW + i];
@@ -1539,15 +1581,15 @@ __device__ inline void emit_record(EvalArgs const& a, bool want, uint32_t const*
 // one record per legal candidate of the stage; label = slot kept by the level (ply 1, 2) or the list's best
 // leaf (ply 3)
 template<int G>
-__device__ void emit_stage(EvalArgs const& a, GameLds const& L, PruneLds const* P, uint32_t total, uint32_t ply,
-                           uint32_t wv) {
+__device__ void emit_stage(EvalArgs const& a, GameLds const& L, uint16_t const* cand, PruneLds const* P, uint32_t total,
+                           uint32_t ply, uint32_t wv) {
     uint32_t const lane = lane_id();
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
         uint32_t const f = base + lane;
         bool legal = false;
         uint32_t meta = 0, pb = COMMITTED;
         if(f < total) {
-            uint32_t const si = L.candSeg[f];
+            uint32_t const si = cand[f];
             uint32_t const s = si >> 6, i = si & 63u;
             uint32_t const info = L.segInfo[s];
             pb = info & 0x7Fu;
@@ -1613,10 +1655,10 @@ __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, 
 
 // wave 0 builds the stage's segment table; every wave gets the candidate count
 template<int STAGE, int G, bool DEDUP = false>
-__device__ inline uint32_t stage_segments(GameLds& L, uint32_t ti, int layer, uint32_t keptCount, uint32_t cur,
-                                          uint32_t wv) {
+__device__ inline uint32_t stage_segments(GameLds& L, uint16_t* cand, uint32_t ti, int layer, uint32_t keptCount,
+                                          uint32_t cur, uint32_t wv) {
     if(wv == 0) {
-        uint32_t const total = build_segments<STAGE, DEDUP>(L, ti, layer, keptCount, cur);
+        uint32_t const total = build_segments<STAGE, DEDUP>(L, cand, ti, layer, keptCount, cur);
         if(lane_id() == 0) L.stageTotal = total;
     }
     group_sync<G>();
@@ -1675,8 +1717,8 @@ __device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
 // The root stage from the previous move's scores (RootCacheMeta): when every root list matches one of the chosen
 // root's level-1 lists, each wave builds the exact top-K of the root keys from them (no scoring) and returns true.
 template<int K>
-__device__ inline bool root_reuse(GameLds const& L, RootCacheMeta const& RC, uint32_t const* level1, uint32_t total,
-                                  uint64_t& top) {
+__device__ inline bool root_reuse(GameLds const& L, uint16_t const* cand, RootCacheMeta const& RC,
+                                  uint32_t const* level1, uint32_t total, uint64_t& top) {
     if(uniform(RC.valid) == 0) return false;
     uint32_t const lane = lane_id();
     uint32_t const info = L.segInfo[lane & 1u]; // lanes 0, 1: the root lists
@@ -1690,7 +1732,7 @@ __device__ inline bool root_reuse(GameLds const& L, RootCacheMeta const& RC, uin
         uint32_t const f = base + lane;
         uint64_t k = 0;
         if(f < total) {
-            uint32_t const si = L.candSeg[f];
+            uint32_t const si = cand_of(L, cand, f);
             uint32_t const s = si >> 6, i = si & 63u;
             uint32_t const score = level1[(s == 0 ? src0 : src1) * 34 + i];
             k = score != 0 ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - (s * 34 + i))) : 0ull;
@@ -1702,9 +1744,9 @@ __device__ inline bool root_reuse(GameLds const& L, RootCacheMeta const& RC, uin
 }
 
 template<int KIND, int G, int MODE>
-__device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheMeta* RC, uint32_t* level1, wptr w,
-                          WideArgs const& wide, uint32_t wv, uint32_t* col, bool traceMove, uint64_t& nApply,
-                          uint32_t& nLegal, uint64_t& nSlots) {
+__device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLds* P, RootCacheMeta* RC,
+                          uint32_t* level1, wptr w, WideArgs const& wide, uint32_t wv, uint32_t* col, bool traceMove,
+                          uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
@@ -1731,14 +1773,14 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     float* const prs = MODE == MODE_PRUNED ? reinterpret_cast<float*>(col - lane - wv * (WIDTH * SCR_STRIDE)) : nullptr;
 
     // stage 0: roots -> top K0
-    uint32_t total = stage_segments<STAGE_ROOT, G>(L, ti, 0, 0, 0, wv);
+    uint32_t total = stage_segments<STAGE_ROOT, G>(L, cand, ti, 0, 0, 0, wv);
     uint64_t mask = 0;
     uint64_t top = 0;
     bool reused = false;
-    if constexpr(ROOT_CACHE) reused = root_reuse<K0>(L, *RC, level1, total, top);
+    if constexpr(ROOT_CACHE) reused = root_reuse<K0>(L, cand, *RC, level1, total, top);
     if(!reused) {
         nApply += total;
-        top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, L.candSeg, total, w, wv, col, nullptr, nLegal, wide);
+        top = run_stage<KIND, STAGE_ROOT, K0, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, cand, total, w, wv, col, nullptr, nLegal, wide);
         nSlots += (total + 63u) & ~63u; // lanes the stage's chunks occupied
         top = combine_parts<G, K0, true>(L, top, mask, wv);
     }
@@ -1757,7 +1799,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     // middle levels 1 and 2 -> top K1, K2
 #pragma unroll
     for(int level = 1; level <= 2; ++level) {
-        total = stage_segments<STAGE_MID, G, DEDUP>(L, ti, level, count, cur, wv);
+        total = stage_segments<STAGE_MID, G, DEDUP>(L, cand, ti, level, count, cur, wv);
         nApply += total;
         uint32_t* keep = nullptr;
         if constexpr(ROOT_CACHE) {
@@ -1766,16 +1808,16 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
                 keep = level1;
             }
         }
-        uint16_t const* seg = L.candSeg;
+        uint16_t const* seg = cand;
         uint32_t evalTotal = total;
         if constexpr(MODE == MODE_PRUNED) {
             if((a.pruneMask >> level) & 1u) {
-                uint32_t const nl = pruner_legal<G>(L, P->keptCount, total, prs, P->kept, wv);
+                uint32_t const nl = pruner_legal<G>(L, cand, P->keptCount, total, prs, P->kept, wv);
                 pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
                 group_sync<G>();
                 pruner_scores<G>(L, a.pruner, P->kept, nl, cur, static_cast<uint32_t>(level), P->bp, prs, wv);
                 group_sync<G>();
-                evalTotal = pruner_select<G>(L, P->keptCount, total, prs, P->kept, a.pruneK, wv);
+                evalTotal = pruner_select<G>(L, cand, P->keptCount, total, prs, P->kept, a.pruneK, wv);
                 seg = P->kept;
             }
         }
@@ -1810,7 +1852,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
                     }
                 }
                 group_sync<G>();
-                emit_stage<G>(a, L, P, total, static_cast<uint32_t>(level), wv);
+                emit_stage<G>(a, L, cand, P, total, static_cast<uint32_t>(level), wv);
             }
         }
         if(wv == 0) write_frontier<REG_BOARDS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
@@ -1820,18 +1862,18 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     }
 
     // leaves
-    total = stage_segments<STAGE_LEAF, G, DEDUP>(L, ti, 3, count, cur, wv);
+    total = stage_segments<STAGE_LEAF, G, DEDUP>(L, cand, ti, 3, count, cur, wv);
     nApply += total;
-    uint16_t const* seg = L.candSeg;
+    uint16_t const* seg = cand;
     uint32_t evalTotal = total;
     if constexpr(MODE == MODE_PRUNED) {
         if((a.pruneMask >> 3) & 1u) {
-            uint32_t const nl = pruner_legal<G>(L, P->keptCount, total, prs, P->kept, wv);
+            uint32_t const nl = pruner_legal<G>(L, cand, P->keptCount, total, prs, P->kept, wv);
             pruner_board_projection<G>(L, a.pruner, cur, count, P->bp, wv);
             group_sync<G>();
             pruner_scores<G>(L, a.pruner, P->kept, nl, cur, 3u, P->bp, prs, wv);
             group_sync<G>();
-            evalTotal = pruner_select<G>(L, P->keptCount, total, prs, P->kept, a.pruneK, wv);
+            evalTotal = pruner_select<G>(L, cand, P->keptCount, total, prs, P->kept, a.pruneK, wv);
             seg = P->kept;
         }
     }
@@ -1852,7 +1894,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, PruneLds* P, RootCacheM
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(DEDUP) mask = repeat_legality(L, mask); // a repeat's leaves never beat its origin's (higher orders)
     if constexpr(MODE == MODE_TRACE) {
-        if(traceMove) emit_stage<G>(a, L, P, total, 3u, wv);
+        if(traceMove) emit_stage<G>(a, L, cand, P, total, 3u, wv);
     }
     {
         // a list where no placement fits scores the frontier board itself at order base (beam.hpp:439-440)
@@ -1936,6 +1978,10 @@ void beam_eval_kernel(EvalArgs a) {
     constexpr bool SCRATCH = (KIND == MODEL_F32W && !WIDE_LDS) || MODE == MODE_PRUNED;
     __shared__ __attribute__((aligned(16))) uint32_t scratch[SCRATCH ? G * WIDTH * SCR_STRIDE : 1];
     __shared__ std::conditional_t<MODE == MODE_PLAIN, uint32_t, PruneLds> pruneStore;
+    // the candidate table (the wide network's plain kernel uses the chunk headers of GameLds instead)
+    constexpr bool CAND_TABLE = !(KIND == MODEL_F32W && MODE == MODE_PLAIN);
+    __shared__ std::conditional_t<CAND_TABLE, CandLds, uint32_t> candStore;
+    uint16_t* const cand = CAND_TABLE ? reinterpret_cast<CandLds*>(&candStore)->seg : nullptr;
     PruneLds* const pruneLds = MODE == MODE_PLAIN ? nullptr : reinterpret_cast<PruneLds*>(&pruneStore);
     // wide network's feature tiles (kernels that keep the candidate boards in LDS)
     __shared__ std::conditional_t<KIND == MODEL_F32W && !WIDE_LDS, FeatLds<G>, uint32_t> featStore;
@@ -2013,7 +2059,7 @@ void beam_eval_kernel(EvalArgs a) {
                                : (a.maxMoves - mv > a.phaseMoves ? mv + a.phaseMoves : a.maxMoves);
         bool ended = false;
         for(; mv < limit; ++mv) {
-            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, pruneLds, RC, level1, w, wide, wv, col,
+            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, cand, pruneLds, RC, level1, w, wide, wv, col,
                                                                          MODE == MODE_TRACE && a.traceEvery != 0
                                                                              && mv % a.traceEvery == tracePhase,
                                                                          nApply, nLegal, nSlots)) {

@@ -136,6 +136,7 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     size_t const wBytes = round256(static_cast<size_t>(num_models) * stride_of(kind) * 4);
     size_t const parkBytes = round256(static_cast<size_t>(items) * PARK_WORDS * 4);
     size_t const resumeBytes = round256((static_cast<size_t>(items) + num_models + 2 + ringSlots) * 4);
+    if(256 + wBytes + parkBytes + resumeBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH; // before touching the device
     int dev = 0;
     if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return TA3H_E_HIP;
     void* base = nullptr;
@@ -147,7 +148,6 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     if(max_active_blocks(kind, wpg, 0, &perCU) != hipSuccess) return TA3H_E_HIP;
     uint64_t blocks = static_cast<uint64_t>(cus) * static_cast<uint32_t>(perCU > 0 ? perCU : 1);
     if(blocks > items) blocks = items;
-    if(256 + wBytes + parkBytes + resumeBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH;
     auto* const s = static_cast<hipStream_t>(stream);
     uint8_t* const p = static_cast<uint8_t*>(base);
     uint32_t* const counters = reinterpret_cast<uint32_t*>(p);
