@@ -126,7 +126,7 @@ int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements)
  * scores a repeated list once (a list of a level whose parent board, clear histogram, piece and held-is-I equal
  * an earlier list's: the same candidates with the same values) and, after a placement that cleared no line,
  * takes the root lists from the previous move's level-1 scores when they match (V11 and 16-32-32-1: any kept
- * root; the wide network: the best one; not in pruned searches that prune level 1, nor in trace and move-log
+ * root; the wide network: the six best; not in pruned searches that prune level 1, nor in trace and move-log
  * launches, which score every candidate): the reference's search fits and scores ~21% more candidates. */
 int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 
