@@ -947,15 +947,15 @@ __device__ inline int place_regs(uint32_t const* parent, uint32_t cols, int col0
     }
     if(full == 0) return 0;
     int const n = popc(full);
+    // one pass per full row over all ten columns (the row's masks are computed once): the rows above the
+    // top-most remaining full row move down one, the rows under it stay
+    uint32_t rows = full;
+    for(int i = 0; i < n; ++i) {
+        uint32_t const row = rows & (~rows + 1);
+        uint32_t const under = ~((row << 1) - 1);
 #pragma unroll
-    for(int x = 0; x < WIDTH; ++x) {
-        uint32_t c = b[x], rows = full;
-        for(int i = 0; i < n; ++i) {
-            uint32_t const row = rows & (~rows + 1); // top-most remaining full row
-            c = ((c & (row - 1)) << 1) | (c & ~((row << 1) - 1));
-            rows ^= row;
-        }
-        b[x] = c;
+        for(int x = 0; x < WIDTH; ++x) b[x] = (b[x] & under) | ((b[x] << 1) & ~under);
+        rows ^= row;
     }
     return n;
 }
