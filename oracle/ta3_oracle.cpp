@@ -1067,13 +1067,38 @@ struct RootReuse {
     uint32_t level1[64] = {}; // this search's level-1 lists of kept root k: [2 k + child]
 };
 
+// Lists of levels 1 and 2 scored by the previous move (the device's plain 16-32-32-1 kernels keep the scores of
+// its level-2 and leaf lists). When RootReuse applies, a root of this search is a level-1 candidate of the chosen
+// root; if that candidate was a kept level-1 node of the previous search, each of its level-2 lists with the same
+// piece and held-is-I is a level-1 list of this search with the same content (the board follows from the path,
+// and the clear histograms agree because the committed placement cleared no line). Likewise one level deeper: a
+// kept level-1 node of this search in such a list is a level-2 candidate of the previous search, and if that was
+// kept, its leaf lists are this search's level-2 lists. The device takes those lists' scores from the previous
+// move when they are not repeats; the search here scores them and the counters record them (with RootReuse's).
+struct DeepReuse {
+    uint32_t rank = 0;                  // the chosen root's rank: its level-1 lists are 2 rank and 2 rank + 1
+    uint32_t rootChild[2] = {};         // this search: child c of the chosen root whose level-1 list root list g is
+    uint32_t kept1[32] = {}, n1 = 0;    // slots (list * 34 + placement) of the kept level-1 nodes, in rank order
+    uint32_t kept2[32] = {}, n2 = 0;    // and of the kept level-2 nodes
+    uint32_t key2[64] = {}, keyL[64] = {}; // level-2 and leaf lists: piece | heldIsI << 3 | 16 (0 = no list)
+    int32_t id1[64] = {};               // this search's level-1 list -> the previous search's level-2 list, or -1
+};
+
 struct BeamWorkspace {
     std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
     std::vector<uint8_t> hasChild;
     Frontier kept[2];
     ListSeen seen;
     RootReuse reuse;
+    DeepReuse deep;
 };
+
+// the searches whose device kernels keep level-2 and leaf scores for the next move (DeepReuse): the full-beam
+// 16-32-32-1 network at depth 4 (the LDS of V11's and the wide network's kernels has no room for them; pruned
+// searches score only part of a pruned level)
+inline bool deep_reuse_on(ModelRef const& model, PrunerRef const& pr, int depth) {
+    return model.kind == TA3O_MODEL_F16 && pr.q == nullptr && depth == 4;
+}
 
 // search_move_beam (:483-570) at runtime depth = ks.size() + 1
 SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uint32_t> const& ks,
@@ -1108,7 +1133,11 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         Node const& node = tree.layers[0][gi];
         uint32_t const key = slot[node.slot] | (heldIsI(node) ? 8u : 0u) | 16u;
         reuseRoots = reuseRoots && (ws.reuse.keys[0] == key || ws.reuse.keys[1] == key);
+        if(gi < 2) ws.deep.rootChild[gi] = ws.reuse.keys[0] == key ? 0u : 1u;
     }
+    // DeepReuse: this search takes lists of the previous one (its state is written by every search of such a kind)
+    bool const deepOn = deep_reuse_on(model, pr, depth) && maxChildMid == 2 && maxLast == 2;
+    bool const deepUse = deepOn && reuseRoots;
     std::fill(std::begin(ws.reuse.level1), std::end(ws.reuse.level1), 0u);
     ws.rootScores.assign(static_cast<size_t>(TREES.maxRootGroups[depth]) * MP, 0u);
     for(size_t gi = 0; gi < tree.layers[0].size(); ++gi) {
@@ -1167,8 +1196,30 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> level) & 1u) != 0;
         std::vector<uint8_t> legalMid(midSlots, 0);
         ws.seen.clear();
+        DeepReuse& D = ws.deep;
+        uint32_t* const newKeys = level == 2 ? D.key2 : nullptr; // this level's list keys, for the next search
+        if(deepOn && level == 1) std::fill(std::begin(D.id1), std::end(D.id1), -1);
+        if(deepOn && level == 2) std::fill(std::begin(D.key2), std::end(D.key2), 0u);
         for(uint32_t k = 0; k < keptCount; ++k) {
             Node const& pn = tree.layers[level - 1][F.node[k]];
+            // DeepReuse: kept node k as a kept node of the previous search one level down (index into its kept list)
+            int prevNode = -1;
+            if(deepUse) {
+                int32_t prevSlot = -1;
+                if(level == 1) {
+                    uint32_t const rs = ws.topRootSlots[k];
+                    prevSlot = static_cast<int32_t>((2 * D.rank + D.rootChild[rs / MP]) * 34 + rs % MP);
+                } else if(D.id1[D.kept1[k] / 34] >= 0) {
+                    prevSlot = D.id1[D.kept1[k] / 34] * 34 + static_cast<int32_t>(D.kept1[k] % 34);
+                }
+                uint32_t const* const prevKept = level == 1 ? D.kept1 : D.kept2;
+                uint32_t const prevCount = level == 1 ? D.n1 : D.n2;
+                for(uint32_t q = 0; q < prevCount && prevSlot >= 0; ++q)
+                    if(prevKept[q] == static_cast<uint32_t>(prevSlot)) {
+                        prevNode = static_cast<int>(q);
+                        break;
+                    }
+            }
             for(int ci = pn.childBegin; ci < pn.childEnd; ++ci) {
                 Node const& cn = tree.layers[level][ci];
                 uint32_t const piece = slot[cn.slot];
@@ -1176,6 +1227,17 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 size_t const base = (static_cast<size_t>(k) * maxChildMid + (ci - pn.childBegin)) * MP;
                 int const n = LUT.count(piece);
                 bool const rep = ws.seen.repeat(F.board[k], F.clears[k], piece, hI);
+                uint32_t const lkey = piece | (hI ? 8u : 0u) | 16u;
+                uint32_t const li = 2 * k + static_cast<uint32_t>(ci - pn.childBegin); // the list's index
+                int32_t prevList = -1; // the previous search's list with this content (DeepReuse)
+                if(prevNode >= 0) {
+                    uint32_t const* const prevKeys = level == 1 ? D.key2 : D.keyL;
+                    for(uint32_t c2 = 0; c2 < 2 && prevList < 0; ++c2)
+                        if(prevKeys[2 * prevNode + c2] == lkey) prevList = static_cast<int32_t>(2 * prevNode + c2);
+                }
+                if(deepOn && level == 1) D.id1[li] = prevList;
+                if(deepOn && newKeys != nullptr) newKeys[li] = lkey;
+                bool const deepHit = prevList >= 0 && !rep;
                 if(level == 1 && k < 32 && ci - pn.childBegin < 2)
                     ws.reuse.level1[2 * k + (ci - pn.childBegin)] = piece | (hI ? 8u : 0u) | 16u;
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
@@ -1184,6 +1246,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                     Fit const f = fit(F.board[k], LUT.nth(piece, i).shape);
                     ++cnt.applies;
                     cnt.repeatApplies += rep;
+                    cnt.reuseApplies += deepHit;
                     legal[i] = f.legal ? 1 : 0;
                     dropTo[i] = f.drop;
                     keep[i] = legal[i];
@@ -1196,6 +1259,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                     if(!keep[i]) continue;
                     ++cnt.legal;
                     cnt.repeatLegal += rep;
+                    cnt.reuseLegal += deepHit;
                     Step const st = apply(F.board[k], F.clears[k], LUT.nth(piece, i));
                     (void)dropTo;
                     ws.midScores[base + i] = ordered_bits(model.value(st.clears, st.board, hI));
@@ -1203,6 +1267,11 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
             }
         }
         uint32_t const nextCount = select_top(ws.midScores, ks[level], ws.topSlots.data(), ws.topScores.data());
+        if(deepOn) { // this level's kept nodes, for the next level and the next search
+            uint32_t* const kept = level == 1 ? D.kept1 : D.kept2;
+            for(uint32_t j = 0; j < nextCount; ++j) kept[j] = ws.topSlots[j];
+            (level == 1 ? D.n1 : D.n2) = nextCount;
+        }
         if(trace.on) { // every legal candidate of the level; label = it survived the level's top-K
             std::vector<uint8_t> survived(midSlots, 0);
             for(uint32_t j = 0; j < nextCount; ++j) survived[ws.topSlots[j]] = 1;
@@ -1249,6 +1318,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
         int const leafPly = depth - 1;
         bool const pruneHere = pr.q != nullptr && ((pr.plyMask >> leafPly) & 1u) != 0;
         ws.seen.clear();
+        if(deepOn) std::fill(std::begin(ws.deep.keyL), std::end(ws.deep.keyL), 0u);
         for(uint32_t j = 0; j < keptCount; ++j) {
             Node const& ln = tree.layers[depth - 2][F.node[j]];
             for(int ci = ln.childBegin; ci < ln.childEnd; ++ci) {
@@ -1258,6 +1328,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 uint32_t const base = (j * maxLast + (ci - ln.childBegin)) * MP;
                 int const n = LUT.count(piece);
                 bool const rep = ws.seen.repeat(F.board[j], F.clears[j], piece, hI);
+                if(deepOn) ws.deep.keyL[2 * j + (ci - ln.childBegin)] = piece | (hI ? 8u : 0u) | 16u;
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
                 for(int i = 0; i < n; ++i) {
                     legal[i] = fit(F.board[j], LUT.nth(piece, i).shape).legal ? 1 : 0;
@@ -1316,6 +1387,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     bool const keep = rootRank < keptRoots && !level1Pruned;
     ws.reuse.keys[0] = keep ? ws.reuse.level1[2 * rootRank] : 0u;
     ws.reuse.keys[1] = keep ? ws.reuse.level1[2 * rootRank + 1] : 0u;
+    ws.deep.rank = rootRank; // DeepReuse applies to the next search where RootReuse does
     return {true, rn.rootHold, pl0.o, pl0.x, ordered_bits_inv(static_cast<uint32_t>(bestKey >> 32))};
 }
 

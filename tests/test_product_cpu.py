@@ -79,7 +79,7 @@ def test_noalloc_refuses_batches_beyond_its_scratch_without_gpu():
     refused with TA3H_E_SCRATCH, bad arguments with TA3H_E_ARG, an empty batch succeeds"""
     fake = 1 << 20  # never dereferenced: the checks come first
     assert T.eval_noalloc(T.MODEL_F16, fake, 1000, fake, 1000, 100, fake) == T.E_SCRATCH
-    assert T.eval_noalloc(T.MODEL_F32W, fake, 1024, fake, 1, 100, fake) == T.E_SCRATCH  # 43.6 MB of weights
+    assert T.eval_noalloc(T.MODEL_F32W, fake, 8192, fake, 1, 100, fake) == T.E_SCRATCH  # 348.7 MB of weights
     assert T.eval_noalloc(2, fake, 4, fake, 4, 100, fake) == 1
     assert T.eval_noalloc(T.MODEL_V11, 0, 4, fake, 4, 100, fake) == 1
     assert T.eval_noalloc(T.MODEL_V11, fake, 0, fake, 4, 100, fake) == 0

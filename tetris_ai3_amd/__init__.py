@@ -577,7 +577,7 @@ def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_m
 
 
 E_SCRATCH = 4
-NOALLOC_SCRATCH_BYTES = 32 << 20
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 
 def eval_noalloc(kind: int, d_weights: int, num_models: int, d_seeds: int, games: int, max_moves: int,

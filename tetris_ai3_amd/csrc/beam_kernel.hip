@@ -269,6 +269,37 @@ struct alignas(16) RootScoresLds {
 };
 static_assert(RC_WORDS == 3 + 2 * 34, "a parked game's root scores: valid, nextKey, two lists");
 
+// Lists of levels 1 and 2 scored by the previous move (the plain 16-32-32-1 kernels). Where the root lists come
+// from the previous move (RootCacheMeta), a root of this move is a level-1 candidate of the chosen root; if that
+// candidate was a kept level-1 node of the previous search, its level-2 lists with this list's piece and held-is-I
+// are this move's level-1 lists of that root, with the same content (the board follows from the path; the clear
+// histograms agree because the committed placement cleared no line). One level deeper likewise: a kept level-1
+// node of this move in such a list was a level-2 candidate of the previous search, and if that was kept, its leaf
+// lists are this move's level-2 lists. Such lists, when not repeats, are not scored: their candidates follow the
+// scored ones in the stage's candidate table, and their keys come from the previous move's scores. Level 2 and
+// the leaves keep their scores, list infos and legality, levels 1 and 2 their kept slots, for the next move; a
+// parked game carries them (the record part) when its roots may be reused.
+struct alignas(16) DeepLds {
+    uint32_t s2[64 * 34]; // level-2 candidate scores by slot (0: illegal or 0), read by the next move's level 1
+    uint32_t sL[64 * 34]; // leaf candidate scores by slot, read by the next move's level 2
+    uint32_t info2[64];   // segInfo of the level-2 lists
+    uint32_t infoL[64];   // segInfo of the leaf lists
+    uint64_t legal2;      // level-2 lists with a legal placement
+    uint64_t legalL;      // leaf lists with a legal placement
+    uint16_t kept1[32];   // slots of the kept level-1 nodes, in rank order
+    uint16_t kept2[32];   // slots of the kept level-2 nodes
+    uint32_t n1, n2;      // their counts
+    uint32_t rank;        // rank of the committed root: its level-1 lists were 2 rank and 2 rank + 1
+    // the current move (not in the record)
+    uint32_t rootChild[2]; // roots reused: child of the previous chosen root whose level-1 list is root list g
+    uint32_t reusedTotal;  // candidates of the stage's lists taken from the previous move
+    uint64_t reusedLegal;  // those lists with a legal placement
+    int8_t id1[64];        // level-1 list -> the previous move's level-2 list with its content, or -1
+    uint8_t src[64];       // a list taken from the previous move: the previous list holding its scores
+};
+constexpr uint32_t DEEP_RECORD_WORDS = offsetof(DeepLds, rootChild) / 4;
+static_assert(DEEP_RECORD_WORDS <= DEEP_PARK_WORDS, "the record part of DeepLds fits a parked game's record");
+
 __device__ inline uint64_t rng_next(uint64_t* s) { // xoshiro256ss.hpp:33-45
     uint64_t const r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9;
     uint64_t const t = s[1] << 17;
@@ -1034,10 +1065,13 @@ __device__ inline uint32_t cand_of(GameLds const& L, uint16_t const* seg, uint32
 
 // Builds the stage's segment table in LDS. Segment s = 2k + c: child c of kept node k of buffer `cur`
 // (roots: s = root group, parent = the committed board). With DEDUP (middle levels and leaves), repeated lists
-// get no candidates. Returns the candidate count.
+// get no candidates. With D (DeepLds; middle levels and leaves), the middle levels' lists with the content of a
+// previous-move list (deepUse: the roots were reused) get no candidates either; theirs follow in the candidate
+// table, D->reusedTotal of them, and the level-2 and leaf lists' infos are recorded for the next move. Returns the
+// count of candidates to score.
 template<int STAGE, bool DEDUP = false>
 __device__ inline uint32_t build_segments(GameLds& L, uint16_t* cand, uint32_t ti, int layer, uint32_t keptCount,
-                                          uint32_t cur) {
+                                          uint32_t cur, DeepLds* D = nullptr, bool deepUse = false) {
     uint32_t const s = lane_id();
     uint32_t n = 0, info = 0;
     if(STAGE == STAGE_ROOT) {
@@ -1087,10 +1121,68 @@ __device__ inline uint32_t build_segments(GameLds& L, uint16_t* cand, uint32_t t
             n = 0;
         }
     }
+    uint32_t nReused = 0; // candidates of this list taken from the previous move (DeepLds)
+    if(STAGE == STAGE_MID && D != nullptr) {
+        bool const lvl1 = layer == 1;
+        // lane k < keptCount: kept node k as a candidate slot of the previous search one level down
+        int32_t prevSlot = -1;
+        if(deepUse && s < keptCount) {
+            if(lvl1) { // a root: a level-1 candidate of the previous chosen root
+                uint32_t const rs = L.rootSlot[s];
+                prevSlot = static_cast<int32_t>((2 * D->rank + D->rootChild[rs / 34]) * 34 + rs % 34);
+            } else { // a kept level-1 node: a level-2 candidate where its list has a previous-move match
+                uint32_t const ks = D->kept1[s];
+                int32_t const id = D->id1[ks / 34];
+                prevSlot = id >= 0 ? id * 34 + static_cast<int32_t>(ks % 34) : -1;
+            }
+        }
+        // ... and the previous search's kept node of that slot (slots are unique; 12 bits)
+        uint32_t const prevCount = lvl1 ? D->n1 : D->n2;
+        bool const theirValid = s < 32 && s < prevCount;
+        uint32_t const theirs = theirValid ? static_cast<uint32_t>(lvl1 ? D->kept1[s] : D->kept2[s]) : 0u;
+        uint64_t const hit = match_keys<12>(static_cast<uint32_t>(prevSlot) & 0xFFFu, theirs, __ballot(theirValid));
+        int32_t const prevNode = prevSlot >= 0 && hit != 0 ? static_cast<int32_t>(__builtin_ctzll(hit)) : -1;
+        int32_t const pn = __shfl(prevNode, static_cast<int>(s >> 1)); // this list's parent's
+        // the previous node's list with this list's piece and held-is-I
+        int32_t prevList = -1;
+        uint32_t prevInfo = 0;
+        if(pn >= 0 && (info & SEG_EXISTS) != 0) {
+            uint32_t const* const pinfo = lvl1 ? D->info2 : D->infoL;
+#pragma unroll
+            for(uint32_t c2 = 0; c2 < 2; ++c2) {
+                uint32_t const q = pinfo[2 * static_cast<uint32_t>(pn) + c2];
+                if(prevList < 0 && (q & SEG_EXISTS) != 0 && ((q >> 8) & 15u) == ((info >> 8) & 15u)) {
+                    prevList = 2 * pn + static_cast<int32_t>(c2);
+                    prevInfo = q;
+                }
+            }
+        }
+        if(lvl1) D->id1[s] = static_cast<int8_t>(prevList);
+        bool const reuse = prevList >= 0 && (info & SEG_REPEAT) == 0;
+        uint64_t const prevLegal = lvl1 ? D->legal2 : D->legalL;
+        uint64_t const reusedLegal = __ballot(reuse && ((prevLegal >> (static_cast<uint32_t>(prevList) & 63u)) & 1ull) != 0);
+        if(reuse) { // its scores: the previous list's, or its origin's when that was a repeat
+            D->src[s] = static_cast<uint8_t>((prevInfo & SEG_REPEAT) != 0 ? seg_origin(prevInfo)
+                                                                          : static_cast<uint32_t>(prevList));
+            nReused = n;
+            n = 0;
+        }
+        if(s == 0) D->reusedLegal = reusedLegal;
+    }
     uint32_t const off = wave_excl_scan(n);
     L.segOff[s] = off;
     L.segInfo[s] = info;
     uint32_t const total = __builtin_amdgcn_readlane(off + n, 63);
+    if(STAGE != STAGE_ROOT && D != nullptr) {
+        if(STAGE == STAGE_MID) { // the reused lists' candidates after the scored ones
+            uint32_t const roff = wave_excl_scan(nReused);
+            for(uint32_t i = 0; i < nReused; ++i) cand[total + roff + i] = static_cast<uint16_t>((s << 6) | i);
+            uint32_t const rt = __builtin_amdgcn_readlane(roff + nReused, 63);
+            if(s == 0) D->reusedTotal = rt;
+        }
+        if(layer == 2) D->info2[s] = info; // for the next move
+        if(STAGE == STAGE_LEAF) D->infoL[s] = info;
+    }
     if(cand != nullptr) {
         for(uint32_t i = 0; i < n; ++i) cand[off + i] = static_cast<uint16_t>((s << 6) | i);
     } else { // chunk headers (cand_of); every list has at most 34 < 64 placements
@@ -1139,9 +1231,10 @@ struct WideArgs {
 
 template<int KIND, int STAGE, int K, int G, bool TRACE = false, int WIDE_CB = 4, bool LDSW = false>
 __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t total, wptr w, uint32_t wv, uint32_t* col,
-                                     uint64_t* mask, uint32_t& nLegal, WideArgs const& wide, uint32_t* keep = nullptr) {
+                                     uint64_t* mask, uint32_t& nLegal, WideArgs const& wide, uint32_t* keep = nullptr,
+                                     uint32_t keepSlots = 0, uint64_t acc0 = 0) {
     uint32_t const lane = lane_id();
-    uint64_t acc = 0; // ROOT/MID: running top-K; LEAF: per-lane best key
+    uint64_t acc = acc0; // ROOT/MID: running top-K (acc0: keys merged before); LEAF: per-lane best key
     uint64_t ms = 0;
     constexpr bool REGF = KIND == MODEL_F32W && LDSW; // wide features stay in registers (no feature tile)
     for(uint32_t base = wv * 64; base < total; base += 64 * G) {
@@ -1201,7 +1294,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
         }
         nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
         uint32_t const slot = s * 34 + i;
-        if(keep != nullptr && f < total && slot < RC_ROOTS<KIND> * 2 * 34) // level-1 scores for the next move's roots
+        if(keep != nullptr && f < total && slot < keepSlots) // scores kept for the next move (its roots, DeepLds)
             keep[slot] = legal ? score : 0u;
         if constexpr(STAGE == STAGE_LEAF) {
             uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
@@ -1656,9 +1749,9 @@ __device__ inline void write_frontier(GameLds& L, uint64_t key, uint32_t count, 
 // wave 0 builds the stage's segment table; every wave gets the candidate count
 template<int STAGE, int G, bool DEDUP = false>
 __device__ inline uint32_t stage_segments(GameLds& L, uint16_t* cand, uint32_t ti, int layer, uint32_t keptCount,
-                                          uint32_t cur, uint32_t wv) {
+                                          uint32_t cur, uint32_t wv, DeepLds* D = nullptr, bool deepUse = false) {
     if(wv == 0) {
-        uint32_t const total = build_segments<STAGE, DEDUP>(L, cand, ti, layer, keptCount, cur);
+        uint32_t const total = build_segments<STAGE, DEDUP>(L, cand, ti, layer, keptCount, cur, D, deepUse);
         if(lane_id() == 0) L.stageTotal = total;
     }
     group_sync<G>();
@@ -1679,6 +1772,10 @@ static_assert(f32w::BYTES % 16 == 0, "copied as 16-byte words");
 
 
 enum KernelMode { MODE_PLAIN = 0, MODE_TRACE = 1, MODE_PRUNED = 2 };
+// the kernels that take lists of levels 1 and 2 from the previous move (DeepLds): the full-beam 16-32-32-1 search
+// (V11's workgroups are LDS-bound at 2 waves per game, and the wide network's LDS is full)
+template<int KIND, int MODE>
+constexpr bool DEEP_REUSE = KIND == MODEL_F16 && MODE == MODE_PLAIN;
 
 // Work ring of the second phase (EvalArgs::ring; thread 0 of a workgroup). Returns the next parked game, or
 // RING_EMPTY when no slot is left to take. A workgroup that finds the ring empty ends: every unfinished game is then
@@ -1714,6 +1811,34 @@ __device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
 // body, model_eval_host.cpp:37-47), played by the G waves of the workgroup. Returns false when the game
 // ends (no move, or a top-out). MODE_PRUNED scores only the pruner's top-k placements of every list at the
 // pruned plies; MODE_TRACE records the candidates of plies 1..3 when traceMove is set.
+// The wave's top-K of a middle level's lists taken from the previous move (DeepLds), which seeds the top-K of its
+// scored chunks: their candidates follow the scored ones in the stage's table (chunks wv, wv + G, ...), their keys
+// come from the previous move's scores (`prev`, at the list that holds them), and `keep` gets their scores for the
+// next move as a scored candidate's. Taken first, these keys (the previous chosen root's best continuations) raise
+// the threshold that most scored chunks then fail without a sort: the 16-32-32-1 generation ran 1.2% faster than
+// with these keys merged after the scored chunks.
+template<int K, int G>
+__device__ inline uint64_t deep_topk(uint16_t const* cand, DeepLds const& D, uint32_t const* prev, uint32_t total,
+                                     uint32_t* keep, uint32_t keepSlots, uint32_t wv, uint64_t& mask) {
+    uint32_t const rt = uniform(D.reusedTotal);
+    uint32_t const lane = lane_id();
+    uint64_t acc = 0;
+    for(uint32_t base = wv * 64; base < rt; base += 64 * G) {
+        uint32_t const f = base + lane;
+        uint64_t key = 0;
+        if(f < rt) {
+            uint32_t const si = cand[total + f];
+            uint32_t const s = si >> 6, i = si & 63u, slot = s * 34 + i;
+            uint32_t const score = prev[D.src[s] * 34u + i];
+            if(keep != nullptr && slot < keepSlots) keep[slot] = score;
+            key = score != 0 ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0ull;
+        }
+        acc = topk_merge<K>(acc, key);
+    }
+    if(wv == 0) mask |= D.reusedLegal;
+    return acc;
+}
+
 // The root stage from the previous move's scores (RootCacheMeta): when every root list matches one of the chosen
 // root's level-1 lists, each wave builds the exact top-K of the root keys from them (no scoring) and returns true.
 template<int K>
@@ -1745,8 +1870,8 @@ __device__ inline bool root_reuse(GameLds const& L, uint16_t const* cand, RootCa
 
 template<int KIND, int G, int MODE>
 __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLds* P, RootCacheMeta* RC,
-                          uint32_t* level1, wptr w, WideArgs const& wide, uint32_t wv, uint32_t* col, bool traceMove,
-                          uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
+                          uint32_t* level1, DeepLds* D, wptr w, WideArgs const& wide, uint32_t wv, uint32_t* col,
+                          bool traceMove, uint64_t& nApply, uint32_t& nLegal, uint64_t& nSlots) {
     uint32_t const lane = lane_id();
     constexpr int CB = MODE == MODE_PLAIN ? 4 : 2; // wide network: column tiles per weight load (register budget)
     // candidate boards in registers: the FP32 networks' kernels and the wide network's 4-wave plain kernel
@@ -1757,6 +1882,9 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
     // roots from the previous move's level-1 scores (RootCacheMeta; a pruned search whose level 1 is pruned keeps
     // only some of its scores and takes none)
     constexpr bool ROOT_CACHE = MODE != MODE_TRACE;
+    // lists of levels 1 and 2 from the previous move, when its roots are reused (DeepLds)
+    constexpr bool DEEP = DEEP_REUSE<KIND, MODE>;
+    constexpr uint32_t LEVEL1_SLOTS = RC_ROOTS<KIND> * 2 * 34, DEEP_SLOTS = 64 * 34;
     // gather the committed state (beam.hpp:503-505)
     if(wv == 0) {
         if(lane < WIDTH) L.boards[COMMITTED * WIDTH + lane] = L.eng.cols[lane];
@@ -1790,6 +1918,10 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
     if(count == 0) return false;
     if(wv == 0) {
         if(lane < K0) L.rootSlot[lane] = 0xFFFFFFFFu - static_cast<uint32_t>(top);
+        if constexpr(DEEP) { // root list g is the previous chosen root's level-1 list of child rootChild[g]
+            uint32_t const key = ((L.segInfo[lane & 1u] >> 8) & 15u) | 16u;
+            if(lane < 2) D->rootChild[lane] = RC->nextKey[0] == key ? 0u : 1u;
+        }
         write_frontier<REG_BOARDS>(L, top, count, ti, 0, 0, 0, true, col);
     }
     group_sync<G>();
@@ -1799,13 +1931,21 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
     // middle levels 1 and 2 -> top K1, K2
 #pragma unroll
     for(int level = 1; level <= 2; ++level) {
-        total = stage_segments<STAGE_MID, G, DEDUP>(L, cand, ti, level, count, cur, wv);
+        total = stage_segments<STAGE_MID, G, DEDUP>(L, cand, ti, level, count, cur, wv, DEEP ? D : nullptr, reused);
         nApply += total;
         uint32_t* keep = nullptr;
+        uint32_t keepSlots = 0;
         if constexpr(ROOT_CACHE) {
             if(level == 1) {
                 if(wv == 0 && lane < 32) RC->info1[lane] = L.segInfo[lane];
                 keep = level1;
+                keepSlots = LEVEL1_SLOTS;
+            }
+        }
+        if constexpr(DEEP) {
+            if(level == 2) {
+                keep = D->s2;
+                keepSlots = DEEP_SLOTS;
             }
         }
         uint16_t const* seg = cand;
@@ -1822,13 +1962,22 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
             }
         }
         mask = 0;
-        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep)
-                         : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        // the lists taken from the previous move (their scores: its level 2 / its leaves) seed the top-K
+        uint64_t first = 0, reusedMask = 0;
+        if constexpr(DEEP)
+            first = level == 1 ? deep_topk<K1, G>(cand, *D, D->s2, total, keep, keepSlots, wv, reusedMask)
+                               : deep_topk<K2, G>(cand, *D, D->sL, total, keep, keepSlots, wv, reusedMask);
+        top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep, keepSlots, first)
+                         : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep, keepSlots, first);
+        mask |= reusedMask;
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         if constexpr(DEDUP) {
             mask = repeat_legality(L, mask);
             top = add_repeats<K2>(L, top);
+        }
+        if constexpr(DEEP) {
+            if(level == 2 && wv == 0 && lane == 0) D->legal2 = mask; // for the next move
         }
         top = lane < a.beamWidth[level] ? top : 0; // narrower beam: the first beamWidth[level] keys
         mask = node_mask(mask);
@@ -1855,14 +2004,21 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
                 emit_stage<G>(a, L, cand, P, total, static_cast<uint32_t>(level), wv);
             }
         }
-        if(wv == 0) write_frontier<REG_BOARDS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
+        if(wv == 0) {
+            if constexpr(DEEP) { // the level's kept slots, for the next level and the next move
+                if(lane < 32) (level == 1 ? D->kept1 : D->kept2)[lane] =
+                    static_cast<uint16_t>(0xFFFFFFFFu - static_cast<uint32_t>(top));
+                if(lane == 0) (level == 1 ? D->n1 : D->n2) = next;
+            }
+            write_frontier<REG_BOARDS>(L, top, next, ti, level, cur, cur ^ 1u, false, col);
+        }
         group_sync<G>();
         cur ^= 1u;
         count = next;
     }
 
     // leaves
-    total = stage_segments<STAGE_LEAF, G, DEDUP>(L, cand, ti, 3, count, cur, wv);
+    total = stage_segments<STAGE_LEAF, G, DEDUP>(L, cand, ti, 3, count, cur, wv, DEEP ? D : nullptr);
     nApply += total;
     uint16_t const* seg = cand;
     uint32_t evalTotal = total;
@@ -1888,11 +2044,15 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
             leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
         }
     } else {
-        leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide);
+        leaf = run_stage<KIND, STAGE_LEAF, 1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide,
+                                                                                     DEEP ? D->sL : nullptr, DEEP_SLOTS);
     }
     nSlots += (evalTotal + 63u) & ~63u;
     leaf = combine_parts<G, 1, false>(L, leaf, mask, wv);
     if constexpr(DEDUP) mask = repeat_legality(L, mask); // a repeat's leaves never beat its origin's (higher orders)
+    if constexpr(DEEP) {
+        if(wv == 0 && lane == 0) D->legalL = mask; // for the next move
+    }
     if constexpr(MODE == MODE_TRACE) {
         if(traceMove) emit_stage<G>(a, L, cand, P, total, 3u, wv);
     }
@@ -1943,6 +2103,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
                 RC->src[c] = !valid ? c : ((li & SEG_REPEAT) != 0 ? seg_origin(li) : 2 * rank + c);
             }
             RC->valid = valid ? 1u : 0u;
+            if constexpr(DEEP) D->rank = rank;
         }
     }
     group_sync<G>();
@@ -1992,6 +2153,11 @@ void beam_eval_kernel(EvalArgs a) {
     __shared__ std::conditional_t<ROOT_CACHE, RootScoresLds<KIND>, uint32_t> rcScores;
     RootCacheMeta* const RC = ROOT_CACHE ? reinterpret_cast<RootCacheMeta*>(&rcMeta) : nullptr;
     uint32_t* const level1 = ROOT_CACHE ? reinterpret_cast<RootScoresLds<KIND>*>(&rcScores)->s : nullptr;
+    // the previous move's level-2 and leaf lists (16-32-32-1 full beam)
+    constexpr bool DEEP = DEEP_REUSE<KIND, MODE>;
+    __shared__ std::conditional_t<DEEP, DeepLds, uint32_t> deepStore;
+    DeepLds* const D = DEEP ? reinterpret_cast<DeepLds*>(&deepStore) : nullptr;
+    constexpr uint32_t PW = park_words(KIND);
     uint32_t const wv = uniform(threadIdx.x >> 6);
     uint32_t const lane = lane_id();
     uint32_t* const col = SCRATCH ? scratch + wv * (WIDTH * SCR_STRIDE) + lane : nullptr;
@@ -2025,7 +2191,7 @@ void beam_eval_kernel(EvalArgs a) {
                 inf |= (wbits[k] & 0x7F800000u) == 0x7F800000u;
             wide.finite = __ballot(inf) == 0;
         }
-        uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
+        uint32_t const* const park = a.parked + static_cast<uint64_t>(item) * PW;
         uint64_t nApply = 0, nSlots = 0;
         uint32_t nLegal = 0, mv = 0;
         if constexpr(MODE != MODE_PLAIN)
@@ -2043,11 +2209,18 @@ void beam_eval_kernel(EvalArgs a) {
                     for(uint32_t k = lane; k < 2 * 34; k += 64) level1[k] = park[PARK_RC + 3 + k];
                 }
             }
+            if constexpr(DEEP) { // the deeper lists, when the roots may be reused
+                if(wv == 0 && uniform(park[PARK_RC]) != 0) {
+                    uint32_t* const d = reinterpret_cast<uint32_t*>(D);
+                    for(uint32_t k = lane; k < DEEP_RECORD_WORDS; k += 64) d[k] = park[PARK_WORDS + k];
+                }
+            }
         } else if(threadIdx.x == 0) {
             if(a.timeline) a.timeline[2 * static_cast<uint64_t>(item)] = wall_clock64();
             engine_reset(L.eng, a.seeds[g]);
             if(a.garbage) add_garbage_rows(L.eng, a.seeds[g]);
             if constexpr(ROOT_CACHE) RC->valid = 0; // the first move has no previous search
+            if constexpr(DEEP) D->n1 = D->n2 = 0;
         }
         group_sync<G>();
         // traced moves: index congruent to the game seed modulo traceEvery (spreads them over the games)
@@ -2059,7 +2232,7 @@ void beam_eval_kernel(EvalArgs a) {
                                : (a.maxMoves - mv > a.phaseMoves ? mv + a.phaseMoves : a.maxMoves);
         bool ended = false;
         for(; mv < limit; ++mv) {
-            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, cand, pruneLds, RC, level1, w, wide, wv, col,
+            if(uniform(L.eng.gameOver) != 0 || !play_move<KIND, G, MODE>(a, L, cand, pruneLds, RC, level1, D, w, wide, wv, col,
                                                                          MODE == MODE_TRACE && a.traceEvery != 0
                                                                              && mv % a.traceEvery == tracePhase,
                                                                          nApply, nLegal, nSlots)) {
@@ -2071,13 +2244,19 @@ void beam_eval_kernel(EvalArgs a) {
         group_sync<G>();
         if(!ended && limit < a.maxMoves) { // park the game for the second phase or its next turn
             // (wave 0 writes the whole record, so the release of ring_push, in the same wave, covers it)
-            uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PARK_WORDS;
+            uint32_t* const out = a.parked + static_cast<uint64_t>(item) * PW;
             if(wv == 0 && lane < ENGINE_WORDS) out[lane] = reinterpret_cast<uint32_t const*>(&L.eng)[lane];
             if constexpr(ROOT_CACHE) {
                 if(wv == 0) {
                     if(lane < 3) out[PARK_RC + lane] = (&RC->valid)[lane];
                     for(uint32_t k = lane; k < 2 * 34; k += 64)
                         out[PARK_RC + 3 + k] = level1[RC->src[k / 34] * 34 + k % 34];
+                }
+            }
+            if constexpr(DEEP) {
+                if(wv == 0 && uniform(RC->valid) != 0) {
+                    uint32_t const* const d = reinterpret_cast<uint32_t const*>(D);
+                    for(uint32_t k = lane; k < DEEP_RECORD_WORDS; k += 64) out[PARK_WORDS + k] = d[k];
                 }
             }
             if(threadIdx.x == 0) {

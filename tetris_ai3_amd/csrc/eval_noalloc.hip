@@ -4,7 +4,7 @@
 // static device array per GPU:
 //   [0, 256)            work-queue counter, parked-game counter
 //   weights             the models' weights repacked into the network's device layout (a kernel, not the host)
-//   parked games        PARK_WORDS words per item (two-phase schedule)
+//   parked games        park_words(kind) words per item (two-phase schedule)
 //   resume lists        parked list [items], resume order [items], per-model parked counts [models]
 // Nothing here calls hipMalloc, hipFree or any other allocating routine. Calls that overlap on one GPU share the
 // array and race; every counter the search reads is cleared by a kernel at the start of the call.
@@ -134,7 +134,7 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     if(ringSlots > 0xFFFFFFFFull) return TA3H_E_SCRATCH;
     // scratch sections: counters, repacked weights, parked games, resume list, per-model counts and work ring
     size_t const wBytes = round256(static_cast<size_t>(num_models) * stride_of(kind) * 4);
-    size_t const parkBytes = round256(static_cast<size_t>(items) * PARK_WORDS * 4);
+    size_t const parkBytes = round256(static_cast<size_t>(items) * park_words(kind) * 4);
     size_t const resumeBytes = round256((static_cast<size_t>(items) + num_models + 2 + ringSlots) * 4);
     if(256 + wBytes + parkBytes + resumeBytes > SCRATCH_BYTES) return TA3H_E_SCRATCH; // before touching the device
     int dev = 0;

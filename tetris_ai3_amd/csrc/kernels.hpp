@@ -24,6 +24,10 @@ inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 4198
 }
 
 inline constexpr uint32_t PARK_WORDS = 128; // parked game: engine state, counters, move index, root scores (512 B)
+// the 16-32-32-1 network's plain kernels also carry the previous move's level-2 and leaf scores, list infos,
+// legality and kept slots (the first words of DeepLds in the kernel source)
+inline constexpr uint32_t DEEP_PARK_WORDS = 4520;
+inline constexpr uint32_t park_words(int kind) { return kind == MODEL_F16 ? PARK_WORDS + DEEP_PARK_WORDS : PARK_WORDS; }
 
 // Work-ring slots of a two-phase launch: every parked game enters once after the first phase and once after
 // each further turn of `phase` moves it ends unfinished, at most (maxMoves - phase) / phase times.
@@ -61,7 +65,7 @@ struct EvalArgs {
     // most parked games first. phaseMoves = 0: one phase.
     uint32_t phaseMoves;
     uint32_t resumePhase;
-    uint32_t* parked;      // [items][PARK_WORDS]
+    uint32_t* parked;      // [items][park_words(kind)]
     uint32_t* resumeList;  // [items], in parking order
     uint32_t* resumeCount; // one counter
     uint32_t* modelParked; // [items / games] parked games per model

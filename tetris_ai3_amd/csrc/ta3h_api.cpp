@@ -442,7 +442,7 @@ int prepare_launch(ta3h_evaluator* ev) {
         ev->err = "launch: too many second-phase turns (raise the first-phase budget)";
         return TA3H_E_ARG;
     }
-    if((r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::PARK_WORDS, "malloc parked")))
+    if((r = ev->grow(ev->dParked, ev->parkedCap, static_cast<size_t>(ev->items) * ta3h::park_words(ev->kind), "malloc parked")))
         return r;
     return ev->grow(ev->dResume, ev->resumeCap, resume_words(ev), "malloc resume list");
 }
@@ -475,7 +475,7 @@ int enqueue_eval(ta3h_evaluator* ev, bool events) {
     a.garbage = static_cast<uint32_t>(ev->garbage);
     uint32_t const phase = phase_budget(ev);
     if(phase != 0) {
-        if(ev->parkedCap < static_cast<size_t>(ev->items) * ta3h::PARK_WORDS || ev->resumeCap < resume_words(ev)) {
+        if(ev->parkedCap < static_cast<size_t>(ev->items) * ta3h::park_words(ev->kind) || ev->resumeCap < resume_words(ev)) {
             ev->err = "launch: two-phase buffers not prepared";
             return TA3H_E_STATE;
         }
