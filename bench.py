@@ -17,8 +17,9 @@ champion (its own pruner from its own games on training seeds, judged on 1,000 h
 the dominant kernel's algorithmic integer/FP32 VALU work (exact per-item counts of the candidates the kernel fits
 and scores x a fixed per-candidate op model) against the MI355X VALU peak. The kernel scores each repeated list
 once (a list whose parent board, clear histogram, piece and held-is-I equal an earlier list's at the same level:
-the reference scores it again), so these counts are the reference's less its repeats; the CPU leg reports the
-repeat share on its sample. `traffic` is the profiled HBM bytes per launch when a PMC summary for the workload is
+the reference scores it again) and takes lists the previous move scored (its roots, and for 16-32-32-1 lists of
+levels 1 and 2) from that move's scores, so these counts are the reference's less its repeats and those lists;
+the CPU leg reports both shares on its sample. `traffic` is the profiled HBM bytes per launch when a PMC summary for the workload is
 committed under profiles/. `cpu_baseline` times the CPU oracle on a fixed sample of the same workload (items spread
 over the generation) on all host cores, and a smaller sample on one thread; the sampled items' fitness,
 placements and counters must equal the GPU's.
@@ -214,6 +215,8 @@ def cpu_baseline(model_kind: int, w, num_models: int, seeds, cap, gpu_fit, gpu_p
     repeats = {"scored_by_the_reference_search": legal,
                "in_repeated_lists": sum(c.repeat_legal for _, _, c in res),
                "share": sum(c.repeat_legal for _, _, c in res) / max(legal, 1),
+               "taken_from_the_previous_move": sum(c.reuse_legal for _, _, c in res),
+               "share_taken": sum(c.reuse_legal for _, _, c in res) / max(legal, 1),
                "sample": f"the {CPU_SAMPLE} sampled items, counted by the oracle"}
     return {"value": placements / dt, "unit": "placements/s", "cores": cores, "kind": "port",
             "sample": f"{CPU_SAMPLE} (model, game) items evenly spaced over the generation (item k * {items} / "

@@ -128,7 +128,9 @@ int ta3h_fetch(ta3h_evaluator* ev, float* out_fitness, uint32_t* out_placements)
  * an earlier list's: the same candidates with the same values) and, after a placement that cleared no line,
  * takes the root lists from the previous move's level-1 scores when they match (V11 and 16-32-32-1: any kept
  * root; the wide network: the six best; not in pruned searches that prune level 1, nor in trace and move-log
- * launches, which score every candidate): the reference's search fits and scores ~21% more candidates. */
+ * launches, which score every candidate); the full-beam 16-32-32-1 search then also takes the level-1 and
+ * level-2 lists that the previous move scored along the committed root's kept nodes (bench.py's 16-32-32-1
+ * generation: the reference's search fits 3,476 candidates per placement, this one 2,378). */
 int ta3h_fetch_counters(ta3h_evaluator* ev, uint64_t* out_counters);
 
 /* Optional per-item trace (no reference counterpart): when enabled before ta3h_stage, each launch records
