@@ -588,6 +588,66 @@ __device__ inline float value_f16(wptr w, float const (&f)[16], bool sparse) {
     return out;
 }
 
+// Two chunks of candidates through the network at once (fa / fb: the features of this lane's candidates of the
+// two chunks; la / lb: whether they are legal, the lanes whose values are used): every weight is loaded into an
+// SGPR once for both, which halves the scalar loads and the scalar-cache misses per candidate. Each candidate's
+// operations are value_f16's in the same order; the exact skipping considers the legal lanes of the chunk it
+// skips for (a unit or feature skipped for one chunk may still be computed for the other).
+template<uint32_t SKIP, bool SPARSE>
+__device__ inline void f16_hidden2(wptr w, float const (&fa)[16], float const (&fb)[16], bool la, bool lb,
+                                   float (&accA)[32], float (&accB)[32]) {
+#pragma unroll 2
+    for(int o = 0; o < 32; ++o) {
+        float va = w[F16_B1 + o], vb = va;
+#pragma unroll
+        for(int i = 0; i < 16; ++i)
+            if(((SKIP >> i) & 1u) == 0) {
+                float const wi = w[F16_W1 + o * 16 + i];
+                va = fma_rn(va, fa[i], wi);
+                vb = fma_rn(vb, fb[i], wi);
+            }
+        float const ha = relu(va), hb = relu(vb);
+        bool const doA = !SPARSE || __ballot(la && ha != 0.0f) != 0;
+        bool const doB = !SPARSE || __ballot(lb && hb != 0.0f) != 0;
+        if(doA) {
+#pragma unroll
+            for(int j = 0; j < 32; ++j) accA[j] = fma_rn(accA[j], ha, w[F16_W2T + o * 32 + j]);
+        }
+        if(doB) {
+#pragma unroll
+            for(int j = 0; j < 32; ++j) accB[j] = fma_rn(accB[j], hb, w[F16_W2T + o * 32 + j]);
+        }
+    }
+}
+
+__device__ inline void value_f16x2(wptr w, float const (&fa)[16], float const (&fb)[16], bool la, bool lb,
+                                   bool sparse, float& outA, float& outB) {
+    float accA[32], accB[32];
+#pragma unroll
+    for(int j = 0; j < 32; ++j) accA[j] = accB[j] = w[F16_B2 + j];
+    if(!sparse) {
+        f16_hidden2<0u, false>(w, fa, fb, la, lb, accA, accB);
+    } else {
+        bool rare = false, often = false;
+#pragma unroll
+        for(int i = 0; i < 16; ++i) {
+            if((RARE_FEATURES >> i) & 1u) rare |= (la && fa[i] != 0.0f) || (lb && fb[i] != 0.0f);
+            if((F16_OFTEN_ZERO >> i) & 1u) often |= (la && fa[i] != 0.0f) || (lb && fb[i] != 0.0f);
+        }
+        if(__ballot(often) == 0) f16_hidden2<F16_OFTEN_ZERO, true>(w, fa, fb, la, lb, accA, accB);
+        else if(__ballot(rare) == 0) f16_hidden2<RARE_FEATURES, true>(w, fa, fb, la, lb, accA, accB);
+        else f16_hidden2<0u, true>(w, fa, fb, la, lb, accA, accB);
+    }
+    outA = w[F16_B3];
+    outB = outA;
+#pragma unroll
+    for(int j = 0; j < 32; ++j) {
+        float const w3 = w[F16_W3 + j];
+        outA = fma_rn(outA, relu(accA[j]), w3);
+        outB = fma_rn(outB, relu(accB[j]), w3);
+    }
+}
+
 // x / c for an integer-valued x: reciprocal multiply plus one FMA correction. Equals the IEEE quotient for
 // every numerator the features can take (checked exhaustively on the device by the feature-norm kernel).
 __device__ inline float div_const(float x, float c, float r) {
@@ -1237,7 +1297,69 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
     uint64_t acc = acc0; // ROOT/MID: running top-K (acc0: keys merged before); LEAF: per-lane best key
     uint64_t ms = 0;
     constexpr bool REGF = KIND == MODEL_F32W && LDSW; // wide features stay in registers (no feature tile)
-    for(uint32_t base = wv * 64; base < total; base += 64 * G) {
+    // a scored candidate f (this lane's) of list s, placement i: counts, kept score, key into the wave's result
+    auto consume = [&](uint32_t f, bool legal, uint32_t score, uint32_t s, uint32_t i) {
+        nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
+        uint32_t const slot = s * 34 + i;
+        if(keep != nullptr && f < total && slot < keepSlots) // scores kept for the next move (its roots, DeepLds)
+            keep[slot] = legal ? score : 0u;
+        if constexpr(STAGE == STAGE_LEAF) {
+            uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
+            acc = key > acc ? key : acc;
+            if constexpr(TRACE) {
+                if(legal) atomicMax(reinterpret_cast<unsigned long long*>(&wide.segBest[s]), static_cast<unsigned long long>(key));
+            }
+            ms |= legal ? (1ull << s) : 0ull;
+        } else {
+            if(STAGE == STAGE_MID) ms |= legal ? (1ull << s) : 0ull;
+            uint64_t const key = (legal && score != 0) ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0;
+            acc = topk_merge<K>(acc, key);
+        }
+    };
+    // the plain 4-wave 16-32-32-1 kernel takes two of the wave's chunks (c and c + G) through the network at once
+    constexpr bool PAIR = KIND == MODEL_F16 && LDSW;
+    if constexpr(PAIR) {
+        // this lane's candidate f: its list and placement, whether it fits, and its features when it does
+        auto features_of = [&](uint32_t f, bool& legal, uint32_t& s, uint32_t& i, float (&ft)[16]) {
+            if(f >= total) return;
+            uint32_t const si = cand_of(L, seg, f);
+            s = si >> 6;
+            i = si & 63u;
+            uint32_t const info = L.segInfo[s];
+            uint32_t const piece = (info >> 8) & 7u;
+            uint32_t const* parent = &L.boards[(info & 0x7Fu) * WIDTH];
+            LutEntry const e = c_tab.lut[c_tab.lutOff[piece] + i];
+            int drop = 0;
+            legal = fit_span(parent, e.cols, e.col0, drop);
+            if(legal) {
+                uint32_t b[WIDTH];
+                int const cleared = place_regs(parent, e.cols, e.col0, drop, b);
+                uint32_t const clears = hist_added(STAGE == STAGE_ROOT ? 0u : L.clears[info & 0x7Fu], cleared);
+                leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, nullptr);
+            }
+        };
+        for(uint32_t base = wv * 64; base < total; base += 128 * G) {
+            wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per pair of chunks
+            uint32_t const baseB = base + 64 * G;
+            bool const hasB = baseB < total;
+            float fa[16] = {}, fb[16] = {};
+            bool la = false, lb = false;
+            uint32_t sa = 0, ia = 0, sb = 0, ib = 0, scoreA = 0, scoreB = 0;
+            features_of(base + lane, la, sa, ia, fa);
+            if(hasB) {
+                features_of(baseB + lane, lb, sb, ib, fb);
+                float va, vb;
+                value_f16x2(wl, fa, fb, la, lb, wide.finite, va, vb);
+                scoreA = la ? ordered_bits(va) : 0u;
+                scoreB = lb ? ordered_bits(vb) : 0u;
+            } else if(la) {
+                scoreA = ordered_bits(value_f16(wl, fa, wide.finite));
+            }
+            consume(base + lane, la, scoreA, sa, ia);
+            if(hasB) consume(baseB + lane, lb, scoreB, sb, ib);
+        }
+    }
+    for(uint32_t base = wv * 64; !PAIR && base < total; base += 64 * G) {
         wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per chunk
         uint32_t const f = base + lane;
         bool legal = false;
@@ -1292,22 +1414,7 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             if(legal) score = ordered_bits(v);
             wave_sync(); // the tile is rewritten by the next chunk
         }
-        nLegal += static_cast<uint32_t>(__popcll(__ballot(legal)));
-        uint32_t const slot = s * 34 + i;
-        if(keep != nullptr && f < total && slot < keepSlots) // scores kept for the next move (its roots, DeepLds)
-            keep[slot] = legal ? score : 0u;
-        if constexpr(STAGE == STAGE_LEAF) {
-            uint64_t const key = legal ? ((static_cast<uint64_t>(score) << 32) | (SPAN - slot)) : 0;
-            acc = key > acc ? key : acc;
-            if constexpr(TRACE) {
-                if(legal) atomicMax(reinterpret_cast<unsigned long long*>(&wide.segBest[s]), static_cast<unsigned long long>(key));
-            }
-            ms |= legal ? (1ull << s) : 0ull;
-        } else {
-            if(STAGE == STAGE_MID) ms |= legal ? (1ull << s) : 0ull;
-            uint64_t const key = (legal && score != 0) ? ((static_cast<uint64_t>(score) << 32) | (0xFFFFFFFFu - slot)) : 0;
-            acc = topk_merge<K>(acc, key);
-        }
+        consume(f, legal, score, s, i);
     }
     if(STAGE != STAGE_ROOT) *mask = wave_or_u64(ms);
     return acc;
