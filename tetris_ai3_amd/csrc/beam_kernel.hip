@@ -59,6 +59,9 @@ __device__ inline void wave_sync() {
 }
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t uniform64(uint64_t v) {
+    return static_cast<uint64_t>(uniform(static_cast<uint32_t>(v))) | static_cast<uint64_t>(uniform(static_cast<uint32_t>(v >> 32))) << 32;
+}
 
 __device__ inline uint32_t shfl_xor_u32(uint32_t v, int m) { return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m)); }
 __device__ inline uint32_t shfl_u32(uint32_t v, int src) { return static_cast<uint32_t>(__shfl(static_cast<int>(v), src)); }
@@ -1926,7 +1929,7 @@ __device__ inline void ring_push(uint32_t* ring, uint32_t cap, uint32_t item) {
 // with these keys merged after the scored chunks.
 template<int K, int G>
 __device__ inline uint64_t deep_topk(uint16_t const* cand, DeepLds const& D, uint32_t const* prev, uint32_t total,
-                                     uint32_t* keep, uint32_t keepSlots, uint32_t wv, uint64_t& mask) {
+                                     uint32_t* keep, uint32_t keepSlots, uint32_t wv) {
     uint32_t const rt = uniform(D.reusedTotal);
     uint32_t const lane = lane_id();
     uint64_t acc = 0;
@@ -1942,7 +1945,6 @@ __device__ inline uint64_t deep_topk(uint16_t const* cand, DeepLds const& D, uin
         }
         acc = topk_merge<K>(acc, key);
     }
-    if(wv == 0) mask |= D.reusedLegal;
     return acc;
 }
 
@@ -2070,13 +2072,14 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
         }
         mask = 0;
         // the lists taken from the previous move (their scores: its level 2 / its leaves) seed the top-K
-        uint64_t first = 0, reusedMask = 0;
+        uint64_t first = 0;
         if constexpr(DEEP)
-            first = level == 1 ? deep_topk<K1, G>(cand, *D, D->s2, total, keep, keepSlots, wv, reusedMask)
-                               : deep_topk<K2, G>(cand, *D, D->sL, total, keep, keepSlots, wv, reusedMask);
+            first = level == 1 ? deep_topk<K1, G>(cand, *D, D->s2, total, keep, keepSlots, wv)
+                               : deep_topk<K2, G>(cand, *D, D->sL, total, keep, keepSlots, wv);
         top = level == 1 ? run_stage<KIND, STAGE_MID, K1, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep, keepSlots, first)
                          : run_stage<KIND, STAGE_MID, K2, G, false, CB, MODE == MODE_PLAIN && G == 4>(L, seg, evalTotal, w, wv, col, &mask, nLegal, wide, keep, keepSlots, first);
-        mask |= reusedMask;
+        if constexpr(DEEP) // the taken lists' legality (read here: no register holds it across the scoring)
+            if(wv == 0) mask |= D->reusedLegal;
         nSlots += (evalTotal + 63u) & ~63u;
         top = combine_parts<G, K2, true>(L, top, mask, wv);
         if constexpr(DEDUP) {
@@ -2095,7 +2098,7 @@ __device__ bool play_move(EvalArgs const& a, GameLds& L, uint16_t* cand, PruneLd
             fb = (static_cast<uint64_t>(L.score[cur * 32 + lane]) << 32) | (SPAN - (LEAF_ORDERS + rank));
         }
         fb = wave_max_u64(fb);
-        bestKey = fb > bestKey ? fb : bestKey;
+        bestKey = uniform64(fb > bestKey ? fb : bestKey); // wave-uniform: kept in SGPRs
         uint32_t const next = __popcll(__ballot(top != 0));
         if constexpr(MODE == MODE_TRACE) {
             if(traceMove) {
