@@ -113,6 +113,7 @@ def lib():
             "ta3o_eval_host_mt": (None, [C.c_int, f32p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, C.c_int,
                                          f32p, u32p, C.POINTER(Counters)]),
             "ta3o_play_trace": (C.c_int, [C.c_int, f32p, C.c_uint64, C.c_uint32, u8p, u32p, f32p]),
+            "ta3o_check_deep_reuse": (C.c_int, [C.c_int, f32p, C.c_uint64, C.c_uint32, u64p]),
             "ta3o_pruner_num_params": (C.c_int, []),
             "ta3o_pruner_score": (C.c_float, [f32p, C.c_float, u32p, C.c_int, C.c_int, C.c_int, C.c_int]),
             "ta3o_pruner_scores": (None, [f32p, C.c_float, u32p, C.c_int, f32p]),
@@ -411,6 +412,16 @@ def eval_host_mt(kind: int, w, num_models: int, seeds, max_moves: int, threads: 
     lib().ta3o_eval_host_mt(kind, _p(w, C.c_float), num_models, _p(s, C.c_uint64), len(s), max_moves, threads,
                             _p(out, C.c_float), _p(pl, C.c_uint32), C.byref(cnt))
     return out, pl
+
+
+def check_deep_reuse(kind: int, w, seed: int, max_moves: int) -> dict:
+    """one game with the content check of the lists of levels 1 and 2 the search counts as taken from the previous
+    move (ta3_oracle.cpp DeepCheck): lists counted, those whose content no list of the previous search one level
+    down had, moves, reuse_applies"""
+    w = _f32(w if len(w) else np.zeros(1))
+    out = np.zeros(4, dtype=np.uint64)
+    lib().ta3o_check_deep_reuse(kind, _p(w, C.c_float), seed, max_moves, _p(out, C.c_uint64))
+    return {"counted": int(out[0]), "mismatched": int(out[1]), "moves": int(out[2]), "reuse_applies": int(out[3])}
 
 
 def play_trace(kind: int, w, seed: int, max_moves: int):

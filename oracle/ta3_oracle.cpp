@@ -1054,6 +1054,13 @@ struct ListSeen {
         k.w[WIDTH + 1] = piece | (heldIsI ? 8u : 0u);
         return !keys.insert(k).second;
     }
+    bool has(Board const& b, uint32_t clears, uint32_t piece, bool heldIsI) const {
+        Key k;
+        std::memcpy(k.w, b.c, sizeof(b.c));
+        k.w[WIDTH] = clears;
+        k.w[WIDTH + 1] = piece | (heldIsI ? 8u : 0u);
+        return keys.count(k) != 0;
+    }
 };
 
 // Roots scored by the previous move. When the committed placement cleared no line, the next move's root lists
@@ -1084,6 +1091,15 @@ struct DeepReuse {
     int32_t id1[64] = {};               // this search's level-1 list -> the previous search's level-2 list, or -1
 };
 
+// Content check of DeepReuse (tests only, `checkDeep`): every list it counts must have the content (board, clear
+// histogram, piece, held-is-I) of a list the previous search built one level down (its level-2 lists for this
+// level 1, its leaf lists for this level 2), which the structural identities above are meant to guarantee.
+struct DeepCheck {
+    bool on = false;
+    ListSeen prev2, prevL, cur2, curL;
+    uint64_t counted = 0, mismatched = 0;
+};
+
 struct BeamWorkspace {
     std::vector<uint32_t> rootScores, midScores, topSlots, topScores, topRootSlots, topRootScores;
     std::vector<uint8_t> hasChild;
@@ -1091,6 +1107,7 @@ struct BeamWorkspace {
     ListSeen seen;
     RootReuse reuse;
     DeepReuse deep;
+    DeepCheck check;
 };
 
 // the searches whose device kernels keep level-2 and leaf scores for the next move (DeepReuse): the full-beam
@@ -1138,6 +1155,12 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
     // DeepReuse: this search takes lists of the previous one (its state is written by every search of such a kind)
     bool const deepOn = deep_reuse_on(model, pr, depth) && maxChildMid == 2 && maxLast == 2;
     bool const deepUse = deepOn && reuseRoots;
+    if(ws.check.on) { // the previous search's level-2 and leaf lists
+        std::swap(ws.check.prev2.keys, ws.check.cur2.keys);
+        std::swap(ws.check.prevL.keys, ws.check.curL.keys);
+        ws.check.cur2.clear();
+        ws.check.curL.clear();
+    }
     std::fill(std::begin(ws.reuse.level1), std::end(ws.reuse.level1), 0u);
     ws.rootScores.assign(static_cast<size_t>(TREES.maxRootGroups[depth]) * MP, 0u);
     for(size_t gi = 0; gi < tree.layers[0].size(); ++gi) {
@@ -1238,6 +1261,14 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 if(deepOn && level == 1) D.id1[li] = prevList;
                 if(deepOn && newKeys != nullptr) newKeys[li] = lkey;
                 bool const deepHit = prevList >= 0 && !rep;
+                if(ws.check.on) {
+                    if(deepHit) {
+                        ++ws.check.counted;
+                        ListSeen const& prevLists = level == 1 ? ws.check.prev2 : ws.check.prevL;
+                        ws.check.mismatched += prevLists.has(F.board[k], F.clears[k], piece, hI) ? 0u : 1u;
+                    }
+                    if(level == 2) ws.check.cur2.repeat(F.board[k], F.clears[k], piece, hI);
+                }
                 if(level == 1 && k < 32 && ci - pn.childBegin < 2)
                     ws.reuse.level1[2 * k + (ci - pn.childBegin)] = piece | (hI ? 8u : 0u) | 16u;
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
@@ -1329,6 +1360,7 @@ SearchResult search_beam(Engine const& g, ModelRef const& model, std::vector<uin
                 int const n = LUT.count(piece);
                 bool const rep = ws.seen.repeat(F.board[j], F.clears[j], piece, hI);
                 if(deepOn) ws.deep.keyL[2 * j + (ci - ln.childBegin)] = piece | (hI ? 8u : 0u) | 16u;
+                if(ws.check.on) ws.check.curL.repeat(F.board[j], F.clears[j], piece, hI);
                 uint8_t legal[MAX_PLACEMENTS], keep[MAX_PLACEMENTS];
                 for(int i = 0; i < n; ++i) {
                     legal[i] = fit(F.board[j], LUT.nth(piece, i).shape).legal ? 1 : 0;
@@ -1510,6 +1542,8 @@ float play_game(ModelRef const& model, uint64_t seed, uint32_t maxMoves, BeamWor
     Stats stats;
     int rec = 0;
     ws.reuse.valid = false; // the first move has no previous search
+    ws.check.cur2.clear();
+    ws.check.curL.clear();
     for(uint32_t i = 0; i < maxMoves && !g.gameOver; ++i) {
         TraceOut tr;
         tr.out = traceOut;
@@ -1835,6 +1869,19 @@ void ta3o_eval_host_mt(int kind, const float* w, uint32_t num_models, const uint
     Counters total;
     for(auto const& c : perThread) total.add(c);
     to_counters(total, counters);
+}
+
+int ta3o_check_deep_reuse(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint64_t* out) {
+    ModelRef const m{kind, w};
+    BeamWorkspace ws;
+    ws.check.on = true;
+    Counters c;
+    play_game(m, seed, max_moves, ws, c);
+    out[0] = ws.check.counted;
+    out[1] = ws.check.mismatched;
+    out[2] = c.moves;
+    out[3] = c.reuseApplies;
+    return 0;
 }
 
 int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint8_t* moves, uint32_t* boards,

@@ -144,6 +144,11 @@ void ta3o_eval_host_mt(int kind, const float* w, uint32_t num_models, const uint
 int ta3o_play_trace(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint8_t* moves,
                     uint32_t* boards, float* fitness);
 
+/* one game with the content check of the lists of levels 1 and 2 counted as taken from the previous move:
+ * out[0] = lists counted, out[1] = those whose content no list of the previous search one level down had,
+ * out[2] = moves, out[3] = reuse_applies */
+int ta3o_check_deep_reuse(int kind, const float* w, uint64_t seed, uint32_t max_moves, uint64_t* out);
+
 /* placement-proposal network (build-defined): 265 inputs -> 64 -> 64 -> 1, 21,249 parameters */
 int ta3o_pruner_num_params(void);
 /* inference logit of the integer-grid network (quantized from w with the first hidden layer's calibrated

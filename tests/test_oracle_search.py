@@ -402,3 +402,21 @@ def test_dead_ends_full_width_depth3_equal_plan_tree_walk(linear):
     assert total.dead_mid > 0 and total.fallback_mid_best >= 2
     assert total.dead_leaf > 0 and total.fallback_leaf_best >= 2
     assert differ >= 1  # the reference's per-node rule is visible on these boards
+
+
+def test_lists_taken_from_the_previous_move_have_its_content():
+    """The 16-32-32-1 search counts level-1 and level-2 lists as taken from the previous move by structural
+    identities (the committed root's kept level-1 node, its level-2 lists with the same piece and held-is-I; one
+    level deeper likewise). Independently of those identities, every list counted must have the content (board,
+    clear histogram, piece, held-is-I) of a list the previous search built one level down; a wrong child, rank
+    or slot offset in the identities breaks this (checked by planting each in a copy of the oracle)."""
+    rng = np.random.default_rng(11)
+    kind = 1  # the 16-32-32-1 network (TA3O_MODEL_F16)
+    counted = 0
+    for k in range(3):
+        w = rng.normal(0.0, 0.5, O.num_params(kind)).astype(np.float32)
+        for seed in (5, 77):
+            r = O.check_deep_reuse(kind, w, seed, 150)
+            assert r["mismatched"] == 0, r
+            counted += r["counted"]
+    assert counted > 1000  # the games are long enough to take many lists
