@@ -273,7 +273,7 @@ def main():
         ev.set_launch(args.waves_per_game, True, 0)
         ev.stage(w, M, seeds, cap)
     else:  # launch shape and first-phase budget tuned on this batch (untimed; results never depend on them)
-        tuned = T.autotune(ev, w, M, seeds, cap, candidates=((2, 0), (4, 0)), phases=(None, 256), use_cache=False)
+        tuned = T.autotune(ev, w, M, seeds, cap, candidates=((2, 0), (4, 0)), phases=(None, 32), use_cache=False)
 
     for _ in range(args.warmup):
         ev.launch()

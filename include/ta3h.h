@@ -59,9 +59,10 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind);
  * the first phase plays every game for at most first_phase_moves moves and parks the unfinished ones on the
  * device (engine state, counters, move index); the second phase plays only those on to max_moves, so the long
  * games all start together instead of trickling out of the work queue; the second phase takes the games of the
- * models with the most parked games first. 0 = one phase. TA3H_PHASES_AUTO (the default) = max_moves / 16
- * clamped to [128, 512]: 128 at a 2,000-piece cap (a full F16 generation of 200 models x 50 games runs about 8%
- * faster than in one phase, V11 about 4%), 312 at 5,000 (2-3% faster than 128 on the wide workload). */
+ * models with the most parked games first. 0 = one phase. TA3H_PHASES_AUTO (the default) = max_moves / 32
+ * clamped to [64, 512]: 64 at a 2,000-piece cap (a full 16-32-32-1 generation of 200 models x 50 games: 439 ms
+ * against 457 ms at 128; V11 285 against 292 ms), 156 at 5,000 (the wide workload: 2,170 against 2,187 ms at
+ * 312). Shorter turns keep the games that run to the cap closer together, so fewer end alone. */
 int ta3h_set_phases(ta3h_evaluator* ev, uint32_t first_phase_moves);
 
 /* Start every game of the following launches with 0-4 garbage rows at the bottom, each full but for one hole,
@@ -104,8 +105,8 @@ int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uin
  * 6,592, wide 42,560, each rounded up to 256 B per section) + the parked record per (model, game) (512 B; 18,592 B
  * for 16-32-32-1, whose record carries the previous move's level-2 and leaf scores) + 4 B per (model, game) + 4 B
  * per model + 8 B + 4 B per (model, game) and turn of the second phase (1 + (max_moves - b) / b turns for the
- * budget b = max_moves / 16 clamped to [128, 512]; 15 at 2,000 moves, 16 at 5,000); a larger batch returns
- * TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1, 2,000 moves: 187.9 MB; 256 x 100 wide, 5,000 moves: 25.7 MB). Calls
+ * budget b = max_moves / 32 clamped to [64, 512]; 31 at 2,000 moves, 32 at 5,000); a larger batch returns
+ * TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1, 2,000 moves: 188.5 MB; 256 x 100 wide, 5,000 moves: 27.4 MB). Calls
  * that overlap on one GPU share that array and race: serialise them (one stream, or events). Returns TA3H_OK,
  * TA3H_E_ARG, TA3H_E_SCRATCH or TA3H_E_HIP (there is no evaluator to hold an error message). */
 int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_models, const uint64_t* d_seeds, uint32_t games,

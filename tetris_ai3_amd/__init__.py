@@ -159,7 +159,7 @@ class ModelEvaluator:
 
     def set_phases(self, first_phase_moves: int | None):
         """two-phase schedule: every game to first_phase_moves, then only the survivors, the games of the models
-        with the most survivors first (0 = one phase; None = max_moves / 16 clamped to [128, 512], the default).
+        with the most survivors first (0 = one phase; None = max_moves / 32 clamped to [64, 512], the default).
         Results never change."""
         self._check(lib().ta3h_set_phases(self._h, PHASES_AUTO if first_phase_moves is None else int(first_phase_moves)))
 
@@ -527,7 +527,7 @@ def _load_cache() -> dict:
 
 
 def autotune(evaluator: "ModelEvaluator", weights, num_models: int, seeds, max_moves: int,
-             candidates=None, phases=(None, 64, 256), repeats: int = 2, use_cache: bool = True):
+             candidates=None, phases=(None, 32, 128), repeats: int = 2, use_cache: bool = True):
     """Picks the launch shape (waves per game, grid) and first-phase budget for this batch shape and keeps it on
     `evaluator` (none of them changes a result). The choice is cached per (network, models x games, max_moves,
     garbage rows, kernel mode) in this process and, when TA3H_AUTOTUNE_CACHE names a JSON file, across processes;

@@ -128,8 +128,8 @@ extern "C" int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_
     if(items == 0) return TA3H_OK;
     if(d_weights == nullptr || d_seeds == nullptr || d_fitness == nullptr) return TA3H_E_ARG;
     if(items > 0xFFFFFFFFull / 2) return TA3H_E_ARG;
-    // the two-phase schedule with the default budget: max_moves / 16 clamped to [128, 512]
-    uint32_t const budget = max_moves / 16 < 128 ? 128u : (max_moves / 16 > 512 ? 512u : max_moves / 16);
+    // the two-phase schedule with the default budget: max_moves / 32 clamped to [64, 512]
+    uint32_t const budget = max_moves / 32 < 64 ? 64u : (max_moves / 32 > 512 ? 512u : max_moves / 32);
     uint64_t const ringSlots = budget < max_moves ? ring_slots(items, max_moves, budget) : 0;
     if(ringSlots > 0xFFFFFFFFull) return TA3H_E_SCRATCH;
     // scratch sections: counters, repacked weights, parked games, resume list, per-model counts and work ring

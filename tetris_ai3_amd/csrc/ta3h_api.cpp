@@ -398,7 +398,7 @@ namespace {
 uint32_t phase_budget(ta3h_evaluator const* ev) {
     uint32_t const phase = ev->phaseMoves != TA3H_PHASES_AUTO
                                ? ev->phaseMoves
-                               : (ev->maxMoves / 16 < 128 ? 128u : (ev->maxMoves / 16 > 512 ? 512u : ev->maxMoves / 16));
+                               : (ev->maxMoves / 32 < 64 ? 64u : (ev->maxMoves / 32 > 512 ? 512u : ev->maxMoves / 32));
     return phase != 0 && ev->persistent && phase < ev->maxMoves ? phase : 0u;
 }
 
