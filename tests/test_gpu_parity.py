@@ -113,7 +113,8 @@ def test_zero_skipping_matches_oracle(kind):
 def test_work_counters():
     """Per-item counters: candidates fitted and legal equal the oracle's less its repeated lists (an earlier list
     of the level with the same parent board, clear histogram, piece and held-is-I), which the device scores once,
-    and less the roots the previous move scored (the committed placement cleared no line); candidate lanes issued
+    and less the roots the previous move scored (the committed placement cleared no line) and, for this network,
+    the lists of levels 1 and 2 the device takes from the previous move's scores; candidate lanes issued
     are whole
     64-lane chunks covering every fitted candidate of a plain beam, with less than one chunk of padding per
     stage (four stages per move)"""
