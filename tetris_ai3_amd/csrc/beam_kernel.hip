@@ -651,6 +651,92 @@ __device__ inline void value_f16x2(wptr w, float const (&fa)[16], float const (&
     }
 }
 
+// 4 x 4 transpose of (register, 16-lane row) blocks: t[b] in row a = r_a in row b. v_permlane32_swap exchanges the
+// off-diagonal 2 x 2 blocks of rows, v_permlane16_swap the single rows inside them (the whole wave takes part).
+__device__ inline void transpose_rows(uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, uint32_t (&t)[4]) {
+    auto const s02 = __builtin_amdgcn_permlane32_swap(r0, r2, false, false);
+    auto const s13 = __builtin_amdgcn_permlane32_swap(r1, r3, false, false);
+    auto const s01 = __builtin_amdgcn_permlane16_swap(s02[0], s13[0], false, false);
+    auto const s23 = __builtin_amdgcn_permlane16_swap(s02[1], s13[1], false, false);
+    t[0] = s01[0];
+    t[1] = s01[1];
+    t[2] = s23[0];
+    t[3] = s23[1];
+}
+
+// The 16-32-32-1 network's value for the 64 candidates of a chunk (fin: this lane's features; the whole wave takes
+// part), its two hidden layers on v_mfma_f32_16x16x4_f32 (rows = units, columns = 16 candidates, K = 4 inputs per
+// instruction). That instruction adds its four products to the accumulator one after another, each rounded once
+// (as fmaf, in k order; checked on the device over normal, subnormal, zero, infinite and tie-rounding operands), so
+// chaining the K blocks in input order computes exactly value_f16's chains (v = fma(x_i, w_i, v) for i ascending
+// from the bias), with no step skipped. frag: the model's operands (f16f); the output layer (W3, b3) runs on the
+// VALU from SGPRs as in value_f16, after the outputs are moved back to one candidate per lane.
+__device__ inline float value_f16_mfma(float const* __restrict__ frag, wptr w, float const (&fin)[16]) {
+    uint32_t const lane = lane_id();
+    auto const g = opaque_global(frag);
+    // B operands of layer 1: for inputs 4 s .. 4 s + 3, lane 16 k + n holds input 4 s + k of candidate 16 ct + n
+    uint32_t b1[4][4];
+#pragma unroll
+    for(int s = 0; s < 4; ++s)
+        transpose_rows(__float_as_uint(fin[4 * s]), __float_as_uint(fin[4 * s + 1]), __float_as_uint(fin[4 * s + 2]),
+                       __float_as_uint(fin[4 * s + 3]), b1[s]);
+    // layer 1: acc1[rt][ct] register j of lane 16 q + n = unit 16 rt + 4 j + q of candidate 16 ct + n
+    f32x4 acc1[2][4];
+#pragma unroll
+    for(int rt = 0; rt < 2; ++rt) {
+        f32x4 const c = *reinterpret_cast<f32x4 const __attribute__((address_space(1)))*>(&g[f16f::C1 + (rt * 64 + lane) * 4]);
+#pragma unroll
+        for(int ct = 0; ct < 4; ++ct) acc1[rt][ct] = c;
+#pragma unroll
+        for(int s = 0; s < 4; ++s) {
+            float const a = g[f16f::A1 + (rt * 4 + s) * 64 + lane];
+#pragma unroll
+            for(int ct = 0; ct < 4; ++ct)
+                acc1[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, __uint_as_float(b1[s][ct]), acc1[rt][ct], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for(int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for(int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for(int j = 0; j < 4; ++j) acc1[rt][ct][j] = relu(acc1[rt][ct][j]);
+    // layer 2: register s of acc1[rt1][ct] is the B operand of inputs 16 rt1 + 4 s .. + 3 (lane 16 k + n: input
+    // 16 rt1 + 4 s + k); acc2[rt2][ct] register i of lane 16 q + n = output 16 rt2 + 4 q + i of candidate 16 ct + n
+    f32x4 acc2[2][4];
+#pragma unroll
+    for(int rt2 = 0; rt2 < 2; ++rt2) {
+        f32x4 const c = *reinterpret_cast<f32x4 const __attribute__((address_space(1)))*>(&g[f16f::C2 + (rt2 * 64 + lane) * 4]);
+#pragma unroll
+        for(int ct = 0; ct < 4; ++ct) acc2[rt2][ct] = c;
+#pragma unroll
+        for(int rt1 = 0; rt1 < 2; ++rt1)
+#pragma unroll
+            for(int s = 0; s < 4; ++s) {
+                float const a = g[f16f::A2 + ((rt2 * 2 + rt1) * 4 + s) * 64 + lane];
+#pragma unroll
+                for(int ct = 0; ct < 4; ++ct)
+                    acc2[rt2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, acc1[rt1][ct][s], acc2[rt2][ct], 0, 0, 0);
+            }
+    }
+    // outputs back to one candidate per lane (output 16 rt2 + 4 b + i in o[rt2][i][b]), then the output chain
+    float out = w[F16_B3];
+#pragma unroll
+    for(int rt2 = 0; rt2 < 2; ++rt2) {
+        uint32_t o[4][4];
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+            transpose_rows(__float_as_uint(acc2[rt2][0][i]), __float_as_uint(acc2[rt2][1][i]), __float_as_uint(acc2[rt2][2][i]),
+                           __float_as_uint(acc2[rt2][3][i]), o[i]);
+#pragma unroll
+        for(int b = 0; b < 4; ++b)
+#pragma unroll
+            for(int i = 0; i < 4; ++i)
+                out = fma_rn(out, relu(__uint_as_float(o[i][b])), w[F16_W3 + 16 * rt2 + 4 * b + i]);
+    }
+    return out;
+}
+
 // x / c for an integer-valued x: reciprocal multiply plus one FMA correction. Equals the IEEE quotient for
 // every numerator the features can take (checked exhaustively on the device by the feature-norm kernel).
 __device__ inline float div_const(float x, float c, float r) {
@@ -1319,7 +1405,8 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
             acc = topk_merge<K>(acc, key);
         }
     };
-    // the plain 4-wave 16-32-32-1 kernel takes two of the wave's chunks (c and c + G) through the network at once
+    // the plain 4-wave 16-32-32-1 kernel: a model with finite weights runs its hidden layers on MFMA, one chunk at a
+    // time; otherwise two of the wave's chunks (c and c + G) go through the VALU network at once
     constexpr bool PAIR = KIND == MODEL_F16 && LDSW;
     if constexpr(PAIR) {
         // this lane's candidate f: its list and placement, whether it fits, and its features when it does
@@ -1341,7 +1428,18 @@ __device__ inline uint64_t run_stage(GameLds& L, uint16_t const* seg, uint32_t t
                 leaf_features<KIND, 1>(b, clears, ((info >> 11) & 1u) != 0, ft, nullptr);
             }
         };
-        for(uint32_t base = wv * 64; base < total; base += 128 * G) {
+        if(wide.finite) {
+            for(uint32_t base = wv * 64; base < total; base += 64 * G) {
+                wptr const wl = opaque(w);
+                float ft[16] = {};
+                bool legal = false;
+                uint32_t s = 0, i = 0;
+                features_of(base + lane, legal, s, i, ft);
+                float const v = value_f16_mfma(reinterpret_cast<float const*>(wide.wb), wl, ft);
+                consume(base + lane, legal, legal ? ordered_bits(v) : 0u, s, i);
+            }
+        }
+        for(uint32_t base = wv * 64; !wide.finite && base < total; base += 128 * G) {
             wptr const wl = opaque(w); // wave-uniform point of the loop: reload weights per pair of chunks
             uint32_t const baseB = base + 64 * G;
             bool const hasB = baseB < total;

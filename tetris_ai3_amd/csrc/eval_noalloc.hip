@@ -77,6 +77,22 @@ __global__ void repack_kernel(int kind, float const* __restrict__ src, uint32_t 
     uint32_t v = 0;
     if(kind == MODEL_F32W) {
         v = wide_word(w, j);
+    } else if(kind == MODEL_F16 && j >= f16f::DENSE) { // the MFMA operands of the hidden layers (f16f)
+        uint32_t const lane = j < f16f::C1 ? (j - f16f::DENSE) % 64 : ((j - f16f::C1) % 256) / 4;
+        uint32_t const r = lane % 16, k = lane / 16, q = lane / 16;
+        if(j < f16f::A2) {
+            uint32_t const g = (j - f16f::A1) / 64, rt = g / 4, s = g % 4;
+            v = __float_as_uint(w[(16 * rt + 4 * (r % 4) + r / 4) * 16 + 4 * s + k]);
+        } else if(j < f16f::C1) {
+            uint32_t const g = (j - f16f::A2) / 64, rt2 = g / 8, rt1 = (g / 4) % 2, s = g % 4;
+            v = __float_as_uint(w[544 + (16 * rt2 + r) * 32 + 16 * rt1 + 4 * s + k]);
+        } else if(j < f16f::C2) {
+            uint32_t const t = (j - f16f::C1) / 256, jj = (j - f16f::C1) % 4;
+            v = __float_as_uint(w[512 + 16 * t + 4 * jj + q]);
+        } else {
+            uint32_t const t = (j - f16f::C2) / 256, i = (j - f16f::C2) % 4;
+            v = __float_as_uint(w[1568 + 16 * t + 4 * q + i]);
+        }
     } else if(kind == MODEL_F16) { // W2 [32 out][32 in] -> W2T [in][out]; the rest unchanged
         if(j >= 544 && j < 1568) {
             uint32_t const in = (j - 544) / 32, out = (j - 544) % 32;
@@ -92,7 +108,7 @@ __global__ void repack_kernel(int kind, float const* __restrict__ src, uint32_t 
 
 uint32_t params_of(int kind) { return kind == MODEL_V11 ? 97u : kind == MODEL_F16 ? 1633u : 20865u; }
 uint32_t stride_of(int kind) {
-    uint32_t const n = kind == MODEL_F32W ? WIDE_BYTES / 4 : params_of(kind);
+    uint32_t const n = kind == MODEL_F32W ? WIDE_BYTES / 4 : kind == MODEL_F16 ? f16f::FLOATS : params_of(kind);
     return (n + 15u) & ~15u;
 }
 size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }

@@ -23,6 +23,17 @@ namespace f32w {
 inline constexpr uint32_t W1F = 0, W2F = 8192, B1 = 40960, B2 = 41472, W3 = 41984, B3 = 42496, BYTES = 42512;
 }
 
+// Device layout of the 16-32-32-1 network, per model (floats): the dense part (W1[32][16], b1, W2T[32 in][32 out],
+// b2, W3, b3 at 0 / 512 / 544 / 1568 / 1600 / 1632, padded to 1,648), then the operands of its two hidden layers on
+// v_mfma_f32_16x16x4_f32, lane-major (lane 16 k + r of a wave reads word [lane]): with row r of a 16-unit layer-1
+// tile standing for unit 4 (r % 4) + r / 4, the layer-1 outputs land in the layout layer 2 takes as its B operand.
+//   a1[rt][s][lane r + 16 k] = W1[16 rt + 4 (r % 4) + r / 4][4 s + k]
+//   a2[rt2][rt1][s][lane r + 16 k] = W2[16 rt2 + r][16 rt1 + 4 s + k]
+//   c1[rt][lane n + 16 q][j] = b1[16 rt + 4 j + q]; c2[rt2][lane n + 16 q][i] = b2[16 rt2 + 4 q + i]
+namespace f16f {
+inline constexpr uint32_t DENSE = 1648, A1 = DENSE, A2 = A1 + 512, C1 = A2 + 1024, C2 = C1 + 512, FLOATS = C2 + 512;
+}
+
 inline constexpr uint32_t PARK_WORDS = 128; // parked game: engine state, counters, move index, root scores (512 B)
 // the 16-32-32-1 network's plain kernels also carry the previous move's level-2 and leaf scores, list infos,
 // legality and kept slots (the first words of DeepLds in the kernel source)

@@ -35,8 +35,35 @@ namespace wide = ta3h::f32w; // device layout of the wide network (bytes)
 
 // floats per model in device memory (a multiple of 16 so every model starts 64-byte aligned)
 uint32_t stride_of(int kind) {
-    uint32_t const n = kind == TA3H_MODEL_F32W ? wide::BYTES / 4 : static_cast<uint32_t>(params_of(kind));
+    uint32_t const n = kind == TA3H_MODEL_F32W ? wide::BYTES / 4
+                       : kind == TA3H_MODEL_F16 ? ta3h::f16f::FLOATS
+                                                : static_cast<uint32_t>(params_of(kind));
     return (n + 15u) & ~15u;
+}
+
+// 16-32-32-1 network, host layout -> the MFMA operands after its dense device layout (ta3h::f16f)
+void repack_f16_fragments(float const* src, float* dst) {
+    namespace F = ta3h::f16f;
+    for(int rt = 0; rt < 2; ++rt)
+        for(int s = 0; s < 4; ++s)
+            for(int l = 0; l < 64; ++l) {
+                int const r = l % 16, k = l / 16;
+                dst[F::A1 + (rt * 4 + s) * 64 + l] = src[(16 * rt + 4 * (r % 4) + r / 4) * 16 + 4 * s + k];
+            }
+    for(int rt2 = 0; rt2 < 2; ++rt2)
+        for(int rt1 = 0; rt1 < 2; ++rt1)
+            for(int s = 0; s < 4; ++s)
+                for(int l = 0; l < 64; ++l) {
+                    int const r = l % 16, k = l / 16;
+                    dst[F::A2 + ((rt2 * 2 + rt1) * 4 + s) * 64 + l] = src[544 + (16 * rt2 + r) * 32 + 16 * rt1 + 4 * s + k];
+                }
+    for(int t = 0; t < 2; ++t)
+        for(int l = 0; l < 64; ++l)
+            for(int j = 0; j < 4; ++j) {
+                int const q = l / 16;
+                dst[F::C1 + (t * 64 + l) * 4 + j] = src[512 + 16 * t + 4 * j + q];
+                dst[F::C2 + (t * 64 + l) * 4 + j] = src[1568 + 16 * t + 4 * q + j];
+            }
 }
 
 uint16_t bf16_bits(float v) { // round to nearest even
@@ -86,8 +113,11 @@ void repack(int kind, float const* src, float* dst) {
         for(int o = 0; o < 32; ++o)
             for(int i = 0; i < 32; ++i) dst[544 + i * 32 + o] = src[544 + o * 32 + i];
         std::memcpy(dst + 1568, src + 1568, 65 * sizeof(float));
-    } else
-        std::memcpy(dst, src, static_cast<size_t>(P) * sizeof(float));
+        for(uint32_t i = static_cast<uint32_t>(P); i < ta3h::f16f::DENSE; ++i) dst[i] = 0.0f;
+        repack_f16_fragments(src, dst);
+        return;
+    }
+    std::memcpy(dst, src, static_cast<size_t>(P) * sizeof(float));
     for(uint32_t i = static_cast<uint32_t>(P); i < stride_of(kind); ++i) dst[i] = 0.0f;
 }
 
