@@ -15,7 +15,9 @@ holds the fastest learned setting within the <=1% mean-score bar (else the one w
 `pruned.narrow_best` the same for narrower beams, `pruned.champion` the same study on a trained 16-32-32-1
 champion (its own pruner from its own games on training seeds, judged on 1,000 held-out games). `roofline` prices
 the dominant kernel's algorithmic integer/FP32 VALU work (exact per-item counts of the candidates the kernel fits
-and scores x a fixed per-candidate op model) against the MI355X VALU peak. The kernel scores each repeated list
+and scores x a fixed per-candidate op model) against the MI355X VALU peak; for 16-32-32-1 the hidden layers'
+multiply-adds run on FP32 MFMA and are priced there (`mfma_fp32`), and the bound is the pipe with the longer
+ideal time. The kernel scores each repeated list
 once (a list whose parent board, clear histogram, piece and held-is-I equal an earlier list's at the same level:
 the reference scores it again) and takes lists the previous move scored (its roots, and for 16-32-32-1 lists of
 levels 1 and 2) from that move's scores, so these counts are the reference's less its repeats and those lists;
@@ -57,6 +59,10 @@ C_FEAT = {0: 470, 1: 611}    # fused column metrics + normalisation (V11 / 16-fe
 # kernel skips exactly when an input is zero in every lane of a chunk)
 C_MLP = {0: 184, 1: 1632}
 VALU_PEAK_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz (non-packed 32-bit VALU ops)
+# the plain 16-32-32-1 kernel runs the two hidden layers (16 x 32 + 32 x 32 multiply-adds of C_MLP) on
+# v_mfma_f32_16x16x4_f32 for finite weights (every model bench.py samples); ReLU and the output layer stay on the VALU
+F16_MFMA_MACS = 16 * 32 + 32 * 32
+FP32_MFMA_PEAK_TFLOPS = 157.3  # dense FP32 MFMA
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 F32W_FLOPS = 2 * 20865      # per leaf evaluation of the wide network (on MFMA)
 PRUNE_EPOCHS = 16
@@ -302,13 +308,24 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic_from_profiles("traffic_f32w_wide"),
                 "flops_per_launch": ops}
-    else:
-        ops = applies * C_FIT + legal * (C_PLACE_CLEAR[kind] + C_FEAT[kind] + C_MLP[kind])
+    else:  # VALU lane-ops; for 16-32-32-1 the hidden layers' multiply-adds are priced on the FP32 MFMA peak
+        macs = legal * F16_MFMA_MACS if kind == T.MODEL_F16 else 0
+        ops = applies * C_FIT + legal * (C_PLACE_CLEAR[kind] + C_FEAT[kind] + C_MLP[kind]) - macs
         achieved = ops / (kms / 1000.0) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                 "frac": achieved / VALU_PEAK_TOPS,
                 "traffic": traffic_from_profiles(f"traffic_{'f16' if kind == T.MODEL_F16 else 'v11'}_generation"),
                 "ops_per_launch": ops}
+        if macs:  # the two pipes run side by side (MFMA of one wave, VALU of another): the slower ideal is the bound
+            mf = 2 * macs / (kms / 1000.0) / 1e12
+            mfma = {"achieved": mf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": mf / FP32_MFMA_PEAK_TFLOPS, "flops_per_launch": 2 * macs}
+            if mfma["frac"] > roof["frac"]:
+                roof = {"bound": "mfma", **{k: mfma[k] for k in ("achieved", "peak", "unit", "frac")},
+                        "traffic": roof["traffic"], "flops_per_launch": 2 * macs,
+                        "valu": {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "ops_per_launch")}}
+            else:
+                roof["mfma_fp32"] = mfma
     if wide:
         workload_name = ("scaled generation: 256 models x 100 games = 25,600 simulations, depth-4 full beam "
                          "16/32/32, cap 5,000 pieces, garbage rows 0-4, 32-128-128-1 network on BF16 MFMA")

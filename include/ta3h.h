@@ -102,11 +102,11 @@ int ta3h_eval(ta3h_evaluator* ev, const float* weights, uint32_t num_models, uin
  * two-phase budget are the defaults. The search's scratch (work-queue counters, the weights repacked into the
  * network's device layout, parked games, the resume list and the second phase's work ring) lives in one static
  * device array of TA3H_NOALLOC_SCRATCH_BYTES per GPU: 256 B + models x the layout's bytes (V11 448, 16-32-32-1
- * 6,592, wide 42,560, each rounded up to 256 B per section) + the parked record per (model, game) (512 B; 18,592 B
+ * 16,832 with its MFMA operand fragments, wide 42,560, each rounded up to 256 B per section) + the parked record per (model, game) (512 B; 18,592 B
  * for 16-32-32-1, whose record carries the previous move's level-2 and leaf scores) + 4 B per (model, game) + 4 B
  * per model + 8 B + 4 B per (model, game) and turn of the second phase (1 + (max_moves - b) / b turns for the
  * budget b = max_moves / 32 clamped to [64, 512]; 31 at 2,000 moves, 32 at 5,000); a larger batch returns
- * TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1, 2,000 moves: 188.5 MB; 256 x 100 wide, 5,000 moves: 27.4 MB). Calls
+ * TA3H_E_SCRATCH (e.g. 200 x 50 at 16-32-32-1, 2,000 moves: 190.6 MB; 256 x 100 wide, 5,000 moves: 27.4 MB). Calls
  * that overlap on one GPU share that array and race: serialise them (one stream, or events). Returns TA3H_OK,
  * TA3H_E_ARG, TA3H_E_SCRATCH or TA3H_E_HIP (there is no evaluator to hold an error message). */
 int ta3h_eval_noalloc(int kind, const float* d_weights, uint32_t num_models, const uint64_t* d_seeds, uint32_t games,
