@@ -110,6 +110,42 @@ def test_zero_skipping_matches_oracle(kind):
     assert got.tobytes() == want.tobytes() and (pl == want_pl).all(), (got, want)
 
 
+def test_f16_mfma_rounding_corners():
+    """The 4-wave 16-32-32-1 kernel runs the hidden layers of finite-weight models on v_mfma_f32_16x16x4_f32, whose
+    four products per step must be added in k order and rounded like the oracle's fmaf chain. One batch, bit-exact
+    against the oracle, of models built to expose a different order or a flush to zero:
+    - layer-1 weights, biases and layer-2 biases at subnormal scale (both layers' sums below FLT_MIN; output
+      weights near 1e30 bring the differences back to normal range);
+    - layer-2 weights and biases at subnormal scale, likewise;
+    - weights near 1e15 (hidden values near 1e31, no overflow);
+    - alternating +-1e8 and +-1 weights (heavy cancellation: any other summation order rounds differently);
+    - a random model."""
+    kind = T.MODEL_F16
+    P = T.num_params(kind)
+    rng = np.random.default_rng(73)
+    ws = rng.normal(0, 0.5, (5, P)).astype(np.float32)
+    ws[0, 0:544] *= np.float32(1e-39)       # W1, b1
+    ws[0, 1568:1600] *= np.float32(1e-39)   # b2
+    ws[1, 544:1600] *= np.float32(1e-40)    # W2, b2
+    ws[0:2, 1600:1633] *= np.float32(1e30)  # W3, b3
+    ws[2, 0:1600] *= np.float32(1e15)       # W1, b1, W2, b2
+    big = np.where(np.arange(1568 - 544) % 2 == 0, np.float32(1e8), np.float32(1.0))
+    ws[3, 544:1568] *= big                  # W2: alternating scales
+    ws[3, 0:512] *= np.where(np.arange(512) % 3 == 0, np.float32(1e8), np.float32(1.0))
+    assert np.isfinite(ws).all()
+    assert (np.abs(ws[0, 0:544]) < np.finfo(np.float32).tiny).mean() > 0.9  # mostly subnormal
+    seeds = O.expand_seeds(74, 4)
+    say("case: f16 MFMA rounding corners, 5 models x 4 games, cap 150")
+    e = T.ModelEvaluator(model=kind)
+    e.set_launch(waves_per_game=4)
+    e.stage(ws.reshape(-1), 5, seeds, 150)
+    e.launch()
+    got, pl = e.fetch()
+    e.close()
+    want, want_pl = O.eval_host_mt(kind, ws.reshape(-1), 5, seeds, 150)
+    assert got.tobytes() == want.tobytes() and (pl == want_pl).all(), (got, want)
+
+
 def test_work_counters():
     """Per-item counters: candidates fitted and legal equal the oracle's less its repeated lists (an earlier list
     of the level with the same parent board, clear histogram, piece and held-is-I), which the device scores once,
