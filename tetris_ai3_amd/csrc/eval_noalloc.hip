@@ -79,13 +79,13 @@ __global__ void repack_kernel(int kind, float const* __restrict__ src, uint32_t 
         v = wide_word(w, j);
     } else if(kind == MODEL_F16 && j >= f16f::DENSE) { // the MFMA operands of the hidden layers (f16f)
         uint32_t const lane = j < f16f::C1 ? (j - f16f::DENSE) % 64 : ((j - f16f::C1) % 256) / 4;
-        uint32_t const r = lane % 16, k = lane / 16, q = lane / 16;
+        uint32_t const r = lane % 16, q = lane / 16; // row and quarter of the lane (q is k in the A fragments)
         if(j < f16f::A2) {
             uint32_t const g = (j - f16f::A1) / 64, rt = g / 4, s = g % 4;
-            v = __float_as_uint(w[(16 * rt + 4 * (r % 4) + r / 4) * 16 + 4 * s + k]);
+            v = __float_as_uint(w[(16 * rt + 4 * (r % 4) + r / 4) * 16 + 4 * s + q]);
         } else if(j < f16f::C1) {
             uint32_t const g = (j - f16f::A2) / 64, rt2 = g / 8, rt1 = (g / 4) % 2, s = g % 4;
-            v = __float_as_uint(w[544 + (16 * rt2 + r) * 32 + 16 * rt1 + 4 * s + k]);
+            v = __float_as_uint(w[544 + (16 * rt2 + r) * 32 + 16 * rt1 + 4 * s + q]);
         } else if(j < f16f::C2) {
             uint32_t const t = (j - f16f::C1) / 256, jj = (j - f16f::C1) % 4;
             v = __float_as_uint(w[512 + 16 * t + 4 * jj + q]);
