@@ -681,6 +681,9 @@ __device__ inline float value_f16_mfma(float const* __restrict__ frag, wptr w, f
         transpose_rows(__float_as_uint(fin[4 * s]), __float_as_uint(fin[4 * s + 1]), __float_as_uint(fin[4 * s + 2]),
                        __float_as_uint(fin[4 * s + 3]), b1[s]);
     // layer 1: acc1[rt][ct] register j of lane 16 q + n = unit 16 rt + 4 j + q of candidate 16 ct + n
+    // (the wave's issue priority is raised while it feeds the MFMA unit, so the other waves' VALU work fills the
+    // MFMA latency instead of delaying the chain)
+    __builtin_amdgcn_s_setprio(1);
     f32x4 acc1[2][4];
 #pragma unroll
     for(int rt = 0; rt < 2; ++rt) {
@@ -719,6 +722,7 @@ __device__ inline float value_f16_mfma(float const* __restrict__ frag, wptr w, f
                     acc2[rt2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, acc1[rt1][ct][s], acc2[rt2][ct], 0, 0, 0);
             }
     }
+    __builtin_amdgcn_s_setprio(0);
     // outputs back to one candidate per lane (output 16 rt2 + 4 b + i in o[rt2][i][b]), then the output chain
     float out = w[F16_B3];
 #pragma unroll
