@@ -1025,6 +1025,7 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
 #pragma unroll 1
     for(uint32_t cb = 0; cb < 4; cb += CB) {
         u32x4 h[CB][4]; // layer-2 B operands of each tile (bfloat16 pairs): K block kb = layer-1 tiles 2 kb, 2 kb + 1
+        __builtin_amdgcn_s_setprio(1); // the MFMA layers at raised issue priority (the other waves fill their latency)
         {
             bf16x8 x[CB];
 #pragma unroll
@@ -1064,6 +1065,7 @@ __device__ inline float value_f32w(uint8_t const* __restrict__ wb, uint16_t cons
 #pragma unroll
                 for(int j = 0; j < 4; ++j) part[c] = __fmaf_rn(relu_bits(acc[c][j]), w[j], part[c]);
         }
+        __builtin_amdgcn_s_setprio(0);
 #pragma unroll
         for(int c = 0; c < CB; ++c) {
             float p = part[c];
