@@ -60,9 +60,9 @@ int ta3h_set_model(ta3h_evaluator* ev, int kind);
  * device (engine state, counters, move index); the second phase plays only those on to max_moves, so the long
  * games all start together instead of trickling out of the work queue; the second phase takes the games of the
  * models with the most parked games first. 0 = one phase. TA3H_PHASES_AUTO (the default) = max_moves / 32
- * clamped to [64, 512]: 64 at a 2,000-piece cap (a full 16-32-32-1 generation of 200 models x 50 games: 439 ms
- * against 457 ms at 128; V11 285 against 292 ms), 156 at 5,000 (the wide workload: 2,170 against 2,187 ms at
- * 312). Shorter turns keep the games that run to the cap closer together, so fewer end alone. */
+ * clamped to [64, 512]: 64 at a 2,000-piece cap (a full 16-32-32-1 generation of 200 models x 50 games ran 4%
+ * faster than at 128, V11 2%), 156 at 5,000 (the wide workload 1% faster than at 312). Shorter turns keep the
+ * games that run to the cap closer together, so fewer end alone. */
 int ta3h_set_phases(ta3h_evaluator* ev, uint32_t first_phase_moves);
 
 /* Start every game of the following launches with 0-4 garbage rows at the bottom, each full but for one hole,
